@@ -1,0 +1,177 @@
+/*
+ * libripples_amd -- C ABI of the MI355X-native IMM hot path.
+ *
+ * Plain C types only: every entry point returns 0 on success or a negative
+ * RIP_E* code; rip_last_error() returns a thread-local message for the last
+ * failure on the calling thread.  No exception crosses the boundary.
+ *
+ * Object model (reference citations are relative to the pnnl/ripples tree):
+ *   rip_graph  device copy of a transposed CSR with model-specific edge data.
+ *              Replaces make_gpu_graph / gpu_graph (src/gpu/gpu_graph.cc:48-70,
+ *              include/ripples/gpu/gpu_graph.h:47-90) of a GraphBwd.
+ *   rip_pool   HBM-resident store of RRR sets of one (graph, seed) stream.
+ *              Replaces the RRRsets vector (include/ripples/rrr_sets.h:81-105)
+ *              filled by the streaming generator and the device copy made for
+ *              selection (streaming_find_most_influential.h:139-244).
+ * Sample s of a pool is always the same set for the same (graph, seed): the
+ * RNG stream of a sample depends only on (seed, s), never on scheduling.
+ *
+ * Threading: one object is used by one host thread at a time; distinct
+ * objects may be used concurrently.  Each pool owns one HIP stream.
+ */
+#ifndef RIPPLES_AMD_H
+#define RIPPLES_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RIP_OK 0
+#define RIP_EINVAL (-1)   /* bad argument */
+#define RIP_ENOMEM (-2)   /* device or host allocation failed */
+#define RIP_EDEVICE (-3)  /* HIP runtime error */
+#define RIP_ESTATE (-4)   /* object in the wrong state for the call */
+
+/* Diffusion models: independent_cascade_tag / linear_threshold_tag
+ * (include/ripples/diffusion_simulation.h:56-59). */
+#define RIP_IC 0
+#define RIP_LT 1
+
+typedef struct rip_graph rip_graph;
+typedef struct rip_pool rip_pool;
+
+/* Sampler tuning (gpu_batch_size / streaming engine knobs of
+ * include/ripples/imm_configuration.h:64-109).  Zero fields = automatic. */
+typedef struct {
+  uint32_t colors;     /* IC: samples fused per colour group, 1..64 */
+  uint32_t block;      /* threads per workgroup: 256, 512 or 1024 */
+  uint32_t slots;      /* resident colour groups (persistent workgroups) */
+  uint32_t edge_run;   /* IC: consecutive edges per lane, 1..8 */
+} rip_tune;
+
+/* Timing and size record of one IMM run; times in milliseconds.  Mirrors
+ * IMMExecutionRecord (include/ripples/imm_execution_record.h:55-92). */
+typedef struct {
+  uint64_t theta;            /* Theta */
+  uint64_t num_sets;         /* |RR| at the end */
+  uint64_t rrr_elements;     /* total vertices stored in the pool */
+  uint32_t rounds;           /* theta-estimation rounds run */
+  uint32_t num_seeds;
+  double lb;                 /* lower bound LB of OPT */
+  double final_f;            /* coverage fraction of the seeds on RR[0,theta) */
+  double total_ms;           /* Total */
+  double estimation_ms;      /* ThetaEstimation */
+  double generate_ms;        /* GenerateRRRSets (final round) */
+  double select_ms;          /* FindMostInfluentialSet (final) */
+  int64_t theta_prime[64];   /* ThetaPrimeDeltas are the differences */
+  double f[64];              /* coverage fraction per estimation round */
+  double round_generate_ms[64];  /* ThetaEstimationGenerateRRR */
+  double round_select_ms[64];    /* ThetaEstimationMostInfluential */
+} rip_imm_record;
+
+/* ---------------------------------------------------------------- basics */
+const char *rip_last_error(void);
+int rip_version(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int rip_device_count(int *count);
+
+/* ---------------------------------------------------------------- graph */
+/* Device graph from a transposed CSR: row v lists the in-neighbours u of v
+ * (an edge u -> v of the input) with its weight, as GraphBwd stores it
+ * (include/ripples/graph.h:107-130, 364-1086; neighbors() at :616-618).
+ * index: uint64[num_nodes+1] (graph.h index_type), edges: uint32[num_edges],
+ * weights: float[num_edges].  model selects the edge encoding (RIP_IC or
+ * RIP_LT).  The caller keeps ownership of the host arrays. */
+int rip_graph_create(int device, int model, uint32_t num_nodes,
+                     uint64_t num_edges, const uint64_t *index,
+                     const uint32_t *edges, const float *weights,
+                     rip_graph **out);
+int rip_graph_destroy(rip_graph *g);
+int rip_graph_info(const rip_graph *g, uint32_t *num_nodes,
+                   uint64_t *num_edges, int *model);
+
+/* ---------------------------------------------------------------- pool */
+/* Empty RRR pool bound to graph g and the sampling stream of `seed`
+ * (lcg64.seed(seed).split(2,1), tools/imm.cc:212-214). */
+int rip_pool_create(rip_graph *g, uint64_t seed, rip_pool **out);
+int rip_pool_destroy(rip_pool *p);
+/* Append the RRR sets of samples [size, size + count) -- the generator
+ * call GenerateRRRSets(G, gen, begin, end, ...) of
+ * include/ripples/generate_rrr_sets.h:217-224 /
+ * StreamingRRRGenerator::generate (streaming_rrr_generator.h:1026-1146). */
+int rip_pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune);
+/* Append `count` host-side sets (CSR: offsets uint64[count+1] with
+ * offsets[0] = 0, vertices uint32[offsets[count]]); each set must be sorted,
+ * duplicate-free and hold ids < num_nodes.  The copy of an existing pool to
+ * the device -- LoadDataToDevice / MoveRRRSets
+ * (streaming_find_most_influential.h:139-244). */
+int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
+                         const uint32_t *vertices);
+/* Forget all sets (keeps the allocations). */
+int rip_pool_reset(rip_pool *p);
+int rip_pool_size(const rip_pool *p, uint64_t *num_sets,
+                  uint64_t *num_elements);
+/* Copy sets [first, first+count) to host memory as sorted uint32 lists in
+ * CSR form: offsets uint64[count+1] (offsets[0] = 0), vertices uint32[...].
+ * With vertices == NULL only offsets are written (a size query).  cap is the
+ * capacity of vertices in elements. */
+int rip_pool_export(rip_pool *p, uint64_t first, uint64_t count,
+                    uint64_t *offsets, uint32_t *vertices, uint64_t cap);
+/* Greedy max coverage over sets [0, num_sets) -- FindMostInfluentialSet
+ * (include/ripples/find_most_influential.h:85-196,
+ * streaming_find_most_influential.h:607-643).  Ties go to the lowest vertex
+ * id, the rule of GPUMaxElement (src/gpu/find_most_influential.cc:65-72).
+ * Stops after k seeds or when every set is covered.  seeds: uint32[k]. */
+int rip_pool_select(rip_pool *p, uint64_t num_sets, uint32_t k,
+                    uint32_t *seeds, uint32_t *num_seeds, uint64_t *covered);
+/* Occurrence counts of every vertex over sets [0, num_sets) --
+ * CountOccurrencies (include/ripples/counting.h:65-97). counts: uint32[n]. */
+int rip_pool_count(rip_pool *p, uint64_t num_sets, uint32_t *counts);
+/* Device time of the last rip_pool_sample call's sampling kernels, measured
+ * with HIP events on the pool's stream, and the number of kernel launches. */
+int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
+/* Algorithmic bytes the last LT sampling launch read and wrote:
+ * sum over walks of 8 per visited vertex (its CSR offsets), 8 per scanned
+ * in-edge (id + weight) and 4 per output id.  0 for IC pools, whose
+ * per-launch figure depends on the union of the sets (computed by callers). */
+int rip_pool_last_stats(const rip_pool *p, uint64_t *algo_bytes);
+
+/* ---------------------------------------------------------------- IMM */
+/* The IMM algorithm -- IMM(G, CFG, l=1, gen, record, model, omp) of
+ * include/ripples/imm.h:768-860 with Sampling at :145-260: theta estimation
+ * by doubling, the final theta, the final sampling and selection.
+ * seeds: uint32[k]. */
+int rip_imm(rip_graph *g, uint32_t k, double epsilon, uint64_t seed,
+            const rip_tune *tune, uint32_t *seeds, rip_imm_record *rec);
+
+/* ------------------------------------------------ synthetic input (host) */
+/* Directed R-MAT edge list with 2^scale vertices and edge_factor * 2^scale
+ * draws, self loops and duplicates removed, sorted by (src, dst) -- the
+ * generator the reference benchmarks use
+ * (test/benchmarks/generate_rr_sets.cc:208-210).  With src == NULL only
+ * *num_edges is computed.  Deterministic in `seed`. */
+int rip_rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double b,
+                   double c, uint64_t seed, uint64_t *num_edges,
+                   uint32_t *src, uint32_t *dst, uint64_t cap);
+
+/* Weight schemes for rip_csr_from_edges */
+#define RIP_W_UNIFORM 0   /* w = scale * u01, drawn per edge in input order */
+#define RIP_W_CASCADE 1   /* w = 1 / indeg(v) (weighted cascade) */
+#define RIP_W_LT_NORM 2   /* u01 per edge, normalised per target to `scale` */
+#define RIP_W_CONST 3     /* w = scale */
+/* Transposed CSR of an edge list: index uint64[n+1], edges uint32[m] (the
+ * sources), weights float[m]; rows sorted by source.  Uniform draws come from
+ * lcg64.seed(seed).split(2,0) in input-edge order (tools/imm.cc:172-174,
+ * include/ripples/loaders.h:138-167). */
+int rip_csr_from_edges(uint32_t num_nodes, uint64_t num_edges,
+                       const uint32_t *src, const uint32_t *dst, int scheme,
+                       float scale, uint64_t seed, uint64_t *index,
+                       uint32_t *edges, float *weights);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,547 @@
+/*
+ * CPU oracle for the IMM hot path -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ * Only the tests, the smoke check and the benchmark's CPU-baseline leg may
+ * load this library.  Nothing in the product links it.
+ *
+ * What pins each function (the tests under tests/ carry the checks):
+ *   orc_lcg64_* / orc_master_output / orc_sample_element
+ *       algebra: jump(a)+jump(b) == jump(a+b); split leapfrog == parent's
+ *       outputs; square-and-multiply == stepping; split(s,i) rejects i >= s.
+ *       Equality with TRNG 4.25's own output: parity unpinned (TRNG is not
+ *       vendored in the reference and no reference test checks an RNG value).
+ *   orc_sample (IC)   w = 0 -> {root}; w = 1 -> reverse reachability (scipy
+ *       BFS); three independently written traversals (FIFO BFS, 64-colour
+ *       sweep, per-coin definition) agree; exhaustive live-edge enumeration
+ *       on tiny graphs matches the inclusion frequencies.
+ *   orc_sample (LT)   single in-edge of weight 1 -> deterministic path;
+ *       per-coin definition agrees; walk-step distribution matches weights.
+ *   orc_select        known-answer fixture of test/cuda_find_most_influential.cc
+ *       and brute-force greedy on tiny pools.
+ *   orc_theta*, orc_log_binomial   log(C(n,k)) against math.comb; closed-form
+ *       values recomputed with Python's math module in the tests.
+ *   orc_imm           round structure checked against the closed forms and
+ *       against orc_select/orc_sample run step by step.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* affine maps x -> a*x + b (mod 2^64): the LCG step and all its jumps       */
+/* ------------------------------------------------------------------------ */
+typedef struct { uint64_t a, b; } aff;
+
+static aff aff_then(aff first, aff second) {
+  /* second(first(x)) */
+  aff r;
+  r.a = second.a * first.a;
+  r.b = second.a * first.b + second.b;
+  return r;
+}
+
+static aff aff_pow(aff f, uint64_t n) {
+  aff r = {1, 0};
+  aff p = f;
+  while (n) {
+    if (n & 1) r = aff_then(r, p);
+    p = aff_then(p, p);
+    n >>= 1;
+  }
+  return r;
+}
+
+static uint64_t aff_apply(aff f, uint64_t x) { return f.a * x + f.b; }
+
+static const aff MASTER = {ORC_LCG_A, ORC_LCG_B};
+
+/* ---- lcg64 object, following TRNG's documented semantics ---- */
+void orc_lcg64_seed(orc_lcg64 *g, uint64_t s) {
+  g->a = ORC_LCG_A;
+  g->b = ORC_LCG_B;
+  g->r = s;
+}
+
+uint64_t orc_lcg64_next(orc_lcg64 *g) {
+  g->r = g->a * g->r + g->b;
+  return g->r;
+}
+
+void orc_lcg64_jump(orc_lcg64 *g, uint64_t steps) {
+  aff f = {g->a, g->b};
+  g->r = aff_apply(aff_pow(f, steps), g->r);
+}
+
+static uint64_t inv_odd(uint64_t a) {
+  uint64_t x = a; /* correct to 3 bits for odd a; Newton doubles the bits */
+  for (int i = 0; i < 6; ++i) x *= 2 - a * x;
+  return x;
+}
+
+/* Leapfrog split: afterwards the generator yields the parent's outputs
+ * i, i+s, i+2s, ... (TRNG lcg64::split(s, i)). */
+int orc_lcg64_split(orc_lcg64 *g, uint64_t s, uint64_t i) {
+  if (s < 1 || i >= s) return -1;
+  if (s == 1) return 0;
+  aff f = {g->a, g->b};
+  orc_lcg64_jump(g, i + 1); /* r = parent output i */
+  aff fs = aff_pow(f, s);
+  g->a = fs.a;
+  g->b = fs.b;
+  /* step back once with the new parameters so the next output is r */
+  g->r = (g->r - g->b) * inv_odd(g->a);
+  return 0;
+}
+
+uint64_t orc_master_output(uint64_t seed, uint64_t t) {
+  return aff_apply(aff_pow(MASTER, t + 1), seed);
+}
+
+float orc_u01(uint64_t x) {
+  return (float)((x >> 40) + 1) * 0x1p-24f;
+}
+
+uint32_t orc_uniform_index(uint64_t x, uint32_t n) {
+  return (uint32_t)(((x >> 32) * (uint64_t)n) >> 32);
+}
+
+uint64_t orc_sample_element(uint64_t seed, uint64_t s, uint64_t j) {
+  /* master output index 1 + 2(s + j S); exponents wrap mod 2^64 because the
+   * full-period LCG satisfies M^(2^64) = identity. */
+  uint64_t t = 1 + 2 * (s + j * ORC_SAMPLE_STRIDE);
+  return orc_master_output(seed, t);
+}
+
+uint32_t orc_sample_root(uint64_t seed, uint64_t s, uint32_t n) {
+  return orc_uniform_index(orc_sample_element(seed, s, 0), n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* fast evaluation of the same stream: per-vertex jump + stepping            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  aff P;         /* one step of a sample stream: M^(2S) */
+  aff *vjump;    /* vjump[v] = P^(1 + index[v]) */
+} stream_ctx;
+
+static void stream_ctx_init(stream_ctx *c, uint32_t n, const uint64_t *index) {
+  c->P = aff_pow(MASTER, 2 * ORC_SAMPLE_STRIDE);
+  c->vjump = (aff *)malloc(sizeof(aff) * (size_t)(n ? n : 1));
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < (int64_t)n; ++v)
+    c->vjump[v] = aff_pow(c->P, 1 + index[v]);
+}
+
+static void stream_ctx_free(stream_ctx *c) { free(c->vjump); }
+
+static uint64_t sample_z(uint64_t seed, uint64_t s) {
+  return aff_apply(aff_pow(MASTER, 2 * s + 2), seed);
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-thread output arenas                                                  */
+/* ------------------------------------------------------------------------ */
+struct orc_sets {
+  uint64_t count;
+  uint64_t total;
+  uint64_t *offsets; /* count + 1 */
+  uint32_t *data;
+};
+
+typedef struct {
+  uint32_t *buf;
+  uint64_t len, cap;
+} arena;
+
+static void arena_push(arena *a, uint32_t v) {
+  if (a->len == a->cap) {
+    a->cap = a->cap ? 2 * a->cap : 4096;
+    a->buf = (uint32_t *)realloc(a->buf, sizeof(uint32_t) * a->cap);
+  }
+  a->buf[a->len++] = v;
+}
+
+static int cmp_u32(const void *x, const void *y) {
+  uint32_t a = *(const uint32_t *)x, b = *(const uint32_t *)y;
+  return (a > b) - (a < b);
+}
+
+/* ---- one reverse BFS (IC) or walk (LT) per sample: add_rrrset.h:105-157 ---- */
+static void rrr_one(int model, uint32_t n, const uint64_t *index,
+                    const uint32_t *src, const float *w, const stream_ctx *c,
+                    uint64_t seed, uint64_t s, uint8_t *visited, arena *out,
+                    int definitional) {
+  uint64_t start = out->len;
+  uint64_t z = sample_z(seed, s);
+  uint32_t root = orc_uniform_index(z, n);
+  arena_push(out, root);
+  visited[root] = 1;
+  if (model == ORC_IC) {
+    for (uint64_t head = start; head < out->len; ++head) {
+      uint32_t v = out->buf[head];
+      uint64_t x = aff_apply(c->vjump[v], z);
+      for (uint64_t e = index[v]; e < index[v + 1]; ++e) {
+        uint64_t coin = definitional ? orc_sample_element(seed, s, 1 + e) : x;
+        uint32_t u = src[e];
+        if (!visited[u] && orc_u01(coin) <= w[e]) {
+          visited[u] = 1;
+          arena_push(out, u);
+        }
+        x = aff_apply(c->P, x);
+      }
+    }
+  } else {
+    uint32_t v = root;
+    for (;;) {
+      uint64_t x = definitional ? orc_sample_element(seed, s, 1 + index[v])
+                                : aff_apply(c->vjump[v], z);
+      float threshold = orc_u01(x);
+      int64_t next = -1;
+      for (uint64_t e = index[v]; e < index[v + 1]; ++e) {
+        threshold -= w[e];
+        if (threshold > 0) continue;
+        uint32_t u = src[e];
+        if (!visited[u]) {
+          visited[u] = 1;
+          arena_push(out, u);
+          next = u;
+        }
+        break;
+      }
+      if (next < 0) break;
+      v = (uint32_t)next;
+    }
+  }
+  for (uint64_t i = start; i < out->len; ++i) visited[out->buf[i]] = 0;
+  qsort(out->buf + start, out->len - start, sizeof(uint32_t), cmp_u32);
+}
+
+/* ---- 64-colour sweep, as BatchedBFSNeighborColor (batched_add_rrrset.h:184-257):
+ * full-vertex sweeps until no mask changes; colour i of the batch is bit 63-i. */
+static void rrr_batched64(uint32_t n, const uint64_t *index, const uint32_t *src,
+                          const float *w, const stream_ctx *c, uint64_t seed,
+                          uint64_t s0, uint32_t nb, uint64_t *oldm,
+                          uint64_t *newm, arena *out, uint64_t *sizes) {
+  uint64_t z[64];
+  memset(oldm, 0, sizeof(uint64_t) * n);
+  memset(newm, 0, sizeof(uint64_t) * n);
+  for (uint32_t i = 0; i < nb; ++i) {
+    z[i] = sample_z(seed, s0 + i);
+    newm[orc_uniform_index(z[i], n)] |= 1ull << (63 - i);
+  }
+  int found = 1;
+  while (found) {
+    found = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      if (oldm[v] == newm[v]) continue;
+      found = 1;
+      uint64_t colors = newm[v] ^ oldm[v];
+      oldm[v] |= newm[v];
+      int ids[64];
+      uint64_t xs[64];
+      int nc = 0;
+      while (colors) {
+        int i = __builtin_clzll(colors); /* bit 63-i <-> colour i */
+        colors &= ~(1ull << (63 - i));
+        ids[nc] = i;
+        xs[nc] = aff_apply(c->vjump[v], z[i]);
+        ++nc;
+      }
+      for (uint64_t e = index[v]; e < index[v + 1]; ++e) {
+        uint32_t u = src[e];
+        uint64_t have = newm[u], add = 0;
+        for (int j = 0; j < nc; ++j) {
+          uint64_t bit = 1ull << (63 - ids[j]);
+          if (!(have & bit) && orc_u01(xs[j]) <= w[e]) add |= bit;
+          xs[j] = aff_apply(c->P, xs[j]);
+        }
+        newm[u] |= add;
+      }
+    }
+  }
+  /* decode: per colour, vertices in ascending order */
+  for (uint32_t i = 0; i < nb; ++i) sizes[i] = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    uint64_t m = newm[v];
+    while (m) {
+      int i = __builtin_clzll(m);
+      m &= ~(1ull << (63 - i));
+      ++sizes[i];
+    }
+  }
+  uint64_t base = out->len, acc = 0;
+  uint64_t pos[64];
+  for (uint32_t i = 0; i < nb; ++i) {
+    pos[i] = base + acc;
+    acc += sizes[i];
+  }
+  for (uint64_t i = 0; i < acc; ++i) arena_push(out, 0);
+  for (uint32_t v = 0; v < n; ++v) {
+    uint64_t m = newm[v];
+    while (m) {
+      int i = __builtin_clzll(m);
+      m &= ~(1ull << (63 - i));
+      out->buf[pos[i]++] = v;
+    }
+  }
+}
+
+orc_sets *orc_sample(int model, int algo, uint32_t n, const uint64_t *index,
+                     const uint32_t *src, const float *w, uint64_t seed,
+                     uint64_t first, uint64_t count, int nthreads) {
+  if (n == 0 || (algo == 1 && model != ORC_IC)) return NULL;
+  if (nthreads < 1) nthreads = 1;
+  stream_ctx c;
+  stream_ctx_init(&c, n, index);
+  /* record which arena holds each sample */
+  uint32_t *owner = (uint32_t *)malloc(sizeof(uint32_t) * (count ? count : 1));
+  uint64_t *where = (uint64_t *)malloc(sizeof(uint64_t) * (count ? count : 1));
+  uint64_t *size = (uint64_t *)malloc(sizeof(uint64_t) * (count ? count : 1));
+  arena *arenas = (arena *)calloc((size_t)nthreads, sizeof(arena));
+  uint64_t nbatches = (count + 63) / 64;
+
+#pragma omp parallel num_threads(nthreads)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    arena *a = &arenas[tid];
+    if (algo == 1) {
+      uint64_t *oldm = (uint64_t *)malloc(sizeof(uint64_t) * n);
+      uint64_t *newm = (uint64_t *)malloc(sizeof(uint64_t) * n);
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t b = 0; b < (int64_t)nbatches; ++b) {
+        uint64_t s0 = (uint64_t)b * 64;
+        uint32_t nb = (uint32_t)((count - s0) < 64 ? (count - s0) : 64);
+        uint64_t sizes[64];
+        uint64_t base = a->len;
+        rrr_batched64(n, index, src, w, &c, seed, first + s0, nb, oldm, newm, a,
+                      sizes);
+        for (uint32_t i = 0; i < nb; ++i) {
+          owner[s0 + i] = (uint32_t)tid;
+          where[s0 + i] = base;
+          size[s0 + i] = sizes[i];
+          base += sizes[i];
+        }
+      }
+      free(oldm);
+      free(newm);
+    } else {
+      uint8_t *visited = (uint8_t *)calloc(n, 1);
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t i = 0; i < (int64_t)count; ++i) {
+        uint64_t base = a->len;
+        rrr_one(model, n, index, src, w, &c, seed, first + (uint64_t)i,
+                visited, a, algo == 2);
+        owner[i] = (uint32_t)tid;
+        where[i] = base;
+        size[i] = a->len - base;
+      }
+      free(visited);
+    }
+  }
+
+  orc_sets *h = (orc_sets *)calloc(1, sizeof(orc_sets));
+  h->count = count;
+  h->offsets = (uint64_t *)malloc(sizeof(uint64_t) * (count + 1));
+  h->offsets[0] = 0;
+  for (uint64_t i = 0; i < count; ++i) h->offsets[i + 1] = h->offsets[i] + size[i];
+  h->total = h->offsets[count];
+  h->data = (uint32_t *)malloc(sizeof(uint32_t) * (h->total ? h->total : 1));
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int64_t i = 0; i < (int64_t)count; ++i)
+    memcpy(h->data + h->offsets[i], arenas[owner[i]].buf + where[i],
+           sizeof(uint32_t) * size[i]);
+  for (int t = 0; t < nthreads; ++t) free(arenas[t].buf);
+  free(arenas);
+  free(owner);
+  free(where);
+  free(size);
+  stream_ctx_free(&c);
+  return h;
+}
+
+uint64_t orc_sets_count(const orc_sets *h) { return h ? h->count : 0; }
+uint64_t orc_sets_total(const orc_sets *h) { return h ? h->total : 0; }
+
+void orc_sets_export(const orc_sets *h, uint64_t *offsets, uint32_t *vertices) {
+  memcpy(offsets, h->offsets, sizeof(uint64_t) * (h->count + 1));
+  memcpy(vertices, h->data, sizeof(uint32_t) * h->total);
+}
+
+void orc_sets_free(orc_sets *h) {
+  if (!h) return;
+  free(h->offsets);
+  free(h->data);
+  free(h);
+}
+
+/* ------------------------------------------------------------------------ */
+/* greedy max coverage (find_most_influential.h:85-174, lowest id on ties as  */
+/* src/gpu/find_most_influential.cc:65-72)                                    */
+/* ------------------------------------------------------------------------ */
+void orc_count(uint32_t n, uint64_t nsets, const uint64_t *offsets,
+               const uint32_t *vertices, uint32_t *counts) {
+  memset(counts, 0, sizeof(uint32_t) * n);
+  for (uint64_t i = 0; i < offsets[nsets]; ++i) counts[vertices[i]]++;
+}
+
+uint32_t orc_select(uint32_t n, uint64_t nsets, const uint64_t *offsets,
+                    const uint32_t *vertices, uint32_t k, uint32_t *seeds,
+                    uint64_t *covered) {
+  uint32_t *counts = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  orc_count(n, nsets, offsets, vertices, counts);
+  /* inverted index vertex -> sets by counting sort */
+  uint64_t *start = (uint64_t *)malloc(sizeof(uint64_t) * ((size_t)n + 1));
+  start[0] = 0;
+  for (uint32_t v = 0; v < n; ++v) start[v + 1] = start[v] + counts[v];
+  uint64_t total = start[n];
+  uint64_t *inv = (uint64_t *)malloc(sizeof(uint64_t) * (total ? total : 1));
+  uint64_t *cur = (uint64_t *)malloc(sizeof(uint64_t) * ((size_t)n + 1));
+  memcpy(cur, start, sizeof(uint64_t) * ((size_t)n + 1));
+  for (uint64_t s = 0; s < nsets; ++s)
+    for (uint64_t i = offsets[s]; i < offsets[s + 1]; ++i)
+      inv[cur[vertices[i]]++] = s;
+  uint8_t *dead = (uint8_t *)calloc(nsets ? nsets : 1, 1);
+
+  uint64_t uncovered = nsets;
+  uint32_t got = 0;
+  while (got < k && uncovered != 0) {
+    uint32_t best = 0;
+    for (uint32_t v = 1; v < n; ++v)
+      if (counts[v] > counts[best]) best = v;
+    if (counts[best] == 0) break; /* only empty sets remain */
+    uncovered -= counts[best];
+    seeds[got++] = best;
+    for (uint64_t p = start[best]; p < start[best + 1]; ++p) {
+      uint64_t s = inv[p];
+      if (dead[s]) continue;
+      dead[s] = 1;
+      for (uint64_t i = offsets[s]; i < offsets[s + 1]; ++i) counts[vertices[i]]--;
+    }
+  }
+  if (covered) *covered = nsets - uncovered;
+  free(counts);
+  free(start);
+  free(inv);
+  free(cur);
+  free(dead);
+  return got;
+}
+
+/* ------------------------------------------------------------------------ */
+/* IMM closed forms                                                          */
+/* ------------------------------------------------------------------------ */
+double orc_log_binomial(uint64_t n, uint64_t k) {
+  return lgamma((double)(n + 1)) - lgamma((double)(n - k + 1)) -
+         lgamma((double)(k + 1));
+}
+
+int64_t orc_theta_prime(int64_t x, double eps_prime, double l, uint64_t k,
+                        uint64_t n) {
+  if (k > n / 2) k = n / 2;
+  return (int64_t)((2 + 2. / 3. * eps_prime) *
+                   (l * log((double)n) + orc_log_binomial(n, k) +
+                    log(log2((double)n))) *
+                   pow(2.0, (double)x) / (eps_prime * eps_prime));
+}
+
+uint64_t orc_theta(double eps, double l, uint64_t k, double lb, uint64_t n) {
+  if (lb == 0) return 0;
+  if (k > n / 2) k = n / 2;
+  double term1 = 0.6321205588285577;
+  double alpha = sqrt(l * log((double)n) + log(2));
+  double beta = sqrt(term1 * (orc_log_binomial(n, k) + l * log((double)n) + log(2)));
+  double lamda = (double)(2 * n) * (term1 * alpha + beta) * (term1 * alpha + beta) *
+                 pow(eps, -2);
+  return (uint64_t)(lamda / lb);
+}
+
+/* ------------------------------------------------------------------------ */
+/* IMM driver                                                                */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  uint64_t count, total, cap_sets, cap_data;
+  uint64_t *offsets;
+  uint32_t *data;
+} pool;
+
+static void pool_append(pool *p, orc_sets *h) {
+  if (p->count + h->count + 1 > p->cap_sets) {
+    p->cap_sets = 2 * (p->count + h->count + 1);
+    p->offsets = (uint64_t *)realloc(p->offsets, sizeof(uint64_t) * p->cap_sets);
+  }
+  if (p->total + h->total > p->cap_data) {
+    p->cap_data = 2 * (p->total + h->total) + 1;
+    p->data = (uint32_t *)realloc(p->data, sizeof(uint32_t) * p->cap_data);
+  }
+  if (p->count == 0) p->offsets[0] = 0;
+  for (uint64_t i = 0; i < h->count; ++i)
+    p->offsets[p->count + 1 + i] = p->total + h->offsets[i + 1];
+  memcpy(p->data + p->total, h->data, sizeof(uint32_t) * h->total);
+  p->count += h->count;
+  p->total += h->total;
+}
+
+static void pool_grow_to(pool *p, int model, uint32_t n, const uint64_t *index,
+                         const uint32_t *src, const float *w, uint64_t seed,
+                         uint64_t target, int nthreads) {
+  if (target <= p->count) return;
+  orc_sets *h = orc_sample(model, 0, n, index, src, w, seed, p->count,
+                           target - p->count, nthreads);
+  pool_append(p, h);
+  orc_sets_free(h);
+}
+
+uint32_t orc_imm(int model, uint32_t n, const uint64_t *index,
+                 const uint32_t *src, const float *w, uint32_t k, double eps,
+                 uint64_t seed, int nthreads, uint32_t *seeds,
+                 orc_imm_record *rec) {
+  pool p;
+  memset(&p, 0, sizeof(p));
+  p.cap_sets = 1;
+  p.offsets = (uint64_t *)malloc(sizeof(uint64_t));
+  p.offsets[0] = 0;
+  orc_imm_record r;
+  memset(&r, 0, sizeof(r));
+  uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (k ? k : 1));
+
+  double l = 1.0 * (1 + 1 / log2((double)n));
+  double eps_prime = 1.4142135623730951 * eps;
+  double lb = 0;
+  for (int64_t x = 1; x < log2((double)n); ++x) {
+    int64_t tp = orc_theta_prime(x, eps_prime, l, k, n);
+    if (tp <= (int64_t)p.count) tp = (int64_t)p.count;
+    else pool_grow_to(&p, model, n, index, src, w, seed, (uint64_t)tp, nthreads);
+    uint64_t cov = 0;
+    orc_select(n, p.count, p.offsets, p.data, k, tmp, &cov);
+    double f = (double)cov / (double)p.count;
+    if (r.rounds < 64) {
+      r.theta_prime[r.rounds] = tp;
+      r.f[r.rounds] = f;
+    }
+    r.rounds++;
+    if (f >= pow(2, (double)-x)) {
+      lb = ((double)n * f) / (1 + eps_prime);
+      break;
+    }
+  }
+  uint64_t theta = orc_theta(eps, l, k, lb, n);
+  pool_grow_to(&p, model, n, index, src, w, seed, theta, nthreads);
+  uint64_t cov = 0;
+  uint32_t got = orc_select(n, theta, p.offsets, p.data, k, seeds, &cov);
+  r.theta = theta;
+  r.num_sets = p.count;
+  r.lb = lb;
+  r.final_f = theta ? (double)cov / (double)theta : 0.0;
+  if (rec) *rec = r;
+  free(tmp);
+  free(p.offsets);
+  free(p.data);
+  return got;
+}

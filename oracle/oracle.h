@@ -1,0 +1,113 @@
+/*
+ * CPU oracle for the IMM hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library restates, in plain C, what the reference computes on the IMM
+ * path (RRR-set sampling under IC and LT, greedy max-coverage seed selection,
+ * and the IMM theta-estimation driver) under the same per-sample stream
+ * contract the HIP implementation uses.  It is the checker: only the tests,
+ * the smoke check and the CPU-baseline leg of the benchmark may load it.  The
+ * product (libripples_amd.so) never links, loads or calls it.
+ *
+ * Stream contract (shared with the device code, written out here so the two
+ * can be compared):
+ *   - master generator: TRNG lcg64 recurrence r <- a*r + b (mod 2^64) with
+ *     a = 18145460002477866997, b = 1, seeded with r = seed; the t-th output
+ *     (t = 0, 1, ...) is x_t = M^(t+1)(seed) where M is the affine step.
+ *   - split(s, i) is TRNG's leapfrog split: the i-th of s sub-streams yields
+ *     the parent's outputs i, i+s, i+2s, ...
+ *   - weights stream  = master.split(2, 0)   (tools/imm.cc:172-174)
+ *     sampling stream = master.split(2, 1)   (tools/imm.cc:212-214)
+ *   - sample s draws from sampling.split(S, s) with S = 2^32 - 1:
+ *       element 0      -> the root,  root = (hi32(x) * n) >> 32
+ *       element 1 + e  -> IC coin of transposed-CSR edge e
+ *       element 1 + index[v] -> LT threshold of vertex v
+ *     so element j of sample s is master output 1 + 2*(s + j*S).
+ *   - uniform01: u = ((x >> 40) + 1) * 2^-24, a float in (0, 1].
+ *     An IC edge is live when u <= w (add_rrrset.h:130).
+ *
+ * Parity status of each function is listed in the header of oracle.c.
+ */
+#ifndef RIPPLES_ORACLE_H
+#define RIPPLES_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_LCG_A 18145460002477866997ull
+#define ORC_LCG_B 1ull
+#define ORC_SAMPLE_STRIDE 4294967295ull /* S = 2^32 - 1 */
+
+enum { ORC_IC = 0, ORC_LT = 1 };
+
+/* ---- lcg64 restatement (TRNG 4.25 lcg64, published recurrence) ---- */
+typedef struct { uint64_t a, b, r; } orc_lcg64;
+
+void orc_lcg64_seed(orc_lcg64 *g, uint64_t s);
+uint64_t orc_lcg64_next(orc_lcg64 *g);
+void orc_lcg64_jump(orc_lcg64 *g, uint64_t steps);
+int orc_lcg64_split(orc_lcg64 *g, uint64_t s, uint64_t i);
+/* x_t of the master stream seeded with `seed`, by square-and-multiply. */
+uint64_t orc_master_output(uint64_t seed, uint64_t t);
+float orc_u01(uint64_t x);
+uint32_t orc_uniform_index(uint64_t x, uint32_t n);
+
+/* element j of sample s's stream, computed from the definition */
+uint64_t orc_sample_element(uint64_t seed, uint64_t s, uint64_t j);
+uint32_t orc_sample_root(uint64_t seed, uint64_t s, uint32_t n);
+
+/* ---- RRR sampling over a transposed CSR (in-neighbours of each vertex) ----
+ * index: uint64[n+1], src: uint32[m] (in-neighbour ids), w: float[m].
+ * The result handle holds count sorted vertex lists in sample order. */
+typedef struct orc_sets orc_sets;
+
+/* algo: 0 = one FIFO BFS / walk per sample (add_rrrset.h:105-157),
+ *       1 = 64-colour batched sweep (batched_add_rrrset.h:184-257), IC only,
+ *       2 = per-coin evaluation from the stream definition (slow; tiny tests) */
+orc_sets *orc_sample(int model, int algo, uint32_t n, const uint64_t *index,
+                     const uint32_t *src, const float *w, uint64_t seed,
+                     uint64_t first, uint64_t count, int nthreads);
+uint64_t orc_sets_count(const orc_sets *h);
+uint64_t orc_sets_total(const orc_sets *h);
+/* offsets: uint64[count+1]; vertices: uint32[total] */
+void orc_sets_export(const orc_sets *h, uint64_t *offsets, uint32_t *vertices);
+void orc_sets_free(orc_sets *h);
+
+/* ---- greedy max-coverage selection, lowest vertex id on ties ----
+ * Returns the number of seeds written (<= k); *covered = sets covered. */
+uint32_t orc_select(uint32_t n, uint64_t nsets, const uint64_t *offsets,
+                    const uint32_t *vertices, uint32_t k, uint32_t *seeds,
+                    uint64_t *covered);
+/* Initial occurrence counts (counting.h:65-97). counts: uint32[n] */
+void orc_count(uint32_t n, uint64_t nsets, const uint64_t *offsets,
+               const uint32_t *vertices, uint32_t *counts);
+
+/* ---- IMM closed forms (imm.h:77-128, utility.h:84-87) ---- */
+double orc_log_binomial(uint64_t n, uint64_t k);
+int64_t orc_theta_prime(int64_t x, double eps_prime, double l, uint64_t k,
+                        uint64_t n);
+uint64_t orc_theta(double eps, double l, uint64_t k, double lb, uint64_t n);
+
+/* ---- IMM driver (imm.h:145-260, 768-860) ---- */
+typedef struct {
+  uint64_t theta;          /* final theta */
+  uint64_t num_sets;       /* |RR| at the end */
+  uint32_t rounds;         /* estimation rounds run */
+  int64_t theta_prime[64]; /* theta' per round */
+  double f[64];            /* coverage fraction per round */
+  double lb;               /* lower bound LB */
+  double final_f;          /* coverage of the final seeds on RR[0, theta) */
+} orc_imm_record;
+
+uint32_t orc_imm(int model, uint32_t n, const uint64_t *index,
+                 const uint32_t *src, const float *w, uint32_t k, double eps,
+                 uint64_t seed, int nthreads, uint32_t *seeds,
+                 orc_imm_record *rec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

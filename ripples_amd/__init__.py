@@ -1,0 +1,288 @@
+"""ripples_amd: MI355X-native IMM hot path (ctypes binding of libripples_amd.so).
+
+The compute path is the HIP library behind the C ABI declared in
+``include/ripples_amd.h``; this module only marshals numpy arrays across it.
+There is no CPU fallback: if the library is missing, importing the binding
+raises, and every compute call needs a HIP device.
+
+Names follow the reference's IMM interface (include/ripples/imm.h,
+generate_rrr_sets.h, find_most_influential.h).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "IC", "LT", "Tune", "ImmRecord", "Graph", "Pool", "RipplesError", "lib",
+    "device_count", "rmat_edges", "csr_from_edges", "imm",
+    "W_UNIFORM", "W_CASCADE", "W_LT_NORM", "W_CONST",
+]
+
+IC, LT = 0, 1
+W_UNIFORM, W_CASCADE, W_LT_NORM, W_CONST = 0, 1, 2, 3
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libripples_amd.so")
+
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+class RipplesError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class Tune(C.Structure):
+    _fields_ = [("colors", C.c_uint32), ("block", C.c_uint32),
+                ("slots", C.c_uint32), ("edge_run", C.c_uint32)]
+
+
+class ImmRecord(C.Structure):
+    _fields_ = [
+        ("theta", C.c_uint64), ("num_sets", C.c_uint64), ("rrr_elements", C.c_uint64),
+        ("rounds", C.c_uint32), ("num_seeds", C.c_uint32),
+        ("lb", C.c_double), ("final_f", C.c_double),
+        ("total_ms", C.c_double), ("estimation_ms", C.c_double),
+        ("generate_ms", C.c_double), ("select_ms", C.c_double),
+        ("theta_prime", C.c_int64 * 64), ("f", C.c_double * 64),
+        ("round_generate_ms", C.c_double * 64), ("round_select_ms", C.c_double * 64),
+    ]
+
+    def as_dict(self):
+        r = self.rounds
+        return {
+            "Theta": self.theta, "NumSets": self.num_sets,
+            "RRRElements": self.rrr_elements, "Rounds": r,
+            "NumSeeds": self.num_seeds, "LB": self.lb, "FinalF": self.final_f,
+            "Total": self.total_ms, "ThetaEstimation": self.estimation_ms,
+            "GenerateRRRSets": self.generate_ms,
+            "FindMostInfluentialSet": self.select_ms,
+            "ThetaPrime": list(self.theta_prime[:min(r, 64)]),
+            "F": list(self.f[:min(r, 64)]),
+            "ThetaEstimationGenerateRRR": list(self.round_generate_ms[:min(r, 64)]),
+            "ThetaEstimationMostInfluential": list(self.round_select_ms[:min(r, 64)]),
+        }
+
+
+_LIB = None
+
+
+def lib():
+    """Load libripples_amd.so; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libripples_amd.so is not built (%s); run "
+                           "`python -m ripples_amd.build`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.rip_last_error.restype = C.c_char_p
+    L.rip_version.restype = C.c_int
+    L.rip_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.rip_graph_create.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint64, _u64p,
+                                   _u32p, _f32p, C.POINTER(vp)]
+    L.rip_graph_destroy.argtypes = [vp]
+    L.rip_graph_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_int)]
+    L.rip_pool_create.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.rip_pool_destroy.argtypes = [vp]
+    L.rip_pool_sample.argtypes = [vp, C.c_uint64, C.POINTER(Tune)]
+    L.rip_pool_reset.argtypes = [vp]
+    L.rip_pool_append_host.argtypes = [vp, C.c_uint64, _u64p, vp]
+    L.rip_pool_size.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.rip_pool_export.argtypes = [vp, C.c_uint64, C.c_uint64, _u64p, vp, C.c_uint64]
+    L.rip_pool_select.argtypes = [vp, C.c_uint64, C.c_uint32, _u32p,
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
+    L.rip_pool_count.argtypes = [vp, C.c_uint64, _u32p]
+    L.rip_pool_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_uint32)]
+    L.rip_pool_last_stats.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.rip_imm.argtypes = [vp, C.c_uint32, C.c_double, C.c_uint64, C.POINTER(Tune),
+                          _u32p, C.POINTER(ImmRecord)]
+    L.rip_rmat_edges.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                 C.c_double, C.c_uint64, C.POINTER(C.c_uint64), vp, vp,
+                                 C.c_uint64]
+    L.rip_csr_from_edges.argtypes = [C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_int,
+                                     C.c_float, C.c_uint64, _u64p, _u32p, _f32p]
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise RipplesError(rc, lib().rip_last_error().decode(errors="replace"))
+
+
+def _tune(tune):
+    if tune is None:
+        return None
+    if isinstance(tune, Tune):
+        return C.byref(tune)
+    return C.byref(Tune(**tune))
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().rip_device_count(C.byref(n)))
+    return n.value
+
+
+def rmat_edges(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=None):
+    """Directed R-MAT edge list (deduplicated, no self loops, sorted)."""
+    seed = scale if seed is None else seed
+    cap = edge_factor << scale
+    src = np.empty(max(cap, 1), np.uint32)
+    dst = np.empty(max(cap, 1), np.uint32)
+    m = C.c_uint64(0)
+    _check(lib().rip_rmat_edges(scale, edge_factor, a, b, c, seed, C.byref(m),
+                                src.ctypes.data_as(C.c_void_p),
+                                dst.ctypes.data_as(C.c_void_p), cap))
+    return src[:m.value].copy(), dst[:m.value].copy()
+
+
+def csr_from_edges(n, src, dst, scheme=W_UNIFORM, scale=1.0, seed=0):
+    """Transposed CSR (in-neighbour rows) with weights: (index, edges, weights)."""
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    m = len(src)
+    index = np.zeros(n + 1, np.uint64)
+    edges = np.zeros(max(m, 1), np.uint32)
+    weights = np.zeros(max(m, 1), np.float32)
+    _check(lib().rip_csr_from_edges(n, m, src if m else np.zeros(1, np.uint32),
+                                    dst if m else np.zeros(1, np.uint32), scheme,
+                                    scale, seed, index, edges, weights))
+    return index, edges[:m], weights[:m]
+
+
+class Graph:
+    """Device transposed CSR graph (GraphBwd on the GPU)."""
+
+    def __init__(self, index, edges, weights, model=IC, device=0):
+        index = np.ascontiguousarray(index, np.uint64)
+        edges = np.ascontiguousarray(edges, np.uint32)
+        weights = np.ascontiguousarray(weights, np.float32)
+        n = len(index) - 1
+        m = len(edges)
+        h = C.c_void_p()
+        _check(lib().rip_graph_create(device, model, n, m, index,
+                                      edges if m else np.zeros(1, np.uint32),
+                                      weights if m else np.zeros(1, np.float32),
+                                      C.byref(h)))
+        self._h = h
+        self.n, self.m, self.model = n, m, model
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().rip_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pool(self, seed=0):
+        return Pool(self, seed)
+
+
+class Pool:
+    """HBM-resident RRR pool of one (graph, seed) sampling stream."""
+
+    def __init__(self, graph, seed=0):
+        self.graph = graph
+        h = C.c_void_p()
+        _check(lib().rip_pool_create(graph.handle, seed, C.byref(h)))
+        self._h = h
+        self.seed = seed
+
+    def close(self):
+        if self._h:
+            lib().rip_pool_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def generate_rrr_sets(self, count, tune=None):
+        """Append the RRR sets of the next `count` sample ids."""
+        _check(lib().rip_pool_sample(self._h, count, _tune(tune)))
+
+    sample = generate_rrr_sets
+
+    def append(self, offsets, vertices):
+        """Append host-side sorted sets given in CSR form."""
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        vertices = np.ascontiguousarray(vertices, np.uint32)
+        _check(lib().rip_pool_append_host(self._h, len(offsets) - 1, offsets,
+                                          vertices.ctypes.data_as(C.c_void_p)))
+
+    def reset(self):
+        _check(lib().rip_pool_reset(self._h))
+
+    def size(self):
+        s, e = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().rip_pool_size(self._h, C.byref(s), C.byref(e)))
+        return s.value, e.value
+
+    def export(self, first=0, count=None):
+        """Sets [first, first+count) as (offsets u64[count+1], vertices u32)."""
+        if count is None:
+            count = self.size()[0] - first
+        off = np.zeros(count + 1, np.uint64)
+        _check(lib().rip_pool_export(self._h, first, count, off, None, 0))
+        total = int(off[-1])
+        ver = np.zeros(max(total, 1), np.uint32)
+        _check(lib().rip_pool_export(self._h, first, count, off,
+                                     ver.ctypes.data_as(C.c_void_p), len(ver)))
+        return off, ver[:total]
+
+    def find_most_influential_set(self, k, num_sets=None):
+        """Greedy seeds over sets [0, num_sets): (seeds, covered sets)."""
+        if num_sets is None:
+            num_sets = self.size()[0]
+        seeds = np.zeros(max(k, 1), np.uint32)
+        got, cov = C.c_uint32(0), C.c_uint64(0)
+        _check(lib().rip_pool_select(self._h, num_sets, k, seeds, C.byref(got),
+                                     C.byref(cov)))
+        return seeds[:got.value], cov.value
+
+    select = find_most_influential_set
+
+    def count_occurrences(self, num_sets=None):
+        if num_sets is None:
+            num_sets = self.size()[0]
+        out = np.zeros(self.graph.n, np.uint32)
+        _check(lib().rip_pool_count(self._h, num_sets, out))
+        return out
+
+    def last_kernel_ms(self):
+        ms, launches = C.c_double(0), C.c_uint32(0)
+        _check(lib().rip_pool_last_kernel_ms(self._h, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value
+
+    def last_algo_bytes(self):
+        b = C.c_uint64(0)
+        _check(lib().rip_pool_last_stats(self._h, C.byref(b)))
+        return b.value
+
+
+def imm(graph, k, epsilon, seed=0, tune=None):
+    """IMM (imm.h:768-860): returns (seeds, record dict)."""
+    seeds = np.zeros(max(k, 1), np.uint32)
+    rec = ImmRecord()
+    _check(lib().rip_imm(graph.handle, k, epsilon, seed, _tune(tune), seeds,
+                         C.byref(rec)))
+    return seeds[:rec.num_seeds], rec.as_dict()

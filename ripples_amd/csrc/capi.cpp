@@ -1,0 +1,230 @@
+// extern "C" boundary: argument checks, exception -> status code mapping and
+// the thread-local error message.  The reference reports failures by
+// asserts, `throw` and ignored HIP status codes
+// (include/ripples/gpu/gpu_runtime_trait.h:168-243); here every failure is a
+// negative return value with a message.
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace rip {
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return RIP_OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return RIP_ENOMEM;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return RIP_EINVAL;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return RIP_EINVAL;
+  }
+}
+
+}  // namespace rip
+
+using rip::Error;
+using rip::guarded;
+
+extern "C" {
+
+const char *rip_last_error(void) { return rip::g_last_error.c_str(); }
+
+int rip_version(void) { return 1; }
+
+int rip_device_count(int *count) {
+  return guarded([&] {
+    RIP_REQUIRE(count, RIP_EINVAL, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+int rip_graph_create(int device, int model, uint32_t num_nodes,
+                     uint64_t num_edges, const uint64_t *index,
+                     const uint32_t *edges, const float *weights,
+                     rip_graph **out) {
+  return guarded([&] {
+    RIP_REQUIRE(out, RIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    auto *g = new rip_graph();
+    try {
+      rip::graph_create(g, device, model, num_nodes, num_edges, index, edges, weights);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int rip_graph_destroy(rip_graph *g) {
+  return guarded([&] {
+    if (!g) return;
+    rip::graph_destroy(g);
+    delete g;
+  });
+}
+
+int rip_graph_info(const rip_graph *g, uint32_t *num_nodes, uint64_t *num_edges,
+                   int *model) {
+  return guarded([&] {
+    RIP_REQUIRE(g, RIP_EINVAL, "graph is NULL");
+    if (num_nodes) *num_nodes = g->n;
+    if (num_edges) *num_edges = g->m;
+    if (model) *model = g->model;
+  });
+}
+
+int rip_pool_create(rip_graph *g, uint64_t seed, rip_pool **out) {
+  return guarded([&] {
+    RIP_REQUIRE(g && out, RIP_EINVAL, "graph or out is NULL");
+    *out = nullptr;
+    auto *p = new rip_pool();
+    try {
+      rip::pool_create(p, g, seed);
+    } catch (...) {
+      rip::pool_destroy(p);
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+int rip_pool_destroy(rip_pool *p) {
+  return guarded([&] {
+    if (!p) return;
+    rip::pool_destroy(p);
+    delete p;
+  });
+}
+
+int rip_pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    rip::pool_sample(p, count, tune);
+  });
+}
+
+int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
+                         const uint32_t *vertices) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    rip::pool_append_host(p, count, offsets, vertices);
+  });
+}
+
+int rip_pool_reset(rip_pool *p) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    p->num_sets = 0;
+    p->used_data = 0;
+  });
+}
+
+int rip_pool_size(const rip_pool *p, uint64_t *num_sets, uint64_t *num_elements) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    if (num_sets) *num_sets = p->num_sets;
+    if (num_elements) *num_elements = p->used_data;
+  });
+}
+
+int rip_pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
+                    uint32_t *vertices, uint64_t cap) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    rip::pool_export(p, first, count, offsets, vertices, cap);
+  });
+}
+
+int rip_pool_select(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
+                    uint32_t *num_seeds, uint64_t *covered) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    RIP_REQUIRE(k == 0 || seeds, RIP_EINVAL, "seeds is NULL");
+    RIP_REQUIRE(num_sets <= p->num_sets, RIP_EINVAL, "num_sets beyond the pool");
+    uint32_t got = rip::select_run(p, num_sets, k, seeds, covered);
+    if (num_seeds) *num_seeds = got;
+  });
+}
+
+int rip_pool_count(rip_pool *p, uint64_t num_sets, uint32_t *counts) {
+  return guarded([&] {
+    RIP_REQUIRE(p && counts, RIP_EINVAL, "pool or counts is NULL");
+    RIP_REQUIRE(num_sets <= p->num_sets, RIP_EINVAL, "num_sets beyond the pool");
+    rip::count_run(p, num_sets, counts);
+  });
+}
+
+int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    if (ms) *ms = p->last_kernel_ms;
+    if (launches) *launches = p->last_launches;
+  });
+}
+
+int rip_pool_last_stats(const rip_pool *p, uint64_t *algo_bytes) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    if (algo_bytes) *algo_bytes = p->last_algo_bytes;
+  });
+}
+
+int rip_imm(rip_graph *g, uint32_t k, double epsilon, uint64_t seed,
+            const rip_tune *tune, uint32_t *seeds, rip_imm_record *rec) {
+  return guarded([&] {
+    RIP_REQUIRE(g, RIP_EINVAL, "graph is NULL");
+    RIP_REQUIRE(k == 0 || seeds, RIP_EINVAL, "seeds is NULL");
+    rip::imm_run(g, k, epsilon, seed, tune, seeds, rec);
+  });
+}
+
+int rip_rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double b,
+                   double c, uint64_t seed, uint64_t *num_edges, uint32_t *src,
+                   uint32_t *dst, uint64_t cap) {
+  return guarded([&] {
+    RIP_REQUIRE(num_edges, RIP_EINVAL, "num_edges is NULL");
+    std::vector<uint32_t> s, d;
+    rip::rmat_edges(scale, edge_factor, a, b, c, seed, s, d);
+    *num_edges = s.size();
+    if (!src) return;
+    RIP_REQUIRE(dst, RIP_EINVAL, "dst is NULL");
+    RIP_REQUIRE(cap >= s.size(), RIP_EINVAL, "edge buffers too small");
+    std::memcpy(src, s.data(), sizeof(uint32_t) * s.size());
+    std::memcpy(dst, d.data(), sizeof(uint32_t) * d.size());
+  });
+}
+
+int rip_csr_from_edges(uint32_t num_nodes, uint64_t num_edges, const uint32_t *src,
+                       const uint32_t *dst, int scheme, float scale, uint64_t seed,
+                       uint64_t *index, uint32_t *edges, float *weights) {
+  return guarded([&] {
+    RIP_REQUIRE(index && (num_edges == 0 || (src && dst && edges && weights)),
+                RIP_EINVAL, "NULL buffer");
+    rip::csr_from_edges(num_nodes, num_edges, src, dst, scheme, scale, seed, index,
+                        edges, weights);
+  });
+}
+
+}  // extern "C"

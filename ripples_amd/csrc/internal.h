@@ -1,0 +1,166 @@
+// Internal objects behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+#include "ripples_amd.h"
+
+struct rip_graph {
+  int device = 0;
+  int model = RIP_IC;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  uint32_t max_degree = 0;
+  rip::Affine step{1, 0};           // one step of a sample stream
+  rip::VertexRec *d_vrec = nullptr;  // n records
+  uint2 *d_adj = nullptr;            // m x (source, IC threshold | LT weight bits)
+  rip::Affine *d_jtab = nullptr;     // step^j for j in [0, max_degree]
+};
+
+namespace rip {
+
+// Scratch of the fused IC sampler: one slot per resident colour group.
+struct IcSlots {
+  uint32_t slots = 0;
+  uint32_t n = 0;
+  size_t slot_bytes = 0;
+  unsigned long long *d_vp = nullptr;  // slots x n x {visited, pending}
+  uint32_t *d_bm0 = nullptr;           // slots x ceil(n/32)   union bitmap
+  uint32_t *d_bm1 = nullptr;           // slots x ceil(n/1024) bitmap summary
+  uint32_t *d_queue = nullptr;         // slots x 3n words: frontier queues / output staging
+};
+
+struct LtScratch {
+  uint64_t cap_ids = 0;
+  uint32_t *d_long_ids = nullptr;      // walks that outgrew the LDS list
+};
+
+struct SelectScratch {
+  uint32_t n = 0;
+  uint64_t cap_sets = 0, cap_elems = 0;
+  uint32_t *d_counts = nullptr;        // n
+  unsigned long long *d_inv_start = nullptr;  // n + 1
+  unsigned long long *d_cursor = nullptr;     // n
+  uint32_t *d_inv = nullptr;           // total elements: set ids by vertex
+  uint8_t *d_covered = nullptr;        // cap_sets
+  unsigned long long *d_keys = nullptr;  // per-seed argmax keys (k)
+  uint32_t cap_k = 0;
+};
+
+}  // namespace rip
+
+struct rip_pool {
+  rip_graph *g = nullptr;
+  uint64_t seed = 0;
+  hipStream_t stream = nullptr;
+  uint64_t num_sets = 0;
+  uint64_t cap_sets = 0;
+  unsigned long long *d_off = nullptr;  // per-sample start in d_data
+  uint32_t *d_len = nullptr;            // per-sample size
+  uint32_t *d_data = nullptr;
+  uint64_t cap_data = 0;
+  uint64_t used_data = 0;
+  unsigned long long *d_ctl = nullptr;  // control words, see pool.hip
+  rip::IcSlots ic;
+  rip::LtScratch lt;
+  rip::SelectScratch sel;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_kernel_ms = 0.0;
+  uint32_t last_launches = 0;
+  uint64_t last_algo_bytes = 0;
+  double avg_set_size = 8.0;            // running estimate for capacity planning
+};
+
+namespace rip {
+
+// control word indices in rip_pool::d_ctl
+enum : int {
+  kCtlGroupCounter = 0,  // work counter of the persistent samplers
+  kCtlDataBump = 1,      // used elements of d_data
+  kCtlOverflow = 2,      // nonzero when d_data ran out
+  kCtlLongCount = 3,     // LT: walks that outgrew the per-lane list
+  kCtlCovered = 4,       // selector: sets covered so far
+  kCtlStatBytes = 5,     // LT: algorithmic bytes of the last sampler launch
+  kCtlWords = 16,
+};
+
+// graph.hip
+void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
+                  const uint64_t *index, const uint32_t *edges,
+                  const float *weights);
+void graph_destroy(rip_graph *g);
+
+// pool.hip
+void pool_create(rip_pool *p, rip_graph *g, uint64_t seed);
+void pool_destroy(rip_pool *p);
+void pool_reserve_sets(rip_pool *p, uint64_t cap_sets);
+void pool_reserve_data(rip_pool *p, uint64_t cap_data);
+void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune);
+void pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
+                      const uint32_t *vertices);
+void pool_export(rip_pool *p, uint64_t first, uint64_t count,
+                 uint64_t *offsets, uint32_t *vertices, uint64_t cap);
+
+// sample_ic.hip
+struct IcLaunch {
+  uint32_t colors, block, slots, edge_run;
+};
+IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune);
+void ic_ensure_slots(rip_pool *p, uint32_t slots);
+uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
+                   const IcLaunch &cfg);
+
+// sample_lt.hip
+uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
+                   const rip_tune *tune);
+
+// select.hip
+uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k,
+                    uint32_t *seeds, uint64_t *covered);
+void count_run(rip_pool *p, uint64_t num_sets, uint32_t *counts_host);
+
+// scan.hip: exclusive scan of uint32 values into uint64 prefixes (n + 1 out)
+void exclusive_scan_u32(const uint32_t *d_in, unsigned long long *d_out,
+                        uint64_t n, hipStream_t stream);
+
+// imm.cpp
+void imm_run(rip_graph *g, uint32_t k, double eps, uint64_t seed,
+             const rip_tune *tune, uint32_t *seeds, rip_imm_record *rec);
+double log_binomial(uint64_t n, uint64_t k);
+int64_t theta_prime(int64_t x, double eps_prime, double l, uint64_t k,
+                    uint64_t n);
+uint64_t theta_final(double eps, double l, uint64_t k, double lb, uint64_t n);
+
+// rmat.cpp
+void rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double b,
+                double c, uint64_t seed, std::vector<uint32_t> &src,
+                std::vector<uint32_t> &dst);
+void csr_from_edges(uint32_t n, uint64_t m, const uint32_t *src,
+                    const uint32_t *dst, int scheme, float scale,
+                    uint64_t seed, uint64_t *index, uint32_t *edges,
+                    float *weights);
+
+// device helpers
+template <typename T>
+T *dev_alloc(size_t count) {
+  if (count == 0) count = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess)
+    throw Error(-2, std::string("hipMalloc of ") +
+                        std::to_string(count * sizeof(T)) +
+                        " bytes failed: " + hipGetErrorString(e));
+  return static_cast<T *>(p);
+}
+
+template <typename T>
+void dev_free(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace rip

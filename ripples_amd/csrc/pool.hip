@@ -1,0 +1,339 @@
+// HBM-resident RRR pool.
+//
+// Sets of one (graph, seed) stream are indexed by sample id: d_off[s] and
+// d_len[s] locate sample s's sorted vertex list inside d_data.  Samplers
+// reserve space with one atomic bump per colour group (IC) or per wave (LT),
+// so the lists of a group are contiguous but groups land in claim order;
+// export compacts a range into CSR order.  Replaces the RRRsets vector of
+// include/ripples/rrr_sets.h:81-105 and the flattened device copy built for
+// selection (streaming_find_most_influential.h:139-244).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device_util.h"
+#include "internal.h"
+
+namespace rip {
+namespace {
+
+constexpr uint32_t kScanBT = 256;
+constexpr uint32_t kScanItems = 8;
+constexpr uint32_t kScanTile = kScanBT * kScanItems;
+
+__global__ void scan_tile_sums(const uint32_t *in, uint64_t n,
+                               unsigned long long *tile_sum) {
+  __shared__ uint32_t s_tmp[kScanBT / 64];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+  uint32_t acc = 0;
+  for (uint32_t i = 0; i < kScanItems; ++i) {
+    const uint64_t j = base + static_cast<uint64_t>(i) * kScanBT + threadIdx.x;
+    if (j < n) acc += in[j];
+  }
+  uint32_t total;
+  (void)block_exclusive_scan<kScanBT>(acc, s_tmp, &total);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// single workgroup: exclusive scan of the tile sums (in place)
+__global__ void scan_tile_prefix(unsigned long long *tile_sum, uint64_t tiles) {
+  __shared__ unsigned long long s_w[kScanBT / 64];
+  __shared__ unsigned long long s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  for (uint64_t b = 0; b < tiles; b += kScanBT) {
+    const uint64_t j = b + threadIdx.x;
+    const unsigned long long v = j < tiles ? tile_sum[j] : 0ull;
+    unsigned long long x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+      unsigned long long y = __shfl_up(x, d, 64);
+      if (lane >= static_cast<uint32_t>(d)) x += y;
+    }
+    if (lane == 63) s_w[wid] = x;
+    __syncthreads();
+    unsigned long long off = s_carry;
+    for (uint32_t w = 0; w < wid; ++w) off += s_w[w];
+    if (j < tiles) tile_sum[j] = off + x - v;
+    __syncthreads();
+    if (threadIdx.x == kScanBT - 1) s_carry = off + x;
+    __syncthreads();
+  }
+}
+
+__global__ void scan_apply(const uint32_t *in, uint64_t n,
+                           const unsigned long long *tile_prefix,
+                           unsigned long long *out) {
+  __shared__ uint32_t s_tmp[kScanBT / 64];
+  // each thread owns kScanItems consecutive elements of the tile
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile +
+                        static_cast<uint64_t>(threadIdx.x) * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t acc = 0;
+  for (uint32_t i = 0; i < kScanItems; ++i) {
+    v[i] = (base + i < n) ? in[base + i] : 0u;
+    acc += v[i];
+  }
+  uint32_t total;
+  uint32_t pre = block_exclusive_scan<kScanBT>(acc, s_tmp, &total);
+  unsigned long long run = tile_prefix[blockIdx.x] + pre;
+  for (uint32_t i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+  if (base + kScanItems >= n && base < n) out[n] = run;  // thread holding the last element
+}
+
+__global__ void set_ctl_kernel(unsigned long long *ctl, unsigned long long bump) {
+  ctl[kCtlDataBump] = bump;
+  ctl[kCtlOverflow] = 0;
+  ctl[kCtlGroupCounter] = 0;
+  ctl[kCtlStatBytes] = 0;
+}
+
+// Copy sets [first, first+count) into CSR order at out (prefix from scan).
+// A wave handles 64 consecutive sets: a wave scan of their sizes lets every
+// lane copy elements of any of the 64 sets, so small and large sets balance.
+__global__ void gather_sets(const unsigned long long *set_off,
+                            const uint32_t *set_len, const uint32_t *data,
+                            uint64_t first, uint64_t count,
+                            const unsigned long long *prefix, uint32_t *out) {
+  __shared__ uint32_t s_pre[4][65];
+  __shared__ unsigned long long s_off[4][64];
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint64_t w0 = (static_cast<uint64_t>(blockIdx.x) * 4 + wid) * 64;
+  if (w0 >= count) return;
+  const uint64_t j = w0 + lane;
+  const uint32_t len = j < count ? set_len[first + j] : 0u;
+  s_off[wid][lane] = j < count ? set_off[first + j] : 0ull;
+  const uint32_t incl = wave_inclusive_scan(len);
+  s_pre[wid][lane + 1] = incl;
+  if (lane == 0) s_pre[wid][0] = 0;
+  const uint32_t total = __shfl(incl, 63, 64);
+  __builtin_amdgcn_wave_barrier();
+  const unsigned long long obase = prefix[w0];
+  for (uint32_t f = lane; f < total; f += 64) {
+    uint32_t lo = 0, hi = 64;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (s_pre[wid][mid + 1] <= f) lo = mid + 1;
+      else hi = mid;
+    }
+    out[obase + f] = data[s_off[wid][lo] + (f - s_pre[wid][lo])];
+  }
+}
+
+}  // namespace
+
+void exclusive_scan_u32(const uint32_t *d_in, unsigned long long *d_out,
+                        uint64_t n, hipStream_t stream) {
+  if (n == 0) {
+    const unsigned long long zero = 0;
+    RIP_HIP(hipMemcpyAsync(d_out, &zero, sizeof(zero), hipMemcpyHostToDevice, stream));
+    RIP_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+  unsigned long long *tmp = dev_alloc<unsigned long long>(tiles);
+  hipLaunchKernelGGL(scan_tile_sums, dim3(static_cast<uint32_t>(tiles)), dim3(kScanBT),
+                     0, stream, d_in, n, tmp);
+  hipLaunchKernelGGL(scan_tile_prefix, dim3(1), dim3(kScanBT), 0, stream, tmp, tiles);
+  hipLaunchKernelGGL(scan_apply, dim3(static_cast<uint32_t>(tiles)), dim3(kScanBT), 0,
+                     stream, d_in, n, tmp, d_out);
+  hipError_t e = hipGetLastError();
+  RIP_HIP(hipStreamSynchronize(stream));
+  dev_free(tmp);
+  RIP_HIP(e);
+}
+
+void pool_create(rip_pool *p, rip_graph *g, uint64_t seed) {
+  RIP_HIP(hipSetDevice(g->device));
+  p->g = g;
+  p->seed = seed;
+  RIP_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  RIP_HIP(hipEventCreate(&p->ev0));
+  RIP_HIP(hipEventCreate(&p->ev1));
+  p->d_ctl = dev_alloc<unsigned long long>(kCtlWords);
+  RIP_HIP(hipMemsetAsync(p->d_ctl, 0, sizeof(unsigned long long) * kCtlWords, p->stream));
+  RIP_HIP(hipStreamSynchronize(p->stream));
+}
+
+void pool_destroy(rip_pool *p) {
+  if (!p || !p->g) return;
+  (void)hipSetDevice(p->g->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  dev_free(p->d_off);
+  dev_free(p->d_len);
+  dev_free(p->d_data);
+  dev_free(p->d_ctl);
+  dev_free(p->ic.d_vp);
+  dev_free(p->ic.d_bm0);
+  dev_free(p->ic.d_bm1);
+  dev_free(p->ic.d_queue);
+  dev_free(p->lt.d_long_ids);
+  dev_free(p->sel.d_counts);
+  dev_free(p->sel.d_inv_start);
+  dev_free(p->sel.d_cursor);
+  dev_free(p->sel.d_inv);
+  dev_free(p->sel.d_covered);
+  dev_free(p->sel.d_keys);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  p->stream = nullptr;
+}
+
+void pool_reserve_sets(rip_pool *p, uint64_t cap) {
+  if (cap <= p->cap_sets) return;
+  cap = std::max(cap, 2 * p->cap_sets);
+  auto *off = dev_alloc<unsigned long long>(cap);
+  uint32_t *len;
+  try {
+    len = dev_alloc<uint32_t>(cap);
+  } catch (...) {
+    dev_free(off);
+    throw;
+  }
+  if (p->num_sets) {
+    RIP_HIP(hipMemcpyAsync(off, p->d_off, sizeof(unsigned long long) * p->num_sets,
+                           hipMemcpyDeviceToDevice, p->stream));
+    RIP_HIP(hipMemcpyAsync(len, p->d_len, sizeof(uint32_t) * p->num_sets,
+                           hipMemcpyDeviceToDevice, p->stream));
+  }
+  RIP_HIP(hipStreamSynchronize(p->stream));
+  dev_free(p->d_off);
+  dev_free(p->d_len);
+  p->d_off = off;
+  p->d_len = len;
+  p->cap_sets = cap;
+}
+
+void pool_reserve_data(rip_pool *p, uint64_t cap) {
+  if (cap <= p->cap_data) return;
+  auto *data = dev_alloc<uint32_t>(cap);
+  if (p->used_data)
+    RIP_HIP(hipMemcpyAsync(data, p->d_data, sizeof(uint32_t) * p->used_data,
+                           hipMemcpyDeviceToDevice, p->stream));
+  RIP_HIP(hipStreamSynchronize(p->stream));
+  dev_free(p->d_data);
+  p->d_data = data;
+  p->cap_data = cap;
+}
+
+void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
+  if (count == 0) return;
+  RIP_HIP(hipSetDevice(p->g->device));
+  RIP_REQUIRE(p->num_sets + count < (1ull << 32), -1,
+              "pool: at most 2^32-1 samples per stream");
+  pool_reserve_sets(p, p->num_sets + count);
+  const double est = static_cast<double>(count) * p->avg_set_size * 1.25 + 65536.0;
+  if (p->used_data + static_cast<uint64_t>(est) > p->cap_data)
+    pool_reserve_data(p, std::max<uint64_t>(p->used_data + static_cast<uint64_t>(est),
+                                            p->cap_data + p->cap_data / 2));
+  IcLaunch ic{};
+  if (p->g->model == RIP_IC) ic = ic_tune(p, tune);
+  const uint64_t first = p->num_sets;
+  for (int attempt = 0;; ++attempt) {
+    RIP_REQUIRE(attempt < 40, -2, "pool: data capacity did not converge");
+    hipLaunchKernelGGL(set_ctl_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl,
+                       static_cast<unsigned long long>(p->used_data));
+    // the launchers bracket their main kernel with ev0 / ev1
+    uint32_t launches;
+    if (p->g->model == RIP_IC)
+      launches = ic_launch(p, first, count, ic);
+    else
+      launches = lt_launch(p, first, count, tune);
+    unsigned long long ctl[kCtlWords];
+    RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, p->stream));
+    RIP_HIP(hipStreamSynchronize(p->stream));
+    if (ctl[kCtlOverflow] == 0) {
+      float ms = 0.0f;
+      RIP_HIP(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+      p->last_kernel_ms = ms;
+      p->last_launches = launches;
+      p->last_algo_bytes = ctl[kCtlStatBytes];
+      const uint64_t added = ctl[kCtlDataBump] - p->used_data;
+      p->used_data = ctl[kCtlDataBump];
+      p->avg_set_size = std::max(1.0, static_cast<double>(added) / static_cast<double>(count));
+      p->num_sets += count;
+      return;
+    }
+    // grow to what the failed attempt asked for, with headroom, and redo
+    const uint64_t want = ctl[kCtlDataBump];
+    pool_reserve_data(p, std::max<uint64_t>(want + want / 4, 2 * p->cap_data));
+  }
+}
+
+void pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
+                      const uint32_t *vertices) {
+  if (count == 0) return;
+  RIP_REQUIRE(offsets && offsets[0] == 0, -1, "append: offsets[0] must be 0");
+  RIP_REQUIRE(p->num_sets + count < (1ull << 32), -1,
+              "pool: at most 2^32-1 samples per stream");
+  const uint64_t total = offsets[count];
+  RIP_REQUIRE(total == 0 || vertices, -1, "append: vertices is NULL");
+  std::vector<unsigned long long> off(count);
+  std::vector<uint32_t> len(count);
+  for (uint64_t i = 0; i < count; ++i) {
+    RIP_REQUIRE(offsets[i + 1] >= offsets[i], -1, "append: offsets not monotone");
+    RIP_REQUIRE(offsets[i + 1] - offsets[i] < 0xFFFFFFF0ull, -1, "append: set too large");
+    for (uint64_t j = offsets[i]; j < offsets[i + 1]; ++j) {
+      RIP_REQUIRE(vertices[j] < p->g->n, -1, "append: vertex id out of range");
+      RIP_REQUIRE(j == offsets[i] || vertices[j - 1] < vertices[j], -1,
+                  "append: sets must be sorted and duplicate-free");
+    }
+    off[i] = p->used_data + offsets[i];
+    len[i] = static_cast<uint32_t>(offsets[i + 1] - offsets[i]);
+  }
+  RIP_HIP(hipSetDevice(p->g->device));
+  pool_reserve_sets(p, p->num_sets + count);
+  if (p->used_data + total > p->cap_data)
+    pool_reserve_data(p, std::max<uint64_t>(p->used_data + total,
+                                            p->cap_data + p->cap_data / 2));
+  if (total)
+    RIP_HIP(hipMemcpy(p->d_data + p->used_data, vertices, sizeof(uint32_t) * total,
+                      hipMemcpyHostToDevice));
+  RIP_HIP(hipMemcpy(p->d_off + p->num_sets, off.data(), sizeof(unsigned long long) * count,
+                    hipMemcpyHostToDevice));
+  RIP_HIP(hipMemcpy(p->d_len + p->num_sets, len.data(), sizeof(uint32_t) * count,
+                    hipMemcpyHostToDevice));
+  p->num_sets += count;
+  p->used_data += total;
+}
+
+void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
+                 uint32_t *vertices, uint64_t cap) {
+  RIP_REQUIRE(first + count <= p->num_sets && first + count >= first, -1,
+              "export: range beyond the pool");
+  RIP_REQUIRE(offsets != nullptr, -1, "export: offsets is NULL");
+  RIP_HIP(hipSetDevice(p->g->device));
+  auto *pre = dev_alloc<unsigned long long>(count + 1);
+  uint32_t *out = nullptr;
+  try {
+    exclusive_scan_u32(p->d_len + first, pre, count, p->stream);
+    RIP_HIP(hipMemcpy(offsets, pre, sizeof(uint64_t) * (count + 1), hipMemcpyDeviceToHost));
+    const uint64_t total = offsets[count];
+    if (vertices) {
+      RIP_REQUIRE(total <= cap, -1, "export: vertices buffer too small");
+      out = dev_alloc<uint32_t>(total);
+      const uint64_t waves = (count + 63) / 64;
+      if (waves) {
+        hipLaunchKernelGGL(gather_sets, dim3(static_cast<uint32_t>((waves + 3) / 4)),
+                           dim3(256), 0, p->stream, p->d_off, p->d_len, p->d_data,
+                           first, count, pre, out);
+        RIP_HIP(hipGetLastError());
+      }
+      RIP_HIP(hipStreamSynchronize(p->stream));
+      if (total)
+        RIP_HIP(hipMemcpy(vertices, out, sizeof(uint32_t) * total, hipMemcpyDeviceToHost));
+    }
+  } catch (...) {
+    dev_free(pre);
+    dev_free(out);
+    throw;
+  }
+  dev_free(pre);
+  dev_free(out);
+}
+
+}  // namespace rip

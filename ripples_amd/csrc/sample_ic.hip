@@ -1,0 +1,459 @@
+// Fused multi-colour IC sampler: reverse BFS of up to 64 RRR samples at once.
+//
+// Follows the IC semantics of AddRRRSet (include/ripples/add_rrrset.h:105-157:
+// an in-edge (u -> v) is live when uniform01 <= w) and the colour-per-sample
+// idea of BatchedBFSNeighborColor (batched_add_rrrset.h:184-257) and
+// GPUBatchedBFSMultiColorFused (include/ripples/gpu/bfs.h:1338-1680), but with
+// the per-sample stream contract of common.h: the coin of (sample s, edge e)
+// depends only on (seed, s, e), so the set is the reverse reachability of the
+// root over live edges, independent of traversal order.
+//
+// Execution: a persistent grid, one workgroup per resident colour group
+// ("slot").  Each workgroup claims groups of `colors` consecutive samples
+// from a device counter and runs a level-synchronous BFS for all colours:
+//   visited[v] / pending[v] : 64-bit colour masks per vertex (per slot)
+//   frontier                : sparse queue of vertices whose pending mask
+//                             went 0 -> nonzero (each vertex at most once per
+//                             level); masks are read and cleared when a chunk
+//                             of the queue is staged into LDS
+//   expansion               : the chunk's rows are split into runs of RUN
+//                             consecutive edges, balanced over all lanes by a
+//                             workgroup scan + binary search in LDS; a run
+//                             jumps to its first coin with one table lookup
+//                             and steps the lcg64 per edge
+//   discovery               : atomicOr on visited, then on pending; queue and
+//                             union bookkeeping are wave-aggregated
+//                             (ballot/popcount) into LDS counters
+//   output                  : the union of the group is kept in a two-level
+//                             bitmap, read back in ascending vertex order,
+//                             and each colour's sorted list is scattered to
+//                             the pool with ballot ranks.  Everything touched
+//                             is cleared again, so no O(n) reset per group.
+#include <algorithm>
+
+#include "device_util.h"
+#include "internal.h"
+
+namespace rip {
+namespace {
+
+struct IcParams {
+  const VertexRec *vrec;
+  const uint2 *adj;
+  const Affine *jtab;
+  uint32_t n;
+  Affine step;
+  uint64_t seed;
+  uint64_t first;
+  uint64_t count;
+  uint32_t colors;
+  uint64_t ngroups;
+  unsigned long long *vp;
+  uint32_t *bm0;
+  uint32_t *bm1;
+  uint32_t *queue;
+  uint32_t nw0, nw1;
+  uint64_t npad;  // queue words per buffer (even)
+  unsigned long long *set_off;
+  uint32_t *set_len;
+  uint32_t *data;
+  uint64_t cap_data;
+  unsigned long long *ctl;
+};
+
+// Mark (u, bits) discovered.  Every lane of the wave must call it.
+__device__ inline void discover(const IcParams &P, unsigned long long *vp,
+                                uint32_t *bm0, uint32_t *bm1, uint32_t u,
+                                unsigned long long bits, uint32_t *s_union,
+                                uint32_t *s_qn, uint32_t *qnext) {
+  bool first = false, push = false;
+  if (bits) {
+    unsigned long long old = atomicOr(&vp[2ull * u], bits);
+    unsigned long long fresh = bits & ~old;
+    if (fresh) {
+      first = (old == 0);
+      unsigned long long pold = atomicOr(&vp[2ull * u + 1], fresh);
+      push = (pold == 0);
+    }
+  }
+  if (first) {
+    uint32_t o0 = atomicOr(&bm0[u >> 5], 1u << (u & 31));
+    if (o0 == 0) atomicOr(&bm1[u >> 10], 1u << ((u >> 5) & 31));
+  }
+  uint64_t fm = __ballot(first);
+  if (fm && lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(fm)) - 1))
+    atomicAdd(s_union, static_cast<uint32_t>(__popcll(fm)));
+  wave_append(push, u, s_qn, qnext);
+}
+
+template <int BT, int RUN>
+__global__ void __launch_bounds__(BT)
+ic_fused_kernel(IcParams P) {
+  constexpr int NW = BT / 64;
+  __shared__ unsigned long long s_z[64];
+  __shared__ uint32_t s_cnt[64];
+  __shared__ uint32_t s_cofs[64];
+  __shared__ uint32_t s_tmp[NW];
+  __shared__ unsigned long long s_group;
+  __shared__ unsigned long long s_base;
+  __shared__ uint32_t s_fail;
+  __shared__ uint32_t s_qn[2];
+  __shared__ uint32_t s_union;
+  __shared__ uint32_t s_ebeg[BT];
+  __shared__ uint32_t s_edeg[BT];
+  __shared__ uint32_t s_epre[BT];
+  __shared__ unsigned long long s_ea[BT];
+  __shared__ unsigned long long s_eb[BT];
+  __shared__ unsigned long long s_em[BT];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = tid >> 6;
+  const uint64_t slot = blockIdx.x;
+  unsigned long long *vp = P.vp + slot * 2ull * P.n;
+  uint32_t *bm0 = P.bm0 + slot * P.nw0;
+  uint32_t *bm1 = P.bm1 + slot * P.nw1;
+  uint32_t *qbuf[2] = {P.queue + slot * 3ull * P.npad,
+                       P.queue + slot * 3ull * P.npad + P.npad};
+  // output staging after the BFS: sorted union + masks
+  uint32_t *list = qbuf[0];
+  unsigned long long *masks =
+      reinterpret_cast<unsigned long long *>(qbuf[1]);
+
+  for (;;) {
+    if (tid == 0) s_group = atomicAdd(&P.ctl[kCtlGroupCounter], 1ull);
+    __syncthreads();
+    const uint64_t g = s_group;
+    if (g >= P.ngroups) break;
+    const uint64_t s0 = g * P.colors;
+    const uint32_t nb = static_cast<uint32_t>(
+        (P.count - s0) < P.colors ? (P.count - s0) : P.colors);
+
+    if (tid < 64) s_cnt[tid] = 0;
+    if (tid == 0) {
+      s_qn[0] = 0;
+      s_qn[1] = 0;
+      s_union = 0;
+      s_fail = 0;
+    }
+    uint32_t root = 0;
+    unsigned long long rbits = 0;
+    if (tid < nb) {
+      unsigned long long z = sample_z(P.seed, P.first + s0 + tid);
+      s_z[tid] = z;
+      root = uniform_index(z, P.n);
+      rbits = 1ull << tid;
+    }
+    __syncthreads();
+    if (tid < 64)
+      discover(P, vp, bm0, bm1, root, rbits, &s_union, &s_qn[0], qbuf[0]);
+
+    // ---------------------------------------------------------- BFS levels
+    int cur = 0;
+    for (;;) {
+      __syncthreads();
+      const uint32_t qn = s_qn[cur];
+      const int nxt = cur ^ 1;
+      if (tid == 0) s_qn[nxt] = 0;
+      if (qn == 0) break;
+      const uint32_t *qcur = qbuf[cur];
+      uint32_t *qnext = qbuf[nxt];
+      for (uint32_t cb = 0; cb < qn; cb += BT) {
+        const uint32_t cl = min(static_cast<uint32_t>(BT), qn - cb);
+        uint32_t runs = 0;
+        if (tid < cl) {
+          const uint32_t v = qcur[cb + tid];
+          const unsigned long long M = atomicExch(&vp[2ull * v + 1], 0ull);
+          const VertexRec r = P.vrec[v];
+          const uint32_t deg = r.end - r.begin;
+          s_ebeg[tid] = r.begin;
+          s_edeg[tid] = deg;
+          s_ea[tid] = r.ja;
+          s_eb[tid] = r.jb;
+          s_em[tid] = M;
+          runs = M ? (deg + RUN - 1) / RUN : 0;
+        }
+        uint32_t total;
+        const uint32_t pre = block_exclusive_scan<BT>(runs, s_tmp, &total);
+        if (tid < cl) s_epre[tid] = pre;
+        __syncthreads();
+        for (uint32_t rb = 0; rb < total; rb += BT) {
+          const uint32_t rr = rb + tid;
+          uint32_t src[RUN];
+          unsigned long long acc[RUN];
+#pragma unroll
+          for (int i = 0; i < RUN; ++i) {
+            src[i] = 0;
+            acc[i] = 0;
+          }
+          if (rr < total) {
+            uint32_t lo = 0, hi = cl;
+            while (lo < hi) {
+              uint32_t mid = (lo + hi) >> 1;
+              if (s_epre[mid] <= rr) lo = mid + 1;
+              else hi = mid;
+            }
+            const uint32_t ent = lo - 1;
+            const uint32_t j0 = (rr - s_epre[ent]) * RUN;
+            const uint32_t deg = s_edeg[ent];
+            const uint32_t e0 = s_ebeg[ent] + j0;
+            const unsigned long long M = s_em[ent];
+            const Affine T{s_ea[ent], s_eb[ent]};
+            const Affine E = aff_then(T, P.jtab[j0]);
+            uint32_t thr[RUN];
+#pragma unroll
+            for (int i = 0; i < RUN; ++i) {
+              thr[i] = 0;
+              if (j0 + i < deg) {
+                const uint2 a = P.adj[e0 + i];
+                src[i] = a.x;
+                thr[i] = a.y;
+              }
+            }
+            for (unsigned long long mm = M; mm; mm &= mm - 1) {
+              const int c = __ffsll(mm) - 1;
+              uint64_t x = E.a * s_z[c] + E.b;
+#pragma unroll
+              for (int i = 0; i < RUN; ++i) {
+                if (static_cast<uint32_t>(x >> 40) < thr[i]) acc[i] |= 1ull << c;
+                if (i + 1 < RUN) x = P.step.a * x + P.step.b;
+              }
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < RUN; ++i)
+            discover(P, vp, bm0, bm1, src[i], acc[i], &s_union, &s_qn[nxt], qnext);
+        }
+        __syncthreads();
+      }
+      cur = nxt;
+    }
+
+    // ---------------------------------------------- sorted union + colours
+    const uint32_t U = s_union;
+    uint32_t running = 0;
+    // The bitmaps and visited masks were written by atomics, which complete
+    // in L2; read and clear them with atomics too so no stale L1 line of an
+    // earlier group is ever observed.
+    for (uint32_t t0 = 0; t0 < P.nw1; t0 += BT) {
+      const uint32_t w1i = t0 + tid;
+      const uint32_t w1 =
+          w1i < P.nw1 ? __hip_atomic_load(&bm1[w1i], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT)
+                      : 0u;
+      uint32_t cnt = 0;
+      for (uint32_t b = w1; b; b &= b - 1)
+        cnt += __popc(__hip_atomic_load(&bm0[w1i * 32 + (__ffs(b) - 1)],
+                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      uint32_t tot;
+      uint32_t p = running + block_exclusive_scan<BT>(cnt, s_tmp, &tot);
+      for (uint32_t b = w1; b; b &= b - 1) {
+        const uint32_t w0i = w1i * 32 + (__ffs(b) - 1);
+        const uint32_t w0 = atomicExch(&bm0[w0i], 0u);
+        for (uint32_t bb = w0; bb; bb &= bb - 1) {
+          const uint32_t v = w0i * 32 + (__ffs(bb) - 1);
+          list[p] = v;
+          masks[p] = atomicExch(&vp[2ull * v], 0ull);
+          ++p;
+        }
+      }
+      if (w1) atomicExch(&bm1[w1i], 0u);
+      running += tot;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < U; i += BT)
+      for (unsigned long long mk = masks[i]; mk; mk &= mk - 1)
+        atomicAdd(&s_cnt[__ffsll(mk) - 1], 1u);
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t acc = 0;
+      for (uint32_t c = 0; c < nb; ++c) {
+        s_cofs[c] = acc;
+        acc += s_cnt[c];
+      }
+      unsigned long long base = atomicAdd(&P.ctl[kCtlDataBump], static_cast<unsigned long long>(acc));
+      s_base = base;
+      if (base + acc > P.cap_data) {
+        s_fail = 1;
+        atomicOr(&P.ctl[kCtlOverflow], 1ull);
+      }
+    }
+    __syncthreads();
+    if (!s_fail) {
+      const unsigned long long base = s_base;
+      for (uint32_t c = wid; c < nb; c += NW) {
+        uint32_t run = 0;
+        const unsigned long long dst0 = base + s_cofs[c];
+        for (uint32_t i0 = 0; i0 < U; i0 += 64) {
+          const uint32_t i = i0 + lane;
+          const bool bit = i < U && ((masks[i] >> c) & 1ull);
+          const uint64_t bm = __ballot(bit);
+          if (bit) P.data[dst0 + run + __popcll(bm & lanes_below())] = list[i];
+          run += __popcll(bm);
+        }
+      }
+      if (tid < nb) {
+        P.set_off[P.first + s0 + tid] = base + s_cofs[tid];
+        P.set_len[P.first + s0 + tid] = s_cnt[tid];
+      }
+    } else if (tid < nb) {
+      P.set_len[P.first + s0 + tid] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+  }
+}
+
+template <int BT, int RUN>
+void launch_one(const IcParams &P, uint32_t slots, hipStream_t stream) {
+  hipLaunchKernelGGL((ic_fused_kernel<BT, RUN>), dim3(slots), dim3(BT), 0,
+                     stream, P);
+}
+
+template <int BT>
+void launch_run(const IcParams &P, uint32_t slots, uint32_t run,
+                hipStream_t stream) {
+  switch (run) {
+    case 1: launch_one<BT, 1>(P, slots, stream); break;
+    case 2: launch_one<BT, 2>(P, slots, stream); break;
+    case 8: launch_one<BT, 8>(P, slots, stream); break;
+    default: launch_one<BT, 4>(P, slots, stream); break;
+  }
+}
+
+__global__ void reset_counter_kernel(unsigned long long *ctl) {
+  ctl[kCtlGroupCounter] = 0;
+}
+
+// Workgroups of the persistent kernel that fit on one CU at once.
+template <int BT, int RUN>
+int occupancy_of() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &blocks, reinterpret_cast<const void *>(&ic_fused_kernel<BT, RUN>), BT, 0) !=
+          hipSuccess ||
+      blocks < 1)
+    blocks = 1;
+  return blocks;
+}
+
+template <int BT>
+int occupancy_run(uint32_t run) {
+  switch (run) {
+    case 1: return occupancy_of<BT, 1>();
+    case 2: return occupancy_of<BT, 2>();
+    case 8: return occupancy_of<BT, 8>();
+    default: return occupancy_of<BT, 4>();
+  }
+}
+
+int resident_blocks(uint32_t block, uint32_t run) {
+  switch (block) {
+    case 256: return occupancy_run<256>(run);
+    case 512: return occupancy_run<512>(run);
+    default: return occupancy_run<1024>(run);
+  }
+}
+
+size_t slot_bytes_for(uint32_t n) {
+  const uint64_t npad = (static_cast<uint64_t>(n) + 63) & ~63ull;
+  const uint64_t nw0 = (n + 31) / 32, nw1 = (n + 1023) / 1024;
+  return 16ull * n + 4ull * nw0 + 4ull * nw1 + 12ull * npad;
+}
+
+}  // namespace
+
+IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
+  IcLaunch c{64, 1024, 0, 4};
+  if (tune) {
+    if (tune->colors) c.colors = tune->colors;
+    if (tune->block) c.block = tune->block;
+    if (tune->slots) c.slots = tune->slots;
+    if (tune->edge_run) c.edge_run = tune->edge_run;
+  }
+  RIP_REQUIRE(c.colors >= 1 && c.colors <= 64, -1, "tune: colors must be in [1, 64]");
+  RIP_REQUIRE(c.block == 256 || c.block == 512 || c.block == 1024, -1,
+              "tune: block must be 256, 512 or 1024");
+  RIP_REQUIRE(c.edge_run == 1 || c.edge_run == 2 || c.edge_run == 4 || c.edge_run == 8,
+              -1, "tune: edge_run must be 1, 2, 4 or 8");
+  if (c.slots == 0) {
+    hipDeviceProp_t prop;
+    RIP_HIP(hipGetDeviceProperties(&prop, p->g->device));
+    size_t free_b = 0, total_b = 0;
+    RIP_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per = slot_bytes_for(p->g->n);
+    // keep the sampler scratch within a quarter of the free memory, 64 GiB max
+    size_t budget = std::min(free_b / 4, static_cast<size_t>(64) << 30);
+    if (p->ic.slots && p->ic.n == p->g->n) budget += static_cast<size_t>(p->ic.slots) * per;
+    uint64_t by_mem = std::max<uint64_t>(1, budget / per);
+    uint64_t by_cu = static_cast<uint64_t>(prop.multiProcessorCount) *
+                     resident_blocks(c.block, c.edge_run);
+    c.slots = static_cast<uint32_t>(std::min(by_mem, by_cu));
+  }
+  return c;
+}
+
+void ic_ensure_slots(rip_pool *p, uint32_t slots) {
+  IcSlots &s = p->ic;
+  if (s.slots >= slots && s.n == p->g->n) return;
+  dev_free(s.d_vp);
+  dev_free(s.d_bm0);
+  dev_free(s.d_bm1);
+  dev_free(s.d_queue);
+  s.slots = 0;
+  const uint32_t n = p->g->n;
+  const uint64_t npad = (static_cast<uint64_t>(n) + 63) & ~63ull;
+  const uint64_t nw0 = (n + 31) / 32, nw1 = (n + 1023) / 1024;
+  s.d_vp = dev_alloc<unsigned long long>(2ull * n * slots);
+  s.d_bm0 = dev_alloc<uint32_t>(nw0 * slots);
+  s.d_bm1 = dev_alloc<uint32_t>(nw1 * slots);
+  s.d_queue = dev_alloc<uint32_t>(3ull * npad * slots);
+  RIP_HIP(hipMemsetAsync(s.d_vp, 0, sizeof(unsigned long long) * 2ull * n * slots, p->stream));
+  RIP_HIP(hipMemsetAsync(s.d_bm0, 0, sizeof(uint32_t) * nw0 * slots, p->stream));
+  RIP_HIP(hipMemsetAsync(s.d_bm1, 0, sizeof(uint32_t) * nw1 * slots, p->stream));
+  s.slots = slots;
+  s.n = n;
+  s.slot_bytes = slot_bytes_for(n);
+}
+
+uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
+                   const IcLaunch &cfg) {
+  ic_ensure_slots(p, cfg.slots);
+  const rip_graph *g = p->g;
+  IcParams P;
+  P.vrec = g->d_vrec;
+  P.adj = g->d_adj;
+  P.jtab = g->d_jtab;
+  P.n = g->n;
+  P.step = g->step;
+  P.seed = p->seed;
+  P.first = first;
+  P.count = count;
+  P.colors = cfg.colors;
+  P.ngroups = (count + cfg.colors - 1) / cfg.colors;
+  P.vp = p->ic.d_vp;
+  P.bm0 = p->ic.d_bm0;
+  P.bm1 = p->ic.d_bm1;
+  P.queue = p->ic.d_queue;
+  P.nw0 = (g->n + 31) / 32;
+  P.nw1 = (g->n + 1023) / 1024;
+  P.npad = (static_cast<uint64_t>(g->n) + 63) & ~63ull;
+  P.set_off = p->d_off;
+  P.set_len = p->d_len;
+  P.data = p->d_data;
+  P.cap_data = p->cap_data;
+  P.ctl = p->d_ctl;
+  const uint32_t slots =
+      static_cast<uint32_t>(std::min<uint64_t>(cfg.slots, P.ngroups));
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
+  RIP_HIP(hipEventRecord(p->ev0, p->stream));
+  switch (cfg.block) {
+    case 256: launch_run<256>(P, slots, cfg.edge_run, p->stream); break;
+    case 512: launch_run<512>(P, slots, cfg.edge_run, p->stream); break;
+    default: launch_run<1024>(P, slots, cfg.edge_run, p->stream); break;
+  }
+  RIP_HIP(hipGetLastError());
+  RIP_HIP(hipEventRecord(p->ev1, p->stream));
+  return 2;
+}
+
+}  // namespace rip

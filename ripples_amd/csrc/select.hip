@@ -1,0 +1,281 @@
+// Greedy max-coverage seed selection on the device.
+//
+// Same result as FindMostInfluentialSet (include/ripples/find_most_influential.h:85-174,
+// streaming_find_most_influential.h:607-643) with lowest-id ties as
+// GPUMaxElement (src/gpu/find_most_influential.cc:65-72):
+//   1. counts[v] = number of sets containing v          (counting.h:65-97)
+//   2. inverted index vertex -> sets by counting sort
+//   3. k times: argmax of counts (key = count<<32 | ~v, so the largest key is
+//      the largest count with the lowest id), then a kill pass that visits
+//      only the sets containing the new seed, marks the uncovered ones
+//      covered and decrements the counts of their members.
+// Every set is killed at most once, so the kill passes cost O(T) over all k
+// seeds, where the reference recounts every uncovered set per seed
+// (gpu/find_most_influential.h:98-120).  The seed loop issues 2k kernels
+// back to back with no host synchronisation; the host reads the k keys once.
+#include <algorithm>
+#include <vector>
+
+#include "device_util.h"
+#include "internal.h"
+
+namespace rip {
+namespace {
+
+constexpr uint32_t kBT = 256;
+
+// Each wave walks 64 consecutive sets; lanes take elements of any of them.
+template <typename F>
+__device__ inline void for_each_element(const unsigned long long *set_off,
+                                        const uint32_t *set_len,
+                                        const uint32_t *data, uint64_t num_sets,
+                                        uint32_t (*s_pre)[65],
+                                        unsigned long long (*s_off)[64], F &&f) {
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBT / 64);
+  for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * (kBT / 64) + wid;
+       w * 64 < num_sets; w += nwaves) {
+    const uint64_t s = w * 64 + lane;
+    const uint32_t len = s < num_sets ? set_len[s] : 0u;
+    s_off[wid][lane] = s < num_sets ? set_off[s] : 0ull;
+    const uint32_t incl = wave_inclusive_scan(len);
+    s_pre[wid][lane + 1] = incl;
+    if (lane == 0) s_pre[wid][0] = 0;
+    const uint32_t total = __shfl(incl, 63, 64);
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < total; i += 64) {
+      uint32_t lo = 0, hi = 64;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[wid][mid + 1] <= i) lo = mid + 1;
+        else hi = mid;
+      }
+      f(data[s_off[wid][lo] + (i - s_pre[wid][lo])], w * 64 + lo);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+__global__ void __launch_bounds__(kBT)
+count_kernel(const unsigned long long *set_off, const uint32_t *set_len,
+             const uint32_t *data, uint64_t num_sets, uint32_t *counts) {
+  __shared__ uint32_t s_pre[kBT / 64][65];
+  __shared__ unsigned long long s_off[kBT / 64][64];
+  for_each_element(set_off, set_len, data, num_sets, s_pre, s_off,
+                   [&](uint32_t v, uint64_t) { atomicAdd(&counts[v], 1u); });
+}
+
+__global__ void __launch_bounds__(kBT)
+fill_inverted_kernel(const unsigned long long *set_off, const uint32_t *set_len,
+                     const uint32_t *data, uint64_t num_sets,
+                     unsigned long long *cursor, uint32_t *inv) {
+  __shared__ uint32_t s_pre[kBT / 64][65];
+  __shared__ unsigned long long s_off[kBT / 64][64];
+  for_each_element(set_off, set_len, data, num_sets, s_pre, s_off,
+                   [&](uint32_t v, uint64_t s) {
+                     const unsigned long long p = atomicAdd(&cursor[v], 1ull);
+                     inv[p] = static_cast<uint32_t>(s);
+                   });
+}
+
+__global__ void sum_len_kernel(const uint32_t *set_len, uint64_t num_sets,
+                               unsigned long long *out) {
+  unsigned long long acc = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < num_sets; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    acc += set_len[i];
+  for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d, 64);
+  if (lane_id() == 0 && acc) atomicAdd(out, acc);
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void fill_u8_kernel(uint8_t *p, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+
+__global__ void copy_u64_kernel(unsigned long long *dst,
+                                const unsigned long long *src, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void zero_word_kernel(unsigned long long *w) { *w = 0; }
+
+__global__ void clear_keys_kernel(unsigned long long *keys, uint32_t k,
+                                  unsigned long long *ctl) {
+  for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) keys[i] = 0;
+  if (threadIdx.x == 0) ctl[kCtlCovered] = 0;
+}
+
+__device__ inline unsigned long long wave_max_u64(unsigned long long x) {
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long y = __shfl_xor(x, d, 64);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+__global__ void __launch_bounds__(kBT)
+argmax_kernel(const uint32_t *counts, uint32_t n, unsigned long long *key_out) {
+  __shared__ unsigned long long s_w[kBT / 64];
+  unsigned long long best = 0;
+  for (uint32_t v = blockIdx.x * kBT + threadIdx.x; v < n; v += gridDim.x * kBT) {
+    const unsigned long long key =
+        (static_cast<unsigned long long>(counts[v]) << 32) | (0xFFFFFFFFu - v);
+    best = key > best ? key : best;
+  }
+  best = wave_max_u64(best);
+  if (lane_id() == 0) s_w[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < kBT / 64; ++w) best = s_w[w] > best ? s_w[w] : best;
+    if (best >> 32) atomicMax(key_out, best);
+  }
+}
+
+__global__ void __launch_bounds__(kBT)
+kill_kernel(const unsigned long long *key_in, const unsigned long long *inv_start,
+            const uint32_t *inv, const unsigned long long *set_off,
+            const uint32_t *set_len, const uint32_t *data, uint8_t *covered,
+            uint32_t *counts, unsigned long long *ctl) {
+  const unsigned long long key = *key_in;
+  const uint32_t gain = static_cast<uint32_t>(key >> 32);
+  if (gain == 0) return;
+  const uint32_t seed = 0xFFFFFFFFu - static_cast<uint32_t>(key);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl[kCtlCovered], gain);
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBT / 64);
+  const unsigned long long b = inv_start[seed], e = inv_start[seed + 1];
+  for (unsigned long long p = b + blockIdx.x * (kBT / 64) + (threadIdx.x >> 6); p < e;
+       p += nwaves) {
+    const uint32_t s = inv[p];
+    if (covered[s]) continue;  // wave-uniform
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) covered[s] = 1;
+    const unsigned long long off = set_off[s];
+    const uint32_t len = set_len[s];
+    for (uint32_t i = lane; i < len; i += 64) atomicSub(&counts[data[off + i]], 1u);
+  }
+}
+
+void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t total,
+                           uint32_t k) {
+  SelectScratch &s = p->sel;
+  const uint32_t n = p->g->n;
+  if (s.n != n) {
+    dev_free(s.d_counts);
+    dev_free(s.d_inv_start);
+    dev_free(s.d_cursor);
+    s.d_counts = dev_alloc<uint32_t>(n);
+    s.d_inv_start = dev_alloc<unsigned long long>(static_cast<uint64_t>(n) + 1);
+    s.d_cursor = dev_alloc<unsigned long long>(n);
+    s.n = n;
+  }
+  if (s.cap_sets < num_sets) {
+    dev_free(s.d_covered);
+    s.cap_sets = std::max<uint64_t>(num_sets, s.cap_sets + s.cap_sets / 2);
+    s.d_covered = dev_alloc<uint8_t>(s.cap_sets);
+  }
+  if (s.cap_elems < total) {
+    dev_free(s.d_inv);
+    s.cap_elems = std::max<uint64_t>(total, s.cap_elems + s.cap_elems / 2);
+    s.d_inv = dev_alloc<uint32_t>(s.cap_elems);
+  }
+  if (s.cap_k < k) {
+    dev_free(s.d_keys);
+    s.cap_k = k;
+    s.d_keys = dev_alloc<unsigned long long>(k);
+  }
+}
+
+uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap) {
+  uint64_t g = (work + per_block - 1) / per_block;
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
+}
+
+// counts over sets [0, num_sets); returns the element total
+uint64_t build_counts(rip_pool *p, uint64_t num_sets, uint32_t k) {
+  RIP_REQUIRE(num_sets <= p->num_sets, -1, "select: num_sets beyond the pool");
+  const uint32_t n = p->g->n;
+  hipStream_t st = p->stream;
+  RIP_HIP(hipSetDevice(p->g->device));
+  // total element count
+  hipLaunchKernelGGL(zero_word_kernel, dim3(1), dim3(1), 0, st, p->d_ctl + kCtlWords - 1);
+  hipLaunchKernelGGL(sum_len_kernel, dim3(grid_for(num_sets, kBT, 1024)), dim3(kBT), 0, st,
+                     p->d_len, num_sets, p->d_ctl + kCtlWords - 1);
+  unsigned long long total = 0;
+  RIP_HIP(hipMemcpyAsync(&total, p->d_ctl + kCtlWords - 1, sizeof(total),
+                         hipMemcpyDeviceToHost, st));
+  RIP_HIP(hipStreamSynchronize(st));
+  ensure_select_scratch(p, num_sets, total, k);
+  SelectScratch &s = p->sel;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, kBT, 2048)), dim3(kBT), 0, st,
+                     s.d_counts, static_cast<uint64_t>(n), 0u);
+  if (num_sets)
+    hipLaunchKernelGGL(count_kernel, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
+                       dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets,
+                       s.d_counts);
+  RIP_HIP(hipGetLastError());
+  return total;
+}
+
+}  // namespace
+
+uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
+                    uint64_t *covered) {
+  const uint32_t n = p->g->n;
+  hipStream_t st = p->stream;
+  build_counts(p, num_sets, std::max<uint32_t>(k, 1));
+  SelectScratch &s = p->sel;
+  exclusive_scan_u32(s.d_counts, s.d_inv_start, n, st);
+  hipLaunchKernelGGL(copy_u64_kernel, dim3(grid_for(n, kBT, 2048)), dim3(kBT), 0, st,
+                     s.d_cursor, s.d_inv_start, static_cast<uint64_t>(n));
+  if (num_sets)
+    hipLaunchKernelGGL(fill_inverted_kernel,
+                       dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)), dim3(kBT), 0, st,
+                       p->d_off, p->d_len, p->d_data, num_sets, s.d_cursor, s.d_inv);
+  hipLaunchKernelGGL(fill_u8_kernel, dim3(grid_for(num_sets, kBT, 2048)), dim3(kBT), 0, st,
+                     s.d_covered, num_sets);
+  hipLaunchKernelGGL(clear_keys_kernel, dim3(1), dim3(256), 0, st, s.d_keys, k, p->d_ctl);
+  const uint32_t argmax_grid = grid_for(n, kBT * 8, 1024);
+  for (uint32_t i = 0; i < k; ++i) {
+    hipLaunchKernelGGL(argmax_kernel, dim3(argmax_grid), dim3(kBT), 0, st, s.d_counts, n,
+                       s.d_keys + i);
+    hipLaunchKernelGGL(kill_kernel, dim3(256), dim3(kBT), 0, st, s.d_keys + i,
+                       s.d_inv_start, s.d_inv, p->d_off, p->d_len, p->d_data,
+                       s.d_covered, s.d_counts, p->d_ctl);
+  }
+  RIP_HIP(hipGetLastError());
+  std::vector<unsigned long long> keys(std::max<uint32_t>(k, 1));
+  unsigned long long cov = 0;
+  if (k)
+    RIP_HIP(hipMemcpyAsync(keys.data(), s.d_keys, sizeof(unsigned long long) * k,
+                           hipMemcpyDeviceToHost, st));
+  RIP_HIP(hipMemcpyAsync(&cov, p->d_ctl + kCtlCovered, sizeof(cov),
+                         hipMemcpyDeviceToHost, st));
+  RIP_HIP(hipStreamSynchronize(st));
+  uint32_t got = 0;
+  for (uint32_t i = 0; i < k; ++i) {
+    if ((keys[i] >> 32) == 0) break;
+    seeds[got++] = 0xFFFFFFFFu - static_cast<uint32_t>(keys[i]);
+  }
+  if (covered) *covered = cov;
+  return got;
+}
+
+void count_run(rip_pool *p, uint64_t num_sets, uint32_t *counts_host) {
+  build_counts(p, num_sets, 1);
+  RIP_HIP(hipMemcpyAsync(counts_host, p->sel.d_counts, sizeof(uint32_t) * p->g->n,
+                         hipMemcpyDeviceToHost, p->stream));
+  RIP_HIP(hipStreamSynchronize(p->stream));
+}
+
+}  // namespace rip

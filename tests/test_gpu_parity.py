@@ -1,0 +1,163 @@
+"""GPU parity: the HIP path against the CPU oracle, bit for bit, through the
+C ABI.  RRR sets are compared as sorted vertex lists; seeds, coverage and
+theta must be identical."""
+import numpy as np
+import pytest
+
+import ripples_amd as R
+from graphs import (karate_csr, lt_normalise, random_graph, sets_of,
+                    transposed_csr)
+
+pytestmark = pytest.mark.gpu
+
+
+def rmat_csr(scale, scheme, scale_w, seed=0):
+    s, d = R.rmat_edges(scale)
+    return R.csr_from_edges(1 << scale, s, d, scheme, scale_w, seed)
+
+
+GRAPHS = {
+    "karate_ic": lambda: (R.IC,) + karate_csr(),
+    "karate_lt": lambda: (R.LT,) + karate_csr(0.2),
+    "random_ic": lambda: (R.IC,) + transposed_csr(*random_graph(500, 6000, 1, 0.0, 0.25)),
+    "random_lt": lambda: (R.LT,) + transposed_csr(
+        *(lambda n, s, d, w: (n, s, d, lt_normalise(n, s, d, w)))(*random_graph(500, 6000, 2))),
+    "rmat10_ic": lambda: (R.IC,) + rmat_csr(10, R.W_UNIFORM, 0.1),
+    "rmat12_wc": lambda: (R.IC,) + rmat_csr(12, R.W_CASCADE, 1.0),
+    "rmat12_lt": lambda: (R.LT,) + rmat_csr(12, R.W_LT_NORM, 0.9),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(GRAPHS))
+def case(request):
+    model, index, edges, w = GRAPHS[request.param]()
+    g = R.Graph(index, edges, w, model=model)
+    yield request.param, model, index, edges, w, g
+    g.close()
+
+
+def assert_same_sets(got, exp):
+    assert np.array_equal(got[0], exp[0]), "set sizes differ"
+    assert np.array_equal(got[1], exp[1]), "set contents differ"
+
+
+def test_sampling_matches_oracle(case, orc):
+    name, model, index, edges, w, g = case
+    seed, count = 11, 3000
+    p = g.pool(seed)
+    p.generate_rrr_sets(1000)
+    p.generate_rrr_sets(count - 1000)       # appended call continues the ids
+    got = p.export()
+    exp = orc.sample(model, index, edges, w, seed, 0, count)
+    assert_same_sets(got, exp)
+    part = p.export(1500, 700)
+    assert_same_sets(part, orc.sample(model, index, edges, w, seed, 1500, 700))
+    p.close()
+
+
+def test_selection_matches_oracle(case, orc):
+    name, model, index, edges, w, g = case
+    p = g.pool(3)
+    p.generate_rrr_sets(4000)
+    n = len(index) - 1
+    off, ver = p.export()
+    for k in (1, 5, 50):
+        for num in (4000, 1234):
+            seeds, cov = p.find_most_influential_set(k, num)
+            es, ec = orc.select(n, off[:num + 1], ver[:int(off[num])], k)
+            assert seeds.tolist() == es.tolist() and cov == ec
+    assert np.array_equal(p.count_occurrences(), orc.count(n, off, ver))
+    p.close()
+
+
+@pytest.mark.parametrize("tune", [dict(colors=1), dict(colors=17, block=256),
+                                  dict(colors=64, block=512, edge_run=1),
+                                  dict(colors=32, block=1024, edge_run=8, slots=3),
+                                  dict(colors=64, block=256, edge_run=2, slots=1)])
+def test_ic_tuning_does_not_change_results(tune, orc):
+    index, edges, w = rmat_csr(12, R.W_UNIFORM, 0.1, 1)
+    g = R.Graph(index, edges, w, model=R.IC)
+    p = g.pool(5)
+    p.generate_rrr_sets(2500, tune=tune)
+    assert_same_sets(p.export(), orc.sample(R.IC, index, edges, w, 5, 0, 2500))
+    p.close()
+    g.close()
+
+
+def test_selector_reference_fixture():
+    sets = [[2, 3] if i % 2 else [2, 5] for i in range(1000)]
+    sets += [[[3, 4, 5], [1, 3], [1, 5]][i % 3] for i in range(99)]
+    off = np.zeros(len(sets) + 1, np.uint64)
+    off[1:] = np.cumsum([len(s) for s in sets])
+    ver = np.array([v for s in sets for v in s], np.uint32)
+    index = np.zeros(7, np.uint64)
+    g = R.Graph(index, np.zeros(0, np.uint32), np.zeros(0, np.float32))
+    p = g.pool(0)
+    p.append(off, ver)
+    assert p.count_occurrences().tolist() == [0, 66, 1000, 566, 33, 566]
+    seeds, cov = p.find_most_influential_set(1)
+    assert seeds.tolist() == [2] and cov == 1000
+    seeds, cov = p.find_most_influential_set(10)
+    assert seeds.tolist() == [2, 1, 3] and cov == 1099
+    got = p.export()
+    assert np.array_equal(got[0], off) and np.array_equal(got[1], ver)
+
+
+def test_pool_growth_retry(orc):
+    # every set is the whole reverse closure: far beyond the first estimate
+    n, s, d, _ = random_graph(3000, 30000, 4)
+    index, edges, w = transposed_csr(n, s, d, np.ones(len(s), np.float32))
+    g = R.Graph(index, edges, w, model=R.IC)
+    p = g.pool(1)
+    p.generate_rrr_sets(600)
+    assert_same_sets(p.export(), orc.sample(R.IC, index, edges, w, 1, 0, 600))
+    p.close()
+    g.close()
+
+
+def test_lt_long_walks(orc):
+    # a 300-vertex chain with unit weights: walks far longer than the LDS list
+    n = 300
+    s = np.arange(1, n)
+    d = np.arange(0, n - 1)
+    index, edges, w = transposed_csr(n, s, d, np.ones(n - 1, np.float32))
+    g = R.Graph(index, edges, w, model=R.LT)
+    p = g.pool(2)
+    p.generate_rrr_sets(500)
+    got = p.export()
+    assert_same_sets(got, orc.sample(R.LT, index, edges, w, 2, 0, 500))
+    assert max(np.diff(got[0])) > 64
+    p.close()
+    g.close()
+
+
+@pytest.mark.parametrize("model,k,eps", [(R.IC, 5, 0.5), (R.LT, 10, 0.3)])
+def test_imm_matches_oracle(model, k, eps, orc):
+    if model == R.IC:
+        index, edges, w = rmat_csr(10, R.W_UNIFORM, 0.1)
+    else:
+        index, edges, w = rmat_csr(10, R.W_LT_NORM, 0.9)
+    g = R.Graph(index, edges, w, model=model)
+    seeds, rec = R.imm(g, k, eps, seed=0)
+    es, erec = orc.imm(model, index, edges, w, k, eps, 0)
+    assert seeds.tolist() == es.tolist()
+    assert rec["Theta"] == erec.theta and rec["Rounds"] == erec.rounds
+    assert rec["ThetaPrime"] == list(erec.theta_prime[:erec.rounds])
+    assert rec["F"] == list(erec.f[:erec.rounds])
+    assert rec["FinalF"] == erec.final_f
+    g.close()
+
+
+def test_scale16_properties(orc):
+    """R-MAT scale 16, IC U[0,0.1]: oracle parity on a sample of ids."""
+    index, edges, w = rmat_csr(16, R.W_UNIFORM, 0.1)
+    g = R.Graph(index, edges, w, model=R.IC)
+    p = g.pool(0)
+    p.generate_rrr_sets(20000)
+    off, ver = p.export()
+    sizes = np.diff(off)
+    assert sizes.min() >= 1
+    exp = orc.sample(R.IC, index, edges, w, 0, 17000, 3000)
+    assert_same_sets(p.export(17000, 3000), exp)
+    p.close()
+    g.close()
