@@ -66,7 +66,7 @@ def cpu_baseline(wl, index, edges, w, seed, batch, target_s):
     t = time.time()
     O.sample(wl["model"], index, edges, w, seed, 0, probe, threads=threads)
     dt = max(time.time() - t, 1e-3)
-    count = int(min(batch, max(probe, probe * target_s / dt)))
+    count = int(max(probe, probe * target_s / dt))
     t = time.time()
     O.sample(wl["model"], index, edges, w, seed, 0, count, threads=threads)
     dt = time.time() - t
@@ -87,7 +87,12 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="sets per step (0 = theta)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--tune", default="", help='sampler tuning, e.g. "colors=1,block=256"')
+    ap.add_argument("--skip-imm", action="store_true")
     a = ap.parse_args()
+    tune = None
+    if a.tune:
+        tune = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in a.tune.split(",")}
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -100,26 +105,30 @@ def main():
     g = R.Graph(index, edges, w, model=wl["model"])
 
     # IMM wall time: one warm-up run, one timed run
-    seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=a.seed)
-    t = time.time()
-    seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=a.seed)
-    imm_wall = time.time() - t
-    log("IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
-        rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
+    imm_wall, rec = None, None
+    if not a.skip_imm:
+        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=a.seed, tune=tune)
+        t = time.time()
+        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=a.seed, tune=tune)
+        imm_wall = time.time() - t
+        log("IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
+            rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
 
-    batch = a.batch or int(rec["Theta"])
+    batch = a.batch or (int(rec["Theta"]) if rec else 1 << 20)
     pool = g.pool(a.seed)
     for _ in range(a.warmup):
         pool.reset()
-        pool.generate_rrr_sets(batch)
+        pool.generate_rrr_sets(batch, tune=tune)
     kms, algo = [], []
     t0 = time.time()
     for _ in range(a.steps):
         pool.reset()
-        pool.generate_rrr_sets(batch)      # returns after the device finished
+        pool.generate_rrr_sets(batch, tune=tune)   # returns after the device finished
         kms.append(pool.last_kernel_ms()[0])
         algo.append(pool.last_algo_bytes())
     wall = time.time() - t0
+    stats = pool.last_stats()
+    log("kernel counters (last step): %s" % stats)
     value = batch * a.steps / wall
     n_sets, n_elem = pool.size()
     if wl["model"] == R.IC:
@@ -150,11 +159,14 @@ def main():
                    "batch_sets": batch, "k": wl["k"], "epsilon": wl["eps"],
                    "parallelism": "single GPU"},
         "imm_wall_s": imm_wall,
-        "imm": {"theta": rec["Theta"], "rounds": rec["Rounds"], "num_seeds": rec["NumSeeds"],
-                "final_f": rec["FinalF"], "estimation_ms": rec["ThetaEstimation"],
-                "generate_ms": rec["GenerateRRRSets"],
-                "select_ms": rec["FindMostInfluentialSet"]},
+        "imm": None if rec is None else {
+            "theta": rec["Theta"], "rounds": rec["Rounds"], "num_seeds": rec["NumSeeds"],
+            "final_f": rec["FinalF"], "estimation_ms": rec["ThetaEstimation"],
+            "generate_ms": rec["GenerateRRRSets"],
+            "select_ms": rec["FindMostInfluentialSet"]},
         "pool": {"sets": n_sets, "elements": n_elem},
+        "kernel_counters": stats,
+        "tune": tune,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "lt_walk_kernel" if wl["model"] == R.LT else "ic_fused_kernel",

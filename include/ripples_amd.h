@@ -133,11 +133,24 @@ int rip_pool_count(rip_pool *p, uint64_t num_sets, uint32_t *counts);
 /* Device time of the last rip_pool_sample call's sampling kernels, measured
  * with HIP events on the pool's stream, and the number of kernel launches. */
 int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
-/* Algorithmic bytes the last LT sampling launch read and wrote:
- * sum over walks of 8 per visited vertex (its CSR offsets), 8 per scanned
- * in-edge (id + weight) and 4 per output id.  0 for IC pools, whose
- * per-launch figure depends on the union of the sets (computed by callers). */
-int rip_pool_last_stats(const rip_pool *p, uint64_t *algo_bytes);
+/* Work counters of the last rip_pool_sample call's main kernel, written to
+ * stats[0 .. count) in this order (no counterpart in the reference; used for
+ * roofline accounting):
+ *   RIP_STAT_ALGO_BYTES  LT: sum over walks of 8 per visited vertex (its CSR
+ *                        offsets), 8 per scanned in-edge (id + weight) and
+ *                        4 per output id; 0 for IC, whose per-launch figure
+ *                        depends on the union of the sets
+ *   RIP_STAT_EDGES       in-edges read
+ *   RIP_STAT_COINS       IC: (edge, sample) coin evaluations
+ *   RIP_STAT_EXPANSIONS  IC: frontier entries expanded (vertex, colour group)
+ *   RIP_STAT_LEVELS      IC: BFS levels summed over colour groups */
+#define RIP_STAT_ALGO_BYTES 0
+#define RIP_STAT_EDGES 1
+#define RIP_STAT_COINS 2
+#define RIP_STAT_EXPANSIONS 3
+#define RIP_STAT_LEVELS 4
+#define RIP_STAT_COUNT 5
+int rip_pool_last_stats(const rip_pool *p, uint64_t *stats, uint32_t count);
 
 /* ---------------------------------------------------------------- IMM */
 /* The IMM algorithm -- IMM(G, CFG, l=1, gen, record, model, omp) of

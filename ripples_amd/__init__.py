@@ -101,7 +101,7 @@ def lib():
     L.rip_pool_count.argtypes = [vp, C.c_uint64, _u32p]
     L.rip_pool_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double),
                                           C.POINTER(C.c_uint32)]
-    L.rip_pool_last_stats.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.rip_pool_last_stats.argtypes = [vp, _u64p, C.c_uint32]
     L.rip_imm.argtypes = [vp, C.c_uint32, C.c_double, C.c_uint64, C.POINTER(Tune),
                           _u32p, C.POINTER(ImmRecord)]
     L.rip_rmat_edges.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double,
@@ -273,10 +273,16 @@ class Pool:
         _check(lib().rip_pool_last_kernel_ms(self._h, C.byref(ms), C.byref(launches)))
         return ms.value, launches.value
 
+    STAT_NAMES = ("algo_bytes", "edges", "coins", "expansions", "levels")
+
+    def last_stats(self):
+        """Work counters of the last sampling call's main kernel."""
+        out = np.zeros(len(self.STAT_NAMES), np.uint64)
+        _check(lib().rip_pool_last_stats(self._h, out, len(out)))
+        return {k: int(v) for k, v in zip(self.STAT_NAMES, out)}
+
     def last_algo_bytes(self):
-        b = C.c_uint64(0)
-        _check(lib().rip_pool_last_stats(self._h, C.byref(b)))
-        return b.value
+        return self.last_stats()["algo_bytes"]
 
 
 def imm(graph, k, epsilon, seed=0, tune=None):

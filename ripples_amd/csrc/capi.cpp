@@ -184,10 +184,12 @@ int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches) {
   });
 }
 
-int rip_pool_last_stats(const rip_pool *p, uint64_t *algo_bytes) {
+int rip_pool_last_stats(const rip_pool *p, uint64_t *stats, uint32_t count) {
   return guarded([&] {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
-    if (algo_bytes) *algo_bytes = p->last_algo_bytes;
+    RIP_REQUIRE(count == 0 || stats, RIP_EINVAL, "stats is NULL");
+    for (uint32_t i = 0; i < count; ++i)
+      stats[i] = i < RIP_STAT_COUNT ? p->last_stats[i] : 0;
   });
 }
 

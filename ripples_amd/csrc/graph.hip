@@ -47,7 +47,8 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
     uint64_t d = index[v + 1] - index[v];
     if (d > maxdeg) maxdeg = static_cast<uint32_t>(d);
   }
-  std::vector<uint2> adj(m);
+  // two zero edges of padding: 16-byte loads read edges in aligned pairs
+  std::vector<uint2> adj(m + 2, make_uint2(0, 0));
   for (uint64_t e = 0; e < m; ++e) {
     RIP_REQUIRE(edges[e] < n, -1, "graph: edge endpoint out of range");
     uint32_t y;
@@ -76,14 +77,13 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
   unsigned long long *d_index = nullptr;
   try {
     g->d_vrec = dev_alloc<VertexRec>(n);
-    g->d_adj = dev_alloc<uint2>(m);
+    g->d_adj = dev_alloc<uint2>(m + 2);
     g->d_jtab = dev_alloc<Affine>(jt.size());
     d_index = dev_alloc<unsigned long long>(static_cast<size_t>(n) + 1);
     RIP_HIP(hipMemcpy(d_index, index, sizeof(uint64_t) * (static_cast<size_t>(n) + 1),
                       hipMemcpyHostToDevice));
-    if (m)
-      RIP_HIP(hipMemcpy(g->d_adj, adj.data(), sizeof(uint2) * m,
-                        hipMemcpyHostToDevice));
+    RIP_HIP(hipMemcpy(g->d_adj, adj.data(), sizeof(uint2) * (m + 2),
+                      hipMemcpyHostToDevice));
     RIP_HIP(hipMemcpy(g->d_jtab, jt.data(), sizeof(Affine) * jt.size(),
                       hipMemcpyHostToDevice));
     dim3 grid((n + 255) / 256);

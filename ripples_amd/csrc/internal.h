@@ -36,7 +36,9 @@ struct IcSlots {
 
 struct LtScratch {
   uint64_t cap_ids = 0;
-  uint32_t *d_long_ids = nullptr;      // walks that outgrew the LDS list
+  uint32_t *d_long_ids = nullptr;      // walks that outgrew the LDS + spill list
+  uint64_t cap_spill = 0;
+  uint32_t *d_spill = nullptr;         // per-lane visited-list spill area
 };
 
 struct SelectScratch {
@@ -71,7 +73,7 @@ struct rip_pool {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_kernel_ms = 0.0;
   uint32_t last_launches = 0;
-  uint64_t last_algo_bytes = 0;
+  uint64_t last_stats[RIP_STAT_COUNT] = {};
   double avg_set_size = 8.0;            // running estimate for capacity planning
 };
 
@@ -84,7 +86,11 @@ enum : int {
   kCtlOverflow = 2,      // nonzero when d_data ran out
   kCtlLongCount = 3,     // LT: walks that outgrew the per-lane list
   kCtlCovered = 4,       // selector: sets covered so far
-  kCtlStatBytes = 5,     // LT: algorithmic bytes of the last sampler launch
+  kCtlStatBytes = 5,     // stats of the last sampler launch (RIP_STAT_* order)
+  kCtlStatEdges = 6,
+  kCtlStatCoins = 7,
+  kCtlStatExpansions = 8,
+  kCtlStatLevels = 9,
   kCtlWords = 16,
 };
 

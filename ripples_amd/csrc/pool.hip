@@ -88,7 +88,7 @@ __global__ void set_ctl_kernel(unsigned long long *ctl, unsigned long long bump)
   ctl[kCtlDataBump] = bump;
   ctl[kCtlOverflow] = 0;
   ctl[kCtlGroupCounter] = 0;
-  ctl[kCtlStatBytes] = 0;
+  for (int i = kCtlStatBytes; i <= kCtlStatLevels; ++i) ctl[i] = 0;
 }
 
 // Copy sets [first, first+count) into CSR order at out (prefix from scan).
@@ -171,6 +171,7 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->ic.d_bm1);
   dev_free(p->ic.d_queue);
   dev_free(p->lt.d_long_ids);
+  dev_free(p->lt.d_spill);
   dev_free(p->sel.d_counts);
   dev_free(p->sel.d_inv_start);
   dev_free(p->sel.d_cursor);
@@ -251,7 +252,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
       RIP_HIP(hipEventElapsedTime(&ms, p->ev0, p->ev1));
       p->last_kernel_ms = ms;
       p->last_launches = launches;
-      p->last_algo_bytes = ctl[kCtlStatBytes];
+      for (int i = 0; i < RIP_STAT_COUNT; ++i) p->last_stats[i] = ctl[kCtlStatBytes + i];
       const uint64_t added = ctl[kCtlDataBump] - p->used_data;
       p->used_data = ctl[kCtlDataBump];
       p->avg_set_size = std::max(1.0, static_cast<double>(added) / static_cast<double>(count));

@@ -119,6 +119,8 @@ ic_fused_kernel(IcParams P) {
   uint32_t *list = qbuf[0];
   unsigned long long *masks =
       reinterpret_cast<unsigned long long *>(qbuf[1]);
+  // work counters (edges read, coins, expansions, levels)
+  unsigned long long st_edges = 0, st_coins = 0, st_exp = 0, st_levels = 0;
 
   for (;;) {
     if (tid == 0) s_group = atomicAdd(&P.ctl[kCtlGroupCounter], 1ull);
@@ -156,6 +158,7 @@ ic_fused_kernel(IcParams P) {
       const int nxt = cur ^ 1;
       if (tid == 0) s_qn[nxt] = 0;
       if (qn == 0) break;
+      if (tid == 0) ++st_levels;
       const uint32_t *qcur = qbuf[cur];
       uint32_t *qnext = qbuf[nxt];
       for (uint32_t cb = 0; cb < qn; cb += BT) {
@@ -172,6 +175,11 @@ ic_fused_kernel(IcParams P) {
           s_eb[tid] = r.jb;
           s_em[tid] = M;
           runs = M ? (deg + RUN - 1) / RUN : 0;
+          if (M) {
+            st_edges += deg;
+            st_coins += static_cast<unsigned long long>(deg) * __popcll(M);
+            ++st_exp;
+          }
         }
         uint32_t total;
         const uint32_t pre = block_exclusive_scan<BT>(runs, s_tmp, &total);
@@ -300,6 +308,18 @@ ic_fused_kernel(IcParams P) {
       P.set_len[P.first + s0 + tid] = 0xFFFFFFFFu;
     }
     __syncthreads();
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    st_edges += __shfl_down(st_edges, d, 64);
+    st_coins += __shfl_down(st_coins, d, 64);
+    st_exp += __shfl_down(st_exp, d, 64);
+    st_levels += __shfl_down(st_levels, d, 64);
+  }
+  if (lane == 0) {
+    if (st_edges) atomicAdd(&P.ctl[kCtlStatEdges], st_edges);
+    if (st_coins) atomicAdd(&P.ctl[kCtlStatCoins], st_coins);
+    if (st_exp) atomicAdd(&P.ctl[kCtlStatExpansions], st_exp);
+    if (st_levels) atomicAdd(&P.ctl[kCtlStatLevels], st_levels);
   }
 }
 

@@ -38,83 +38,187 @@ struct LtParams {
   uint64_t cap_data;
   unsigned long long *ctl;
   uint32_t *long_ids;
+  uint32_t *spill;
 };
 
-template <int BT, int CAP>
+// Persistent walkers: every lane runs its own walk and, when it ends, claims
+// the next sample id from a device counter (one atomic per wave and round),
+// so short walks never wait for the long ones of the same wave.  Each round
+// scans up to UNROLL in-edges of the lane's current row with 16-byte loads
+// (two edges each) issued together, then reduces the float threshold in CSR
+// order.  A walk's visited list lives in LDS for its first CAP vertices and
+// in a per-lane HBM spill area for the next SPILL; longer walks are handed to
+// lt_long_kernel.
+template <int BT, int CAP, int SPILL, int UNROLL>
 __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
+  static_assert(UNROLL % 2 == 0, "pairs of edges per load");
   __shared__ uint32_t s_list[CAP * BT];
-  const uint32_t tid = threadIdx.x;
-  const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * BT + tid;
-  const bool active = gid < P.count;
-  const uint64_t s = P.first + gid;
-  uint32_t len = 0;
-  bool is_long = false;
-  unsigned long long bytes = 0;  // algorithmic bytes: 8 per vertex + 8 per scanned edge
-  if (active) {
-    const uint64_t z = sample_z(P.seed, s);
-    uint32_t v = uniform_index(z, P.n);
-    s_list[tid] = v;
-    len = 1;
-    for (;;) {
-      const VertexRec r = P.vrec[v];
-      float t = u01(r.ja * z + r.jb);
-      uint32_t next = kLenFail;
-      bytes += 8;
-      for (uint32_t e = r.begin; e < r.end; ++e) {
-        const uint2 a = P.adj[e];
-        bytes += 8;
-        t -= __uint_as_float(a.y);
-        if (t > 0) continue;
-        bool seen = false;
-        for (uint32_t i = 0; i < len; ++i)
-          if (s_list[i * BT + tid] == a.x) { seen = true; break; }
-        if (!seen) next = a.x;
-        break;
-      }
-      if (next == kLenFail) break;
-      if (len == CAP) { is_long = true; break; }
-      s_list[len * BT + tid] = next;
-      ++len;
-      v = next;
-    }
-    if (!is_long) {
-      for (uint32_t i = 1; i < len; ++i) {
-        const uint32_t key = s_list[i * BT + tid];
-        uint32_t j = i;
-        while (j > 0 && s_list[(j - 1) * BT + tid] > key) {
-          s_list[j * BT + tid] = s_list[(j - 1) * BT + tid];
-          --j;
+  uint32_t *list = s_list + threadIdx.x;  // element i < CAP at list[i * BT]
+  uint32_t *spill = P.spill + (static_cast<uint64_t>(blockIdx.x) * BT + threadIdx.x) * SPILL;
+  auto at = [&](uint32_t i) -> uint32_t & {
+    return i < static_cast<uint32_t>(CAP) ? list[i * BT] : spill[i - CAP];
+  };
+  const uint4 *adj4 = reinterpret_cast<const uint4 *>(P.adj);
+  const uint32_t lane = lane_id();
+  bool busy = false, exhausted = false;
+  uint64_t gid = 0, z = 0;
+  uint32_t len = 0, e = 0, end = 0;
+  float t = 0.0f;
+  unsigned long long bytes = 0;  // 8 per visited vertex, 8 per scanned edge, 4 per output
+  unsigned long long edges = 0;
+
+  for (;;) {
+    // ---- refill idle lanes
+    const bool want = !busy && !exhausted;
+    const uint64_t wm = __ballot(want);
+    if (wm) {
+      const uint32_t leader = __ffsll(static_cast<unsigned long long>(wm)) - 1;
+      unsigned long long base = 0;
+      if (lane == leader)
+        base = atomicAdd(&P.ctl[kCtlGroupCounter], static_cast<unsigned long long>(__popcll(wm)));
+      base = __shfl(base, static_cast<int>(leader), 64);
+      if (want) {
+        gid = base + __popcll(wm & lanes_below());
+        if (gid < P.count) {
+          z = sample_z(P.seed, P.first + gid);
+          const uint32_t root = uniform_index(z, P.n);
+          list[0] = root;
+          len = 1;
+          const VertexRec r = P.vrec[root];
+          t = u01(r.ja * z + r.jb);
+          e = r.begin;
+          end = r.end;
+          bytes += 8;
+          busy = true;
+        } else {
+          exhausted = true;
         }
-        s_list[j * BT + tid] = key;
+      }
+    }
+    if (__ballot(busy) == 0) break;
+
+    // ---- one scan round
+    bool done = false, is_long = false;
+    if (busy) {
+      if (e >= end) {
+        done = true;  // no in-edge crosses the threshold: the walk stops
+      } else {
+        const uint32_t base = e & ~1u;      // pair-aligned start
+        const uint32_t last_pair = (end - 1) >> 1;
+        uint4 q[UNROLL / 2];
+#pragma unroll
+        for (int kk = 0; kk < UNROLL / 2; ++kk)
+          q[kk] = adj4[min((base >> 1) + kk, last_pair)];
+        const uint32_t lo = e - base;       // 0 or 1
+        const uint32_t hi = min(static_cast<uint32_t>(UNROLL), end - base);
+        bool found = false;
+        uint32_t u = 0, stop = hi;
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+          const uint32_t wbits = (k & 1) ? q[k >> 1].w : q[k >> 1].y;
+          const uint32_t src = (k & 1) ? q[k >> 1].z : q[k >> 1].x;
+          if (!found && static_cast<uint32_t>(k) >= lo && static_cast<uint32_t>(k) < hi) {
+            t -= __uint_as_float(wbits);
+            if (!(t > 0)) {
+              found = true;
+              u = src;
+              stop = k + 1;
+            }
+          }
+        }
+        const uint32_t scanned = stop - lo;
+        bytes += 8ull * scanned;
+        edges += scanned;
+        e = base + stop;
+        if (found) {
+          bool seen = false;
+          for (uint32_t i = 0; i < len; ++i)
+            if (at(i) == u) { seen = true; break; }
+          if (seen) {
+            done = true;
+          } else if (len == CAP + SPILL) {
+            done = true;
+            is_long = true;
+          } else {
+            at(len) = u;
+            ++len;
+            const VertexRec r = P.vrec[u];
+            t = u01(r.ja * z + r.jb);
+            e = r.begin;
+            end = r.end;
+            bytes += 8;
+          }
+        } else if (e >= end) {
+          done = true;
+        }
+      }
+    }
+
+    // ---- retire finished walks: sort, reserve pool space per wave, write
+    if (__ballot(done)) {
+      const bool emit = done && !is_long;
+      if (emit) {
+        for (uint32_t i = 1; i < len; ++i) {
+          const uint32_t key = at(i);
+          uint32_t j = i;
+          while (j > 0 && at(j - 1) > key) {
+            at(j) = at(j - 1);
+            --j;
+          }
+          at(j) = key;
+        }
+      }
+      const uint32_t size = emit ? len : 0;
+      const uint32_t incl = wave_inclusive_scan(size);
+      const uint32_t total = __shfl(incl, 63, 64);
+      unsigned long long base = 0;
+      if (lane == 0 && total)
+        base = atomicAdd(&P.ctl[kCtlDataBump], static_cast<unsigned long long>(total));
+      base = __shfl(base, 0, 64);
+      const bool fail = total && base + total > P.cap_data;
+      if (fail && lane == 0) atomicOr(&P.ctl[kCtlOverflow], 1ull);
+      if (done) {
+        const uint64_t s = P.first + gid;
+        if (is_long) {
+          const unsigned long long idx = atomicAdd(&P.ctl[kCtlLongCount], 1ull);
+          P.long_ids[idx] = static_cast<uint32_t>(gid);
+          P.set_len[s] = kLenLong;
+        } else if (fail) {
+          P.set_len[s] = kLenFail;
+        } else {
+          const unsigned long long off = base + incl - size;
+          for (uint32_t i = 0; i < len; ++i) P.data[off + i] = at(i);
+          P.set_off[s] = off;
+          P.set_len[s] = len;
+          bytes += 4ull * len;
+        }
+        busy = false;
+        len = 0;
       }
     }
   }
-  // one pool reservation per wave
-  const uint32_t size = (active && !is_long) ? len : 0;
-  const uint32_t incl = wave_inclusive_scan(size);
-  const uint32_t total = __shfl(incl, 63, 64);
-  unsigned long long base = 0;
-  if (lane_id() == 0 && total)
-    base = atomicAdd(&P.ctl[kCtlDataBump], static_cast<unsigned long long>(total));
-  base = __shfl(base, 0, 64);
-  const bool fail = total && base + total > P.cap_data;
-  if (fail && lane_id() == 0) atomicOr(&P.ctl[kCtlOverflow], 1ull);
-  bytes += 4ull * size;
-  for (int d = 32; d > 0; d >>= 1) bytes += __shfl_down(bytes, d, 64);
-  if (lane_id() == 0 && bytes) atomicAdd(&P.ctl[kCtlStatBytes], bytes);
-  if (!active) return;
-  if (is_long) {
-    const unsigned long long idx = atomicAdd(&P.ctl[kCtlLongCount], 1ull);
-    P.long_ids[idx] = static_cast<uint32_t>(gid);
-    P.set_len[s] = kLenLong;
-  } else if (fail) {
-    P.set_len[s] = kLenFail;
-  } else {
-    const unsigned long long off = base + incl - size;
-    for (uint32_t i = 0; i < len; ++i) P.data[off + i] = s_list[i * BT + tid];
-    P.set_off[s] = off;
-    P.set_len[s] = len;
+  for (int d = 32; d > 0; d >>= 1) {
+    bytes += __shfl_down(bytes, d, 64);
+    edges += __shfl_down(edges, d, 64);
   }
+  if (lane == 0 && bytes) atomicAdd(&P.ctl[kCtlStatBytes], bytes);
+  if (lane == 0 && edges) atomicAdd(&P.ctl[kCtlStatEdges], edges);
+}
+
+template <int BT, int CAP, int SPILL, int UNROLL>
+uint32_t lt_grid(uint64_t count) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(&lt_walk_kernel<BT, CAP, SPILL, UNROLL>), BT, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  int dev = 0, cus = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const uint64_t resident = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(cus);
+  const uint64_t needed = (count + BT - 1) / BT;
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min(resident, needed)));
 }
 
 // Unbounded walks: one lane per walk, visited list + bitmap in HBM.
@@ -191,7 +295,7 @@ __global__ void reset_long_kernel(unsigned long long *ctl) {
 uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
                    const rip_tune *tune) {
   (void)tune;
-  constexpr int BT = 128, CAP = 64;
+  constexpr int BT = 256, CAP = 32, SPILL = 96, UNROLL = 32;
   const rip_graph *g = p->g;
   LtParams P;
   P.vrec = g->d_vrec;
@@ -213,10 +317,17 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   }
   P.long_ids = p->lt.d_long_ids;
   hipLaunchKernelGGL(reset_long_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
-  const uint64_t blocks = (count + BT - 1) / BT;
+  const uint32_t blocks = lt_grid<BT, CAP, SPILL, UNROLL>(count);
+  const uint64_t spill_words = static_cast<uint64_t>(blocks) * BT * SPILL;
+  if (p->lt.cap_spill < spill_words) {
+    dev_free(p->lt.d_spill);
+    p->lt.d_spill = dev_alloc<uint32_t>(spill_words);
+    p->lt.cap_spill = spill_words;
+  }
+  P.spill = p->lt.d_spill;
   RIP_HIP(hipEventRecord(p->ev0, p->stream));
-  hipLaunchKernelGGL((lt_walk_kernel<BT, CAP>), dim3(static_cast<uint32_t>(blocks)),
-                     dim3(BT), 0, p->stream, P);
+  hipLaunchKernelGGL((lt_walk_kernel<BT, CAP, SPILL, UNROLL>), dim3(blocks), dim3(BT), 0,
+                     p->stream, P);
   RIP_HIP(hipGetLastError());
   RIP_HIP(hipEventRecord(p->ev1, p->stream));
   uint32_t launches = 1;
@@ -228,7 +339,7 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
     // lanes of unbounded walks: a visited list (n words) and bitmap each
     const uint64_t per_lane = 4ull * g->n + 4ull * ((g->n + 31) / 32);
     const uint64_t lanes = std::max<uint64_t>(
-        1, std::min<uint64_t>(nlong, (static_cast<uint64_t>(1) << 30) / per_lane));
+        1, std::min<uint64_t>({nlong, 256, (static_cast<uint64_t>(1) << 29) / per_lane}));
     uint32_t *lists = dev_alloc<uint32_t>(lanes * g->n);
     uint32_t *bms = dev_alloc<uint32_t>(lanes * ((g->n + 31) / 32));
     RIP_HIP(hipMemsetAsync(bms, 0, sizeof(uint32_t) * lanes * ((g->n + 31) / 32),
