@@ -46,11 +46,21 @@ typedef struct rip_pool rip_pool;
 /* Sampler tuning (gpu_batch_size / streaming engine knobs of
  * include/ripples/imm_configuration.h:64-109).  Zero fields = automatic. */
 typedef struct {
-  uint32_t colors;     /* IC: samples fused per colour group, 1..64 */
-  uint32_t block;      /* threads per workgroup: 256, 512 or 1024 */
-  uint32_t slots;      /* resident colour groups (persistent workgroups) */
-  uint32_t edge_run;   /* IC: consecutive edges per lane, 1..8 */
+  uint32_t colors;       /* IC: samples fused per colour group, 1..64 */
+  uint32_t block;        /* IC: threads per workgroup: 256, 512 or 1024 */
+  uint32_t slots;        /* IC: resident colour groups (persistent workgroups) */
+  uint32_t edge_run;     /* IC: consecutive in-edges per lane: 4, 8 or 16 */
+  uint32_t heavy_degree; /* IC: rows this long wait for the hub phase */
+  uint32_t sampler;      /* IC: RIP_SAMPLER_AUTO, _WAVE or _COLOUR */
+  uint32_t reserved[2];  /* must be zero */
 } rip_tune;
+
+/* IC sampler kinds: one reverse BFS per wave with colour-group fallback for
+ * large sets (AUTO, WAVE), or 64-sample colour groups for every sample
+ * (COLOUR).  All give the same sets. */
+#define RIP_SAMPLER_AUTO 0
+#define RIP_SAMPLER_WAVE 1
+#define RIP_SAMPLER_COLOUR 2
 
 /* Timing and size record of one IMM run; times in milliseconds.  Mirrors
  * IMMExecutionRecord (include/ripples/imm_execution_record.h:55-92). */

@@ -2,8 +2,8 @@
 // traversals of the samplers:
 //   d_vrec[v] : 32-byte record {P^(1+begin) jump, begin, end} -- one aligned
 //               load gives a vertex's row and the jump to its first coin
-//   d_adj[e]  : (in-neighbour, IC integer threshold | LT float weight bits)
-//   d_jtab[j] : P^j, used to jump inside long rows
+//   d_adj[e]  : (in-neighbour, IC integer threshold | LT float weight bits),
+//               padded with two zero edges for pairwise 16-byte loads
 #include <cstring>
 #include <vector>
 
@@ -42,10 +42,12 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
   RIP_REQUIRE(index[0] == 0 && index[n] == m, -1,
               "graph: index[0] must be 0 and index[n] the edge count");
   uint32_t maxdeg = 0;
+  uint64_t deg_ge[33] = {};
   for (uint32_t v = 0; v < n; ++v) {
     RIP_REQUIRE(index[v + 1] >= index[v], -1, "graph: index is not monotone");
     uint64_t d = index[v + 1] - index[v];
     if (d > maxdeg) maxdeg = static_cast<uint32_t>(d);
+    for (int b = 0; b < 33 && (1ull << b) <= d; ++b) ++deg_ge[b];
   }
   // two zero edges of padding: 16-byte loads read edges in aligned pairs
   std::vector<uint2> adj(m + 2, make_uint2(0, 0));
@@ -70,21 +72,16 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
   g->m = m;
   g->max_degree = maxdeg;
   g->step = sample_step();
-  std::vector<Affine> jt(static_cast<size_t>(maxdeg) + 1);
-  jt[0] = Affine{1, 0};
-  for (uint32_t j = 1; j <= maxdeg; ++j) jt[j] = aff_then(jt[j - 1], g->step);
+  for (int b = 0; b < 33; ++b) g->deg_ge_pow2[b] = deg_ge[b];
 
   unsigned long long *d_index = nullptr;
   try {
     g->d_vrec = dev_alloc<VertexRec>(n);
     g->d_adj = dev_alloc<uint2>(m + 2);
-    g->d_jtab = dev_alloc<Affine>(jt.size());
     d_index = dev_alloc<unsigned long long>(static_cast<size_t>(n) + 1);
     RIP_HIP(hipMemcpy(d_index, index, sizeof(uint64_t) * (static_cast<size_t>(n) + 1),
                       hipMemcpyHostToDevice));
     RIP_HIP(hipMemcpy(g->d_adj, adj.data(), sizeof(uint2) * (m + 2),
-                      hipMemcpyHostToDevice));
-    RIP_HIP(hipMemcpy(g->d_jtab, jt.data(), sizeof(Affine) * jt.size(),
                       hipMemcpyHostToDevice));
     dim3 grid((n + 255) / 256);
     hipLaunchKernelGGL(build_vrec_kernel, grid, dim3(256), 0, 0, d_index, n,
@@ -104,7 +101,17 @@ void graph_destroy(rip_graph *g) {
   (void)hipSetDevice(g->device);
   dev_free(g->d_vrec);
   dev_free(g->d_adj);
-  dev_free(g->d_jtab);
+  dev_free(g->ic.d_vp);
+  dev_free(g->ic.d_bm0);
+  dev_free(g->ic.d_bm1);
+  dev_free(g->ic.d_queue);
+  dev_free(g->ic.d_hub);
+  dev_free(g->wave.d_table);
+  dev_free(g->wave.d_list);
+  dev_free(g->wave.d_epoch);
+  dev_free(g->wave.d_big);
+  dev_free(g->lt.d_long_ids);
+  dev_free(g->lt.d_spill);
 }
 
 }  // namespace rip

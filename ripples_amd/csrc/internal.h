@@ -4,22 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
 #include "ripples_amd.h"
-
-struct rip_graph {
-  int device = 0;
-  int model = RIP_IC;
-  uint32_t n = 0;
-  uint64_t m = 0;
-  uint32_t max_degree = 0;
-  rip::Affine step{1, 0};           // one step of a sample stream
-  rip::VertexRec *d_vrec = nullptr;  // n records
-  uint2 *d_adj = nullptr;            // m x (source, IC threshold | LT weight bits)
-  rip::Affine *d_jtab = nullptr;     // step^j for j in [0, max_degree]
-};
 
 namespace rip {
 
@@ -32,6 +21,19 @@ struct IcSlots {
   uint32_t *d_bm0 = nullptr;           // slots x ceil(n/32)   union bitmap
   uint32_t *d_bm1 = nullptr;           // slots x ceil(n/1024) bitmap summary
   uint32_t *d_queue = nullptr;         // slots x 3n words: frontier queues / output staging
+  uint32_t *d_hub = nullptr;           // slots x 2 x hub_cap deferred-hub lists
+  uint64_t hub_cap = 0;
+};
+
+// Scratch of the wave IC sampler: per resident wave a visited table and a
+// BFS list; samples too large for them are handed to the colour sampler.
+struct WaveScratch {
+  uint64_t waves = 0;
+  unsigned long long *d_table = nullptr;  // waves x table capacity
+  uint32_t *d_list = nullptr;             // waves x list capacity
+  uint32_t *d_epoch = nullptr;            // waves
+  uint64_t cap_big = 0;
+  uint32_t *d_big = nullptr;              // sample indices left over
 };
 
 struct LtScratch {
@@ -55,6 +57,24 @@ struct SelectScratch {
 
 }  // namespace rip
 
+struct rip_graph {
+  int device = 0;
+  int model = RIP_IC;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  uint32_t max_degree = 0;
+  rip::Affine step{1, 0};           // one step of a sample stream
+  rip::VertexRec *d_vrec = nullptr;  // n records
+  uint2 *d_adj = nullptr;            // m x (source, IC threshold | LT weight bits)
+  uint64_t deg_ge_pow2[33] = {};     // vertices with in-degree >= 2^b
+  // Sampler scratch shared by every pool of this graph; pool_sample holds
+  // sampler_mu while a launch uses it.
+  std::mutex sampler_mu;
+  rip::IcSlots ic;
+  rip::WaveScratch wave;
+  rip::LtScratch lt;
+};
+
 struct rip_pool {
   rip_graph *g = nullptr;
   uint64_t seed = 0;
@@ -67,8 +87,6 @@ struct rip_pool {
   uint64_t cap_data = 0;
   uint64_t used_data = 0;
   unsigned long long *d_ctl = nullptr;  // control words, see pool.hip
-  rip::IcSlots ic;
-  rip::LtScratch lt;
   rip::SelectScratch sel;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_kernel_ms = 0.0;
@@ -113,12 +131,21 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count,
 
 // sample_ic.hip
 struct IcLaunch {
-  uint32_t colors, block, slots, edge_run;
+  uint32_t colors, block, slots, edge_run, heavy_degree, sampler;
 };
 IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune);
-void ic_ensure_slots(rip_pool *p, uint32_t slots);
+void ic_ensure_slots(rip_pool *p, uint32_t slots, uint32_t heavy_deg);
+// Colour sampler over samples first + i, i in [0, count), or first + ids[i].
 uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
-                   const IcLaunch &cfg);
+                   const IcLaunch &cfg, const uint32_t *ids = nullptr,
+                   bool record_events = true);
+
+// IC sampling of samples [first, first+count) with the configured sampler
+uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &cfg);
+
+// sample_ic_wave.hip: returns how many samples were left for the colour
+// sampler; their indices (relative to first) are in g->wave.d_big
+uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count);
 
 // sample_lt.hip
 uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,

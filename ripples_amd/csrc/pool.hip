@@ -88,6 +88,7 @@ __global__ void set_ctl_kernel(unsigned long long *ctl, unsigned long long bump)
   ctl[kCtlDataBump] = bump;
   ctl[kCtlOverflow] = 0;
   ctl[kCtlGroupCounter] = 0;
+  ctl[kCtlLongCount] = 0;
   for (int i = kCtlStatBytes; i <= kCtlStatLevels; ++i) ctl[i] = 0;
 }
 
@@ -166,12 +167,6 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->d_len);
   dev_free(p->d_data);
   dev_free(p->d_ctl);
-  dev_free(p->ic.d_vp);
-  dev_free(p->ic.d_bm0);
-  dev_free(p->ic.d_bm1);
-  dev_free(p->ic.d_queue);
-  dev_free(p->lt.d_long_ids);
-  dev_free(p->lt.d_spill);
   dev_free(p->sel.d_counts);
   dev_free(p->sel.d_inv_start);
   dev_free(p->sel.d_cursor);
@@ -234,6 +229,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
   IcLaunch ic{};
   if (p->g->model == RIP_IC) ic = ic_tune(p, tune);
   const uint64_t first = p->num_sets;
+  std::lock_guard<std::mutex> hold(p->g->sampler_mu);  // graph-owned sampler scratch
   for (int attempt = 0;; ++attempt) {
     RIP_REQUIRE(attempt < 40, -2, "pool: data capacity did not converge");
     hipLaunchKernelGGL(set_ctl_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl,
@@ -241,7 +237,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
     // the launchers bracket their main kernel with ev0 / ev1
     uint32_t launches;
     if (p->g->model == RIP_IC)
-      launches = ic_launch(p, first, count, ic);
+      launches = ic_sample(p, first, count, ic);
     else
       launches = lt_launch(p, first, count, tune);
     unsigned long long ctl[kCtlWords];

@@ -40,7 +40,6 @@ namespace {
 struct IcParams {
   const VertexRec *vrec;
   const uint2 *adj;
-  const Affine *jtab;
   uint32_t n;
   Affine step;
   uint64_t seed;
@@ -59,7 +58,18 @@ struct IcParams {
   uint32_t *data;
   uint64_t cap_data;
   unsigned long long *ctl;
+  uint32_t heavy_deg;   // rows at least this long wait for the hub phase
+  uint32_t *hub;        // slots x 2 x hub_cap deferred-hub lists
+  uint64_t hub_cap;
+  uint32_t jlevels;     // 6-bit digits of the run index covered by the jump tables
+  const uint32_t *ids;  // optional: local index -> sample index (minus first)
 };
+
+__device__ inline uint64_t sample_id(const IcParams &P, uint64_t local) {
+  return P.first + (P.ids ? P.ids[local] : local);
+}
+
+constexpr int kJumpLevels = 5;
 
 // Mark (u, bits) discovered.  Every lane of the wave must call it.
 __device__ inline void discover(const IcParams &P, unsigned long long *vp,
@@ -97,8 +107,10 @@ ic_fused_kernel(IcParams P) {
   __shared__ unsigned long long s_group;
   __shared__ unsigned long long s_base;
   __shared__ uint32_t s_fail;
-  __shared__ uint32_t s_qn[2];
+  __shared__ uint32_t s_qn[2];   // light frontier queues
+  __shared__ uint32_t s_hn[2];   // deferred-hub lists
   __shared__ uint32_t s_union;
+  __shared__ Affine s_jt[kJumpLevels][64];  // step^(RUN * d * 64^l)
   __shared__ uint32_t s_ebeg[BT];
   __shared__ uint32_t s_edeg[BT];
   __shared__ uint32_t s_epre[BT];
@@ -113,14 +125,21 @@ ic_fused_kernel(IcParams P) {
   unsigned long long *vp = P.vp + slot * 2ull * P.n;
   uint32_t *bm0 = P.bm0 + slot * P.nw0;
   uint32_t *bm1 = P.bm1 + slot * P.nw1;
-  uint32_t *qbuf[2] = {P.queue + slot * 3ull * P.npad,
-                       P.queue + slot * 3ull * P.npad + P.npad};
+  // (selected with ?: rather than indexed arrays so they stay in registers)
+  uint32_t *const q0 = P.queue + slot * 3ull * P.npad;
+  uint32_t *const q1 = q0 + P.npad;
   // output staging after the BFS: sorted union + masks
-  uint32_t *list = qbuf[0];
-  unsigned long long *masks =
-      reinterpret_cast<unsigned long long *>(qbuf[1]);
+  uint32_t *list = q0;
+  unsigned long long *masks = reinterpret_cast<unsigned long long *>(q1);
+  uint32_t *const h0 = P.hub + slot * 2ull * P.hub_cap;
+  uint32_t *const h1 = h0 + P.hub_cap;
   // work counters (edges read, coins, expansions, levels)
   unsigned long long st_edges = 0, st_coins = 0, st_exp = 0, st_levels = 0;
+  // jump tables: run k of a row starts RUN*k coins after the row's first
+  for (uint32_t i = tid; i < 64u * P.jlevels; i += BT) {
+    const uint32_t l = i >> 6, d = i & 63;
+    s_jt[l][d] = aff_pow(P.step, static_cast<uint64_t>(RUN) * d << (6 * l));
+  }
 
   for (;;) {
     if (tid == 0) s_group = atomicAdd(&P.ctl[kCtlGroupCounter], 1ull);
@@ -135,40 +154,62 @@ ic_fused_kernel(IcParams P) {
     if (tid == 0) {
       s_qn[0] = 0;
       s_qn[1] = 0;
+      s_hn[0] = 0;
+      s_hn[1] = 0;
       s_union = 0;
       s_fail = 0;
     }
     uint32_t root = 0;
     unsigned long long rbits = 0;
     if (tid < nb) {
-      unsigned long long z = sample_z(P.seed, P.first + s0 + tid);
+      unsigned long long z = sample_z(P.seed, sample_id(P, s0 + tid));
       s_z[tid] = z;
       root = uniform_index(z, P.n);
       rbits = 1ull << tid;
     }
     __syncthreads();
     if (tid < 64)
-      discover(P, vp, bm0, bm1, root, rbits, &s_union, &s_qn[0], qbuf[0]);
+      discover(P, vp, bm0, bm1, root, rbits, &s_union, &s_qn[0], q0);
 
     // ---------------------------------------------------------- BFS levels
-    int cur = 0;
+    // Light levels expand the frontier queue; rows of at least heavy_deg
+    // in-edges met there are parked in a hub list with their pending colours
+    // still accumulating.  When the light frontier runs dry the hub list is
+    // expanded as one level with every colour that reached each hub.  The
+    // RRR set is the reverse closure over live edges, so the schedule does
+    // not change the result, only how often a hub row is read.
+    int lcur = 0, hfill = 0;
     for (;;) {
       __syncthreads();
-      const uint32_t qn = s_qn[cur];
-      const int nxt = cur ^ 1;
-      if (tid == 0) s_qn[nxt] = 0;
+      uint32_t qn = s_qn[lcur];
+      const bool hub_phase = (qn == 0);
+      if (hub_phase) qn = s_hn[hfill];
+      __syncthreads();
       if (qn == 0) break;
-      if (tid == 0) ++st_levels;
-      const uint32_t *qcur = qbuf[cur];
-      uint32_t *qnext = qbuf[nxt];
+      const int lnext = lcur ^ 1;
+      if (tid == 0) {
+        s_qn[lnext] = 0;
+        if (hub_phase) s_hn[hfill ^ 1] = 0;
+        ++st_levels;
+      }
+      uint32_t *const hcur = hfill ? h1 : h0;
+      const uint32_t *qsrc = hub_phase ? hcur : (lcur ? q1 : q0);
+      uint32_t *qnext = lnext ? q1 : q0;
+      uint32_t *hdefer = hcur;
       for (uint32_t cb = 0; cb < qn; cb += BT) {
         const uint32_t cl = min(static_cast<uint32_t>(BT), qn - cb);
-        uint32_t runs = 0;
+        uint32_t runs = 0, v = 0;
+        bool defer = false;
         if (tid < cl) {
-          const uint32_t v = qcur[cb + tid];
-          const unsigned long long M = atomicExch(&vp[2ull * v + 1], 0ull);
+          v = qsrc[cb + tid];
           const VertexRec r = P.vrec[v];
           const uint32_t deg = r.end - r.begin;
+          unsigned long long M = 0;
+          if (!hub_phase && deg >= P.heavy_deg) {
+            defer = true;  // pending colours stay set: no re-push until expanded
+          } else {
+            M = atomicExch(&vp[2ull * v + 1], 0ull);
+          }
           s_ebeg[tid] = r.begin;
           s_edeg[tid] = deg;
           s_ea[tid] = r.ja;
@@ -181,6 +222,7 @@ ic_fused_kernel(IcParams P) {
             ++st_exp;
           }
         }
+        wave_append(defer, v, &s_hn[hfill], hdefer);
         uint32_t total;
         const uint32_t pre = block_exclusive_scan<BT>(runs, s_tmp, &total);
         if (tid < cl) s_epre[tid] = pre;
@@ -202,12 +244,14 @@ ic_fused_kernel(IcParams P) {
               else hi = mid;
             }
             const uint32_t ent = lo - 1;
-            const uint32_t j0 = (rr - s_epre[ent]) * RUN;
+            const uint32_t k = rr - s_epre[ent];
+            const uint32_t j0 = k * RUN;
             const uint32_t deg = s_edeg[ent];
             const uint32_t e0 = s_ebeg[ent] + j0;
             const unsigned long long M = s_em[ent];
-            const Affine T{s_ea[ent], s_eb[ent]};
-            const Affine E = aff_then(T, P.jtab[j0]);
+            Affine E{s_ea[ent], s_eb[ent]};
+            for (uint32_t l = 0; l < P.jlevels; ++l)
+              E = aff_then(E, s_jt[l][(k >> (6 * l)) & 63]);
             uint32_t thr[RUN];
 #pragma unroll
             for (int i = 0; i < RUN; ++i) {
@@ -230,11 +274,12 @@ ic_fused_kernel(IcParams P) {
           }
 #pragma unroll
           for (int i = 0; i < RUN; ++i)
-            discover(P, vp, bm0, bm1, src[i], acc[i], &s_union, &s_qn[nxt], qnext);
+            discover(P, vp, bm0, bm1, src[i], acc[i], &s_union, &s_qn[lnext], qnext);
         }
         __syncthreads();
       }
-      cur = nxt;
+      lcur = lnext;
+      if (hub_phase) hfill ^= 1;
     }
 
     // ---------------------------------------------- sorted union + colours
@@ -301,11 +346,12 @@ ic_fused_kernel(IcParams P) {
         }
       }
       if (tid < nb) {
-        P.set_off[P.first + s0 + tid] = base + s_cofs[tid];
-        P.set_len[P.first + s0 + tid] = s_cnt[tid];
+        const uint64_t s = sample_id(P, s0 + tid);
+        P.set_off[s] = base + s_cofs[tid];
+        P.set_len[s] = s_cnt[tid];
       }
     } else if (tid < nb) {
-      P.set_len[P.first + s0 + tid] = 0xFFFFFFFFu;
+      P.set_len[sample_id(P, s0 + tid)] = 0xFFFFFFFFu;
     }
     __syncthreads();
   }
@@ -333,10 +379,9 @@ template <int BT>
 void launch_run(const IcParams &P, uint32_t slots, uint32_t run,
                 hipStream_t stream) {
   switch (run) {
-    case 1: launch_one<BT, 1>(P, slots, stream); break;
-    case 2: launch_one<BT, 2>(P, slots, stream); break;
+    case 4: launch_one<BT, 4>(P, slots, stream); break;
     case 8: launch_one<BT, 8>(P, slots, stream); break;
-    default: launch_one<BT, 4>(P, slots, stream); break;
+    default: launch_one<BT, 16>(P, slots, stream); break;
   }
 }
 
@@ -359,10 +404,9 @@ int occupancy_of() {
 template <int BT>
 int occupancy_run(uint32_t run) {
   switch (run) {
-    case 1: return occupancy_of<BT, 1>();
-    case 2: return occupancy_of<BT, 2>();
+    case 4: return occupancy_of<BT, 4>();
     case 8: return occupancy_of<BT, 8>();
-    default: return occupancy_of<BT, 4>();
+    default: return occupancy_of<BT, 16>();
   }
 }
 
@@ -374,36 +418,50 @@ int resident_blocks(uint32_t block, uint32_t run) {
   }
 }
 
-size_t slot_bytes_for(uint32_t n) {
+// Upper bound of the number of vertices with at least `deg` in-edges.
+uint64_t hub_capacity(const rip_graph *g, uint32_t deg) {
+  if (deg <= 1) return g->n;
+  uint32_t b = 31 - __builtin_clz(deg);  // 2^b <= deg
+  return std::max<uint64_t>(1, g->deg_ge_pow2[b]);
+}
+
+size_t slot_bytes_for(const rip_graph *g, uint32_t heavy_deg) {
+  const uint32_t n = g->n;
   const uint64_t npad = (static_cast<uint64_t>(n) + 63) & ~63ull;
   const uint64_t nw0 = (n + 31) / 32, nw1 = (n + 1023) / 1024;
-  return 16ull * n + 4ull * nw0 + 4ull * nw1 + 12ull * npad;
+  return 16ull * n + 4ull * nw0 + 4ull * nw1 + 12ull * npad +
+         8ull * hub_capacity(g, heavy_deg);
 }
 
 }  // namespace
 
 IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
-  IcLaunch c{64, 1024, 0, 4};
+  // defaults from a sweep on R-MAT scale 22 weighted cascade: 8-edge runs and
+  // 256-thread groups (more independent groups per CU); hub deferral off,
+  // because colours of one group rarely meet at the same hub there
+  IcLaunch c{64, 256, 0, 8, 0xFFFFFFFFu, RIP_SAMPLER_AUTO};
   if (tune) {
     if (tune->colors) c.colors = tune->colors;
     if (tune->block) c.block = tune->block;
     if (tune->slots) c.slots = tune->slots;
     if (tune->edge_run) c.edge_run = tune->edge_run;
+    if (tune->heavy_degree) c.heavy_degree = tune->heavy_degree;
+    c.sampler = tune->sampler;
   }
   RIP_REQUIRE(c.colors >= 1 && c.colors <= 64, -1, "tune: colors must be in [1, 64]");
   RIP_REQUIRE(c.block == 256 || c.block == 512 || c.block == 1024, -1,
               "tune: block must be 256, 512 or 1024");
-  RIP_REQUIRE(c.edge_run == 1 || c.edge_run == 2 || c.edge_run == 4 || c.edge_run == 8,
-              -1, "tune: edge_run must be 1, 2, 4 or 8");
+  RIP_REQUIRE(c.edge_run == 4 || c.edge_run == 8 || c.edge_run == 16, -1,
+              "tune: edge_run must be 4, 8 or 16");
   if (c.slots == 0) {
     hipDeviceProp_t prop;
     RIP_HIP(hipGetDeviceProperties(&prop, p->g->device));
     size_t free_b = 0, total_b = 0;
     RIP_HIP(hipMemGetInfo(&free_b, &total_b));
-    const size_t per = slot_bytes_for(p->g->n);
-    // keep the sampler scratch within a quarter of the free memory, 64 GiB max
-    size_t budget = std::min(free_b / 4, static_cast<size_t>(64) << 30);
-    if (p->ic.slots && p->ic.n == p->g->n) budget += static_cast<size_t>(p->ic.slots) * per;
+    const size_t per = slot_bytes_for(p->g, c.heavy_degree);
+    // keep the sampler scratch within an eighth of the free memory, 24 GiB max
+    size_t budget = std::min(free_b / 8, static_cast<size_t>(24) << 30);
+    if (p->g->ic.slots && p->g->ic.n == p->g->n) budget += p->g->ic.slot_bytes * p->g->ic.slots;
     uint64_t by_mem = std::max<uint64_t>(1, budget / per);
     uint64_t by_cu = static_cast<uint64_t>(prop.multiProcessorCount) *
                      resident_blocks(c.block, c.edge_run);
@@ -412,13 +470,15 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
   return c;
 }
 
-void ic_ensure_slots(rip_pool *p, uint32_t slots) {
-  IcSlots &s = p->ic;
-  if (s.slots >= slots && s.n == p->g->n) return;
+void ic_ensure_slots(rip_pool *p, uint32_t slots, uint32_t heavy_deg) {
+  IcSlots &s = p->g->ic;
+  const uint64_t hub_cap = hub_capacity(p->g, heavy_deg);
+  if (s.slots >= slots && s.n == p->g->n && s.hub_cap >= hub_cap) return;
   dev_free(s.d_vp);
   dev_free(s.d_bm0);
   dev_free(s.d_bm1);
   dev_free(s.d_queue);
+  dev_free(s.d_hub);
   s.slots = 0;
   const uint32_t n = p->g->n;
   const uint64_t npad = (static_cast<uint64_t>(n) + 63) & ~63ull;
@@ -427,22 +487,23 @@ void ic_ensure_slots(rip_pool *p, uint32_t slots) {
   s.d_bm0 = dev_alloc<uint32_t>(nw0 * slots);
   s.d_bm1 = dev_alloc<uint32_t>(nw1 * slots);
   s.d_queue = dev_alloc<uint32_t>(3ull * npad * slots);
+  s.d_hub = dev_alloc<uint32_t>(2ull * hub_cap * slots);
   RIP_HIP(hipMemsetAsync(s.d_vp, 0, sizeof(unsigned long long) * 2ull * n * slots, p->stream));
   RIP_HIP(hipMemsetAsync(s.d_bm0, 0, sizeof(uint32_t) * nw0 * slots, p->stream));
   RIP_HIP(hipMemsetAsync(s.d_bm1, 0, sizeof(uint32_t) * nw1 * slots, p->stream));
   s.slots = slots;
   s.n = n;
-  s.slot_bytes = slot_bytes_for(n);
+  s.hub_cap = hub_cap;
+  s.slot_bytes = slot_bytes_for(p->g, heavy_deg);
 }
 
 uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
-                   const IcLaunch &cfg) {
-  ic_ensure_slots(p, cfg.slots);
+                   const IcLaunch &cfg, const uint32_t *ids, bool record_events) {
+  ic_ensure_slots(p, cfg.slots, cfg.heavy_degree);
   const rip_graph *g = p->g;
   IcParams P;
   P.vrec = g->d_vrec;
   P.adj = g->d_adj;
-  P.jtab = g->d_jtab;
   P.n = g->n;
   P.step = g->step;
   P.seed = p->seed;
@@ -450,10 +511,10 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
   P.count = count;
   P.colors = cfg.colors;
   P.ngroups = (count + cfg.colors - 1) / cfg.colors;
-  P.vp = p->ic.d_vp;
-  P.bm0 = p->ic.d_bm0;
-  P.bm1 = p->ic.d_bm1;
-  P.queue = p->ic.d_queue;
+  P.vp = p->g->ic.d_vp;
+  P.bm0 = p->g->ic.d_bm0;
+  P.bm1 = p->g->ic.d_bm1;
+  P.queue = p->g->ic.d_queue;
   P.nw0 = (g->n + 31) / 32;
   P.nw1 = (g->n + 1023) / 1024;
   P.npad = (static_cast<uint64_t>(g->n) + 63) & ~63ull;
@@ -462,18 +523,38 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
   P.data = p->d_data;
   P.cap_data = p->cap_data;
   P.ctl = p->d_ctl;
+  P.heavy_deg = cfg.heavy_degree;
+  P.hub = p->g->ic.d_hub;
+  P.hub_cap = p->g->ic.hub_cap;
+  // 6-bit digits needed for the largest run index of a row
+  const uint64_t max_runs = (static_cast<uint64_t>(g->max_degree) + cfg.edge_run - 1) /
+                            cfg.edge_run;
+  uint32_t levels = 1;
+  while (levels < kJumpLevels && (max_runs >> (6 * levels)) != 0) ++levels;
+  RIP_REQUIRE((max_runs >> (6 * levels)) == 0, -1, "graph: row too long for the jump tables");
+  P.jlevels = levels;
+  P.ids = ids;
   const uint32_t slots =
       static_cast<uint32_t>(std::min<uint64_t>(cfg.slots, P.ngroups));
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
-  RIP_HIP(hipEventRecord(p->ev0, p->stream));
+  if (record_events) RIP_HIP(hipEventRecord(p->ev0, p->stream));
   switch (cfg.block) {
     case 256: launch_run<256>(P, slots, cfg.edge_run, p->stream); break;
-    case 512: launch_run<512>(P, slots, cfg.edge_run, p->stream); break;
-    default: launch_run<1024>(P, slots, cfg.edge_run, p->stream); break;
+    case 1024: launch_run<1024>(P, slots, cfg.edge_run, p->stream); break;
+    default: launch_run<512>(P, slots, cfg.edge_run, p->stream); break;
   }
   RIP_HIP(hipGetLastError());
   RIP_HIP(hipEventRecord(p->ev1, p->stream));
   return 2;
+}
+
+uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &cfg) {
+  if (cfg.sampler == RIP_SAMPLER_COLOUR) return ic_launch(p, first, count, cfg);
+  const uint64_t big = ic_wave_launch(p, first, count);
+  if (big == 0) return 1;
+  // sets that outgrew a wave: colour groups over the listed sample indices
+  ic_launch(p, first, big, cfg, p->g->wave.d_big, false);
+  return 3;
 }
 
 }  // namespace rip

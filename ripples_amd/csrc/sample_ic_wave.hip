@@ -1,0 +1,299 @@
+// Wave sampler: one IC reverse BFS per wave, many waves per CU, no
+// workgroup barriers.
+//
+// Semantics are those of AddRRRSet (include/ripples/add_rrrset.h:105-157)
+// under the per-sample stream contract of common.h (the coin of edge e in
+// sample s is element 1 + e of s's stream), so the set is the reverse
+// closure of the root over live edges and equals the colour sampler's and
+// the oracle's bit for bit.
+//
+// Layout per resident wave (HBM):
+//   table[TCAP]  visited set: open addressing, 64-bit entries
+//                (epoch << 32 | vertex); an entry counts only if its epoch
+//                is the wave's current one, so the table is never cleared
+//   list[CAP]    BFS queue and output list (FIFO in discovery order)
+// Expansion of a vertex: lane l reads in-edges begin + l + 64 i (coalesced)
+// and starts from coin P^l(x0), x0 the row's first coin, stepping P^64 per
+// round.  Only live edges touch the table.  The finished list is sorted in
+// LDS (bitonic, 1024 entries per wave) or, if longer, in its HBM buffer.
+// Samples whose sets outgrow CAP are listed for the colour sampler.
+#include <algorithm>
+
+#include "device_util.h"
+#include "internal.h"
+
+namespace rip {
+namespace {
+
+constexpr uint32_t kEmptyVertex = 0xFFFFFFFFu;
+constexpr uint32_t kLenFail = 0xFFFFFFFFu;
+constexpr uint32_t kSortCap = 1024;
+
+struct WaveParams {
+  const VertexRec *vrec;
+  const uint2 *adj;
+  uint32_t n;
+  Affine step;
+  uint64_t seed;
+  uint64_t first;
+  uint64_t count;
+  unsigned long long *table;   // waves x tcap
+  uint32_t *list;              // waves x cap
+  uint32_t *epoch;             // per resident wave
+  uint32_t cap, tcap;          // tcap a power of two >= 2 cap
+  uint32_t *big_ids;           // samples left for the colour sampler
+  unsigned long long *set_off;
+  uint32_t *set_len;
+  uint32_t *data;
+  uint64_t cap_data;
+  unsigned long long *ctl;
+};
+
+__device__ inline uint32_t hash_vertex(uint32_t v) {
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  v ^= v >> 16;
+  return v;
+}
+
+// Insert v into the wave's table; true if it was not there yet.
+__device__ inline bool table_insert(unsigned long long *table, uint32_t mask,
+                                    unsigned long long tag, uint32_t v) {
+  const unsigned long long want = tag | v;
+  uint32_t h = hash_vertex(v) & mask;
+  for (;;) {
+    unsigned long long cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+      if ((cur & 0xFFFFFFFF00000000ull) != tag) {  // stale or empty slot
+        const unsigned long long seen = atomicCAS(&table[h], cur, want);
+        if (seen == cur) return true;
+        cur = seen;
+        continue;
+      }
+      if (static_cast<uint32_t>(cur) == v) return false;
+      break;  // occupied by another vertex of this sample
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// Bitonic sort of a[0, len) (len <= cap2, a power of two) padded with
+// kEmptyVertex; one wave, lanes over compare-exchange pairs.
+template <typename Ptr>
+__device__ inline void wave_bitonic(Ptr a, uint32_t len, uint32_t cap2) {
+  const uint32_t lane = lane_id();
+  for (uint32_t i = len + lane; i < cap2; i += 64) a[i] = kEmptyVertex;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  for (uint32_t k = 2; k <= cap2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t t = lane; t < cap2 / 2; t += 64) {
+        // pair (i, i ^ j) with i having bit j clear
+        const uint32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+        const uint32_t p = i | j;
+        const uint32_t x = a[i], y = a[p];
+        const bool up = (i & k) == 0;
+        if ((x > y) == up) {
+          a[i] = y;
+          a[p] = x;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+  }
+}
+
+template <int BT, int U>
+__global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
+  constexpr int NW = BT / 64;
+  __shared__ uint32_t s_sort[NW][kSortCap];
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x >> 6;
+  const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * NW + wid;
+  unsigned long long *table = P.table + wave * P.tcap;
+  uint32_t *list = P.list + wave * P.cap;
+  const uint32_t tmask = P.tcap - 1;
+  uint32_t epoch = P.epoch[wave];
+  const Affine jl = aff_pow(P.step, lane);    // coin offset of this lane
+  const Affine j64 = aff_pow(P.step, 64);     // one round of 64 edges
+  unsigned long long st_edges = 0, st_exp = 0;
+
+  for (;;) {
+    unsigned long long idx = 0;
+    if (lane == 0) idx = atomicAdd(&P.ctl[kCtlGroupCounter], 1ull);
+    idx = __shfl(idx, 0, 64);
+    if (idx >= P.count) break;
+    const uint64_t s = P.first + idx;
+    ++epoch;
+    const unsigned long long tag = static_cast<unsigned long long>(epoch) << 32;
+    const uint64_t z = sample_z(P.seed, s);
+    const uint32_t root = uniform_index(z, P.n);
+    if (lane == 0) {
+      table_insert(table, tmask, tag, root);
+      list[0] = root;
+    }
+    uint32_t head = 0, tail = 1;
+    bool big = false;
+    while (head < tail && !big) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      const uint32_t v = list[head++];
+      const VertexRec r = P.vrec[v];
+      const uint32_t deg = r.end - r.begin;
+      ++st_exp;
+      st_edges += deg;
+      const uint64_t x0 = r.ja * z + r.jb;          // coin of the row's first edge
+      uint64_t x = jl.a * x0 + jl.b;                  // coin of edge begin + lane
+      for (uint32_t j0 = 0; j0 < deg && !big; j0 += 64 * U) {
+        uint2 a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t j = j0 + lane + 64 * u;
+          a[u] = j < deg ? P.adj[r.begin + j] : make_uint2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool live = static_cast<uint32_t>(x >> 40) < a[u].y;  // thr 0 when j >= deg
+          x = j64.a * x + j64.b;
+          const uint64_t lm = __ballot(live);
+          if (lm == 0) continue;
+          if (tail + __popcll(lm) > P.cap) {
+            big = true;
+            break;
+          }
+          const bool fresh = live && table_insert(table, tmask, tag, a[u].x);
+          const uint64_t fm = __ballot(fresh);
+          if (fresh) list[tail + __popcll(fm & lanes_below())] = a[u].x;
+          tail += __popcll(fm);
+        }
+      }
+    }
+    if (big) {
+      if (lane == 0) {
+        const unsigned long long b = atomicAdd(&P.ctl[kCtlLongCount], 1ull);
+        P.big_ids[b] = static_cast<uint32_t>(idx);
+      }
+      continue;
+    }
+    // sort the set and copy it to the pool
+    const uint32_t len = tail;
+    uint32_t cap2 = 64;
+    while (cap2 < len) cap2 <<= 1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    const uint32_t *sorted;
+    if (cap2 <= kSortCap) {
+      uint32_t *a = s_sort[wid];
+      for (uint32_t i = lane; i < len; i += 64) a[i] = list[i];
+      wave_bitonic(a, len, cap2);
+      sorted = a;
+    } else {
+      // the HBM list has room for cap entries; cap is a power of two
+      wave_bitonic(list, len, cap2);
+      sorted = list;
+    }
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&P.ctl[kCtlDataBump], static_cast<unsigned long long>(len));
+    base = __shfl(base, 0, 64);
+    if (base + len > P.cap_data) {
+      if (lane == 0) {
+        atomicOr(&P.ctl[kCtlOverflow], 1ull);
+        P.set_len[s] = kLenFail;
+      }
+      continue;
+    }
+    for (uint32_t i = lane; i < len; i += 64) P.data[base + i] = sorted[i];
+    if (lane == 0) {
+      P.set_off[s] = base;
+      P.set_len[s] = len;
+    }
+  }
+  if (lane == 0) P.epoch[wave] = epoch;
+  for (int d = 32; d > 0; d >>= 1) {
+    st_edges += __shfl_down(st_edges, d, 64);
+    st_exp += __shfl_down(st_exp, d, 64);
+  }
+  // every lane counted the same rows: report lane 0's share once
+  if (lane == 0) {
+    atomicAdd(&P.ctl[kCtlStatEdges], st_edges / 64);
+    atomicAdd(&P.ctl[kCtlStatCoins], st_edges / 64);
+    atomicAdd(&P.ctl[kCtlStatExpansions], st_exp / 64);
+  }
+}
+
+template <int BT, int U>
+uint32_t wave_grid() {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(&ic_wave_kernel<BT, U>), BT, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  int dev = 0, cus = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return static_cast<uint32_t>(per_cu * cus);
+}
+
+}  // namespace
+
+// Returns the number of samples left for the colour sampler (their local
+// indices are in g->wave.d_big).
+uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count) {
+  constexpr int BT = 256, U = 4;
+  constexpr uint32_t CAP = 4096, TCAP = 8192;
+  rip_graph *g = p->g;
+  WaveScratch &w = g->wave;
+  const uint32_t blocks = std::max<uint32_t>(
+      1, std::min<uint64_t>(wave_grid<BT, U>(), (count + BT / 64 - 1) / (BT / 64)));
+  const uint64_t waves = static_cast<uint64_t>(blocks) * (BT / 64);
+  if (w.waves < waves) {
+    dev_free(w.d_table);
+    dev_free(w.d_list);
+    dev_free(w.d_epoch);
+    w.waves = 0;
+    w.d_table = dev_alloc<unsigned long long>(waves * TCAP);
+    w.d_list = dev_alloc<uint32_t>(waves * CAP);
+    w.d_epoch = dev_alloc<uint32_t>(waves);
+    RIP_HIP(hipMemsetAsync(w.d_table, 0, sizeof(unsigned long long) * waves * TCAP, p->stream));
+    RIP_HIP(hipMemsetAsync(w.d_epoch, 0, sizeof(uint32_t) * waves, p->stream));
+    w.waves = waves;
+  }
+  if (w.cap_big < count) {
+    dev_free(w.d_big);
+    w.d_big = dev_alloc<uint32_t>(count);
+    w.cap_big = count;
+  }
+  WaveParams P;
+  P.vrec = g->d_vrec;
+  P.adj = g->d_adj;
+  P.n = g->n;
+  P.step = g->step;
+  P.seed = p->seed;
+  P.first = first;
+  P.count = count;
+  P.table = w.d_table;
+  P.list = w.d_list;
+  P.epoch = w.d_epoch;
+  P.cap = CAP;
+  P.tcap = TCAP;
+  P.big_ids = w.d_big;
+  P.set_off = p->d_off;
+  P.set_len = p->d_len;
+  P.data = p->d_data;
+  P.cap_data = p->cap_data;
+  P.ctl = p->d_ctl;
+  RIP_HIP(hipEventRecord(p->ev0, p->stream));
+  hipLaunchKernelGGL((ic_wave_kernel<BT, U>), dim3(blocks), dim3(BT), 0, p->stream, P);
+  RIP_HIP(hipGetLastError());
+  RIP_HIP(hipEventRecord(p->ev1, p->stream));
+  unsigned long long nbig = 0;
+  RIP_HIP(hipMemcpyAsync(&nbig, p->d_ctl + kCtlLongCount, sizeof(nbig),
+                         hipMemcpyDeviceToHost, p->stream));
+  RIP_HIP(hipStreamSynchronize(p->stream));
+  return nbig;
+}
+
+}  // namespace rip

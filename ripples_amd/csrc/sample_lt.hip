@@ -310,21 +310,21 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   P.cap_data = p->cap_data;
   P.ctl = p->d_ctl;
   // long-walk ids: at most `count`
-  if (p->lt.cap_ids < count) {
-    dev_free(p->lt.d_long_ids);
-    p->lt.d_long_ids = dev_alloc<uint32_t>(count);
-    p->lt.cap_ids = count;
+  if (p->g->lt.cap_ids < count) {
+    dev_free(p->g->lt.d_long_ids);
+    p->g->lt.d_long_ids = dev_alloc<uint32_t>(count);
+    p->g->lt.cap_ids = count;
   }
-  P.long_ids = p->lt.d_long_ids;
+  P.long_ids = p->g->lt.d_long_ids;
   hipLaunchKernelGGL(reset_long_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
   const uint32_t blocks = lt_grid<BT, CAP, SPILL, UNROLL>(count);
   const uint64_t spill_words = static_cast<uint64_t>(blocks) * BT * SPILL;
-  if (p->lt.cap_spill < spill_words) {
-    dev_free(p->lt.d_spill);
-    p->lt.d_spill = dev_alloc<uint32_t>(spill_words);
-    p->lt.cap_spill = spill_words;
+  if (p->g->lt.cap_spill < spill_words) {
+    dev_free(p->g->lt.d_spill);
+    p->g->lt.d_spill = dev_alloc<uint32_t>(spill_words);
+    p->g->lt.cap_spill = spill_words;
   }
-  P.spill = p->lt.d_spill;
+  P.spill = p->g->lt.d_spill;
   RIP_HIP(hipEventRecord(p->ev0, p->stream));
   hipLaunchKernelGGL((lt_walk_kernel<BT, CAP, SPILL, UNROLL>), dim3(blocks), dim3(BT), 0,
                      p->stream, P);

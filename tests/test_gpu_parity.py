@@ -71,9 +71,14 @@ def test_selection_matches_oracle(case, orc):
 
 
 @pytest.mark.parametrize("tune", [dict(colors=1), dict(colors=17, block=256),
-                                  dict(colors=64, block=512, edge_run=1),
+                                  dict(colors=64, block=512, edge_run=4),
                                   dict(colors=32, block=1024, edge_run=8, slots=3),
-                                  dict(colors=64, block=256, edge_run=2, slots=1)])
+                                  dict(colors=64, block=256, edge_run=16, slots=1),
+                                  dict(colors=64, heavy_degree=1),
+                                  dict(colors=64, heavy_degree=1 << 30),
+                                  dict(colors=7, heavy_degree=3, edge_run=4),
+                                  dict(sampler=1), dict(sampler=2),
+                                  dict(sampler=2, colors=64, heavy_degree=64)])
 def test_ic_tuning_does_not_change_results(tune, orc):
     index, edges, w = rmat_csr(12, R.W_UNIFORM, 0.1, 1)
     g = R.Graph(index, edges, w, model=R.IC)
@@ -111,6 +116,28 @@ def test_pool_growth_retry(orc):
     p = g.pool(1)
     p.generate_rrr_sets(600)
     assert_same_sets(p.export(), orc.sample(R.IC, index, edges, w, 1, 0, 600))
+    p.close()
+    g.close()
+
+
+@pytest.mark.parametrize("n", [1500, 3000, 6000])
+def test_ic_large_sets(n, orc):
+    """Sets of ~n vertices: LDS sort (<= 1024), HBM sort (<= 4096) and the
+    colour-group fallback for sets a wave cannot hold."""
+    rng = np.random.default_rng(n)
+    s = rng.integers(0, n, 8 * n)
+    d = rng.integers(0, n, 8 * n)
+    keep = s != d
+    key = np.unique(s[keep] * n + d[keep])
+    s, d = key // n, key % n
+    w = np.where(rng.random(len(s)) < 0.9, 1.0, 0.0).astype(np.float32)
+    index, edges, w = transposed_csr(n, s, d, w)
+    g = R.Graph(index, edges, w, model=R.IC)
+    p = g.pool(4)
+    p.generate_rrr_sets(300)
+    got = p.export()
+    assert_same_sets(got, orc.sample(R.IC, index, edges, w, 4, 0, 300))
+    assert np.diff(got[0]).max() > min(n - 100, 4096) or n < 4096
     p.close()
     g.close()
 
