@@ -58,6 +58,14 @@ def ic_union_bytes(index, off, ver):
     return int(8 * len(u) + 8 * deg.sum() + 4 * len(ver))
 
 
+def ic_plain_bytes(index, ver):
+    """Sum over samples of sum over R_s of (8 + 8 indeg + 4): every sample
+    reading its own rows, the one-BFS-per-sample form."""
+    v = ver.astype(np.int64)
+    deg = (index[v + 1] - index[v]).astype(np.int64)
+    return int(12 * len(v) + 8 * deg.sum())
+
+
 def cpu_baseline(wl, index, edges, w, seed, batch, target_s):
     from oracle import oracle as O
     threads = os.cpu_count() or 1
@@ -77,84 +85,73 @@ def cpu_baseline(wl, index, edges, w, seed, batch, target_s):
                       "samples, %.1f s" % (count, count - 1, dt)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat20_lt", choices=sorted(WORKLOADS))
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--batch", type=int, default=0, help="sets per step (0 = theta)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--tune", default="", help='sampler tuning, e.g. "colors=1,block=256"')
-    ap.add_argument("--skip-imm", action="store_true")
-    a = ap.parse_args()
-    tune = None
-    if a.tune:
-        tune = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in a.tune.split(",")}
-    if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(run with --gpus 1)", file=sys.stderr)
-        return 2
-    if R.device_count() < 1:
-        print("no HIP device visible", file=sys.stderr)
-        return 1
-    wl = WORKLOADS[a.workload]
-    index, edges, w = build_graph(wl, a.seed)
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc runs."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "traffic.json")
+    try:
+        table = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    entry = table.get("%s/%s" % (workload, kernel))
+    return None if entry is None else entry.get("hbm_bytes_per_launch")
+
+
+def measure(name, seed, steps, warmup, batch, tune, skip_imm):
+    wl = WORKLOADS[name]
+    index, edges, w = build_graph(wl, seed)
     g = R.Graph(index, edges, w, model=wl["model"])
-
-    # IMM wall time: one warm-up run, one timed run
     imm_wall, rec = None, None
-    if not a.skip_imm:
-        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=a.seed, tune=tune)
+    if not skip_imm:
+        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)   # warm-up
         t = time.time()
-        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=a.seed, tune=tune)
+        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)
         imm_wall = time.time() - t
-        log("IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
-            rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
-
-    batch = a.batch or (int(rec["Theta"]) if rec else 1 << 20)
-    pool = g.pool(a.seed)
-    for _ in range(a.warmup):
+        log("%s IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
+            name, rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
+    batch = batch or (int(rec["Theta"]) if rec else 1 << 20)
+    pool = g.pool(seed)
+    for _ in range(warmup):
         pool.reset()
         pool.generate_rrr_sets(batch, tune=tune)
     kms, algo = [], []
     t0 = time.time()
-    for _ in range(a.steps):
+    for _ in range(steps):
         pool.reset()
         pool.generate_rrr_sets(batch, tune=tune)   # returns after the device finished
         kms.append(pool.last_kernel_ms()[0])
         algo.append(pool.last_algo_bytes())
     wall = time.time() - t0
     stats = pool.last_stats()
-    log("kernel counters (last step): %s" % stats)
-    value = batch * a.steps / wall
     n_sets, n_elem = pool.size()
+    kernel_ms = float(np.mean(kms))
+    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel_ms": kernel_ms}
     if wl["model"] == R.IC:
         off, ver = pool.export()
         per_launch = ic_union_bytes(index, off, ver)
+        plain = ic_plain_bytes(index, ver)
+        roof["kernel"] = "ic_wave_kernel"
+        roof["algo_bytes_per_launch"] = per_launch
+        roof["algo_bytes_formula"] = ("sum over the union of the launch's sets of "
+                                      "(8 + 8 indeg) + 4 per output id")
+        roof["plain_form_bytes_per_launch"] = plain
+        roof["plain_form_achieved"] = plain / (kernel_ms * 1e-3) / 1e9
+        roof["edges_read"] = stats["edges"]
     else:
         per_launch = float(np.mean(algo))
-    kernel_ms = float(np.mean(kms))
+        roof["kernel"] = "lt_walk_kernel"
+        roof["algo_bytes_per_launch"] = per_launch
+        roof["algo_bytes_formula"] = ("sum over walks of 8 per visited vertex + 8 per "
+                                      "scanned in-edge + 4 per output id")
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9
-    log("sampling: %d x %d sets in %.3fs; kernel %.3f ms; %.1f GB/s algorithmic" % (
-        a.steps, batch, wall, kernel_ms, achieved))
-
-    out = {
-        "metric": "RRR sets/s + IMM wall s at fixed k,eps; p50/p99 latency of served IMM queries",
-        "value": value,
-        "unit": "RRR sets/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32 ids, f32 weights, u64 lcg64",
-        "data": "synthetic R-MAT (generated in-process, seed %d)" % a.seed,
-        "config": {"workload": a.workload, "description": wl["desc"],
+    roof["achieved"] = achieved
+    roof["frac"] = achieved / HBM_PEAK_GBS
+    roof["traffic"] = measured_traffic(roof["kernel"], name)
+    log("%s sampling: %d x %d sets in %.3fs; kernel %.3f ms; %.1f GB/s algorithmic; "
+        "counters %s" % (name, steps, batch, wall, kernel_ms, achieved, stats))
+    res = {
+        "value": batch * steps / wall, "ms_per_step": wall / steps * 1e3,
+        "config": {"workload": name, "description": wl["desc"],
                    "num_nodes": int(len(index) - 1), "num_edges": int(len(edges)),
                    "batch_sets": batch, "k": wl["k"], "epsilon": wl["eps"],
                    "parallelism": "single GPU"},
@@ -166,19 +163,96 @@ def main():
             "select_ms": rec["FindMostInfluentialSet"]},
         "pool": {"sets": n_sets, "elements": n_elem},
         "kernel_counters": stats,
+        "roofline": roof,
+        "graph": (wl, index, edges, w, batch),
+    }
+    pool.close()
+    g.close()
+    return res
+
+
+def copy_probe(gib):
+    """One device-to-device copy of `gib` GiB: the reference kernel for scaling
+    rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(gib * (1 << 30))
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "copies": 3}))
+    return 0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat20_lt", choices=sorted(WORKLOADS))
+    ap.add_argument("--secondary", default="rmat22_wc",
+                    help="second workload reported beside the first ('' for none)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=0, help="sets per step (0 = theta)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--tune", default="", help='sampler tuning, e.g. "sampler=2,block=256"')
+    ap.add_argument("--skip-imm", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a device copy of this many GiB (counter calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+    if a.copy_probe:
+        return copy_probe(a.copy_probe)
+    if R.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+    tune = None
+    if a.tune:
+        tune = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in a.tune.split(",")}
+    res = measure(a.workload, a.seed, a.steps, a.warmup, a.batch, tune, a.skip_imm)
+    wl, index, edges, w, batch = res.pop("graph")
+    out = {
+        "metric": "RRR sets/s + IMM wall s at fixed k,eps; p50/p99 latency of served IMM queries",
+        "value": res["value"],
+        "unit": "RRR sets/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 ids, f32 weights, u64 lcg64",
+        "data": "synthetic R-MAT (generated in-process, seed %d)" % a.seed,
+        "config": res["config"],
+        "imm_wall_s": res["imm_wall_s"],
+        "imm": res["imm"],
+        "pool": res["pool"],
+        "kernel_counters": res["kernel_counters"],
         "tune": tune,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "lt_walk_kernel" if wl["model"] == R.LT else "ic_fused_kernel",
-                     "kernel_ms": kernel_ms, "algo_bytes_per_launch": per_launch},
+        "roofline": res["roofline"],
         "cpu_baseline": None,
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, index, edges, w, a.seed, batch, a.cpu_seconds)
         log("cpu baseline: %.0f sets/s on %d threads" % (out["cpu_baseline"]["value"],
                                                           out["cpu_baseline"]["cores"]))
-    pool.close()
-    g.close()
+    del index, edges, w
+    if a.secondary and a.secondary != a.workload:
+        sec = measure(a.secondary, a.seed, max(1, a.steps // 3), 1, 0, None, False)
+        sec.pop("graph")
+        out["secondary"] = {k: sec[k] for k in ("value", "ms_per_step", "config", "imm_wall_s",
+                                                "imm", "pool", "kernel_counters", "roofline")}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -82,8 +82,14 @@ inline uint32_t ic_threshold(float w) {
 struct alignas(32) VertexRec {
   uint64_t ja, jb;       // P^(1 + begin) as an affine map
   uint32_t begin, end;   // transposed-CSR row [begin, end)
-  uint32_t pad0, pad1;
+  uint32_t uthr;         // IC: the row's threshold when all its edges share it
+  uint32_t flags;        // kRowUniform when uthr is valid
 };
+
+constexpr uint32_t kRowUniform = 1u;
+// Extra zero elements after the per-edge arrays, so 16-byte loads of an
+// aligned group of four never leave the allocation.
+constexpr uint64_t kEdgePad = 4;
 
 // ---------------------------------------------------------------- errors
 struct Error : std::runtime_error {

@@ -1,9 +1,12 @@
 // Device copy of a transposed CSR graph (GraphBwd), laid out for the reverse
 // traversals of the samplers:
-//   d_vrec[v] : 32-byte record {P^(1+begin) jump, begin, end} -- one aligned
-//               load gives a vertex's row and the jump to its first coin
-//   d_adj[e]  : (in-neighbour, IC integer threshold | LT float weight bits),
-//               padded with two zero edges for pairwise 16-byte loads
+//   d_vrec[v] : 32-byte record {P^(1+begin) jump, begin, end, uniform IC
+//               threshold, flags} -- one aligned load gives a vertex's row,
+//               the jump to its first coin and, for rows whose edges share a
+//               threshold (weighted cascade), that threshold
+//   d_val[e]  : IC integer threshold | LT float weight bits
+//   d_ids[e]  : in-neighbour (read only for live / chosen edges)
+// Both per-edge arrays carry kEdgePad zero elements for 16-byte loads.
 #include <cstring>
 #include <vector>
 
@@ -12,8 +15,8 @@
 namespace rip {
 namespace {
 
-__global__ void build_vrec_kernel(const unsigned long long *index, uint32_t n,
-                                  Affine step, VertexRec *vrec) {
+__global__ void build_vrec_kernel(const unsigned long long *index, const uint32_t *uthr,
+                                  uint32_t n, Affine step, VertexRec *vrec) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   uint64_t b = index[v], e = index[v + 1];
@@ -23,8 +26,8 @@ __global__ void build_vrec_kernel(const unsigned long long *index, uint32_t n,
   r.jb = j.b;
   r.begin = static_cast<uint32_t>(b);
   r.end = static_cast<uint32_t>(e);
-  r.pad0 = 0;
-  r.pad1 = 0;
+  r.uthr = uthr[v];
+  r.flags = uthr[v] != 0xFFFFFFFFu ? kRowUniform : 0u;
   vrec[v] = r;
 }
 
@@ -38,7 +41,7 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
   RIP_REQUIRE(index != nullptr, -1, "graph: index is NULL");
   RIP_REQUIRE(m == 0 || (edges != nullptr && weights != nullptr), -1,
               "graph: edges/weights are NULL");
-  RIP_REQUIRE(m < (1ull << 32), -1, "graph: more than 2^32-1 edges");
+  RIP_REQUIRE(m < (1ull << 32) - kEdgePad, -1, "graph: more than 2^32-5 edges");
   RIP_REQUIRE(index[0] == 0 && index[n] == m, -1,
               "graph: index[0] must be 0 and index[n] the edge count");
   uint32_t maxdeg = 0;
@@ -49,17 +52,26 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
     if (d > maxdeg) maxdeg = static_cast<uint32_t>(d);
     for (int b = 0; b < 33 && (1ull << b) <= d; ++b) ++deg_ge[b];
   }
-  // two zero edges of padding: 16-byte loads read edges in aligned pairs
-  std::vector<uint2> adj(m + 2, make_uint2(0, 0));
+  std::vector<uint32_t> ids(m + kEdgePad, 0), val(m + kEdgePad, 0);
   for (uint64_t e = 0; e < m; ++e) {
     RIP_REQUIRE(edges[e] < n, -1, "graph: edge endpoint out of range");
-    uint32_t y;
+    ids[e] = edges[e];
     if (model == RIP_IC) {
-      y = ic_threshold(weights[e]);
+      val[e] = ic_threshold(weights[e]);
     } else {
-      std::memcpy(&y, &weights[e], 4);
+      std::memcpy(&val[e], &weights[e], 4);
     }
-    adj[e] = make_uint2(edges[e], y);
+  }
+  // IC rows whose edges all share one threshold keep it in the vertex record
+  std::vector<uint32_t> uthr(n, 0xFFFFFFFFu);
+  if (model == RIP_IC) {
+    for (uint32_t v = 0; v < n; ++v) {
+      const uint64_t b = index[v], e = index[v + 1];
+      if (b == e) continue;
+      bool same = true;
+      for (uint64_t i = b + 1; i < e && same; ++i) same = val[i] == val[b];
+      if (same) uthr[v] = val[b];
+    }
   }
   int ndev = 0;
   RIP_HIP(hipGetDeviceCount(&ndev));
@@ -75,32 +87,41 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
   for (int b = 0; b < 33; ++b) g->deg_ge_pow2[b] = deg_ge[b];
 
   unsigned long long *d_index = nullptr;
+  uint32_t *d_uthr = nullptr;
   try {
     g->d_vrec = dev_alloc<VertexRec>(n);
-    g->d_adj = dev_alloc<uint2>(m + 2);
+    g->d_ids = dev_alloc<uint32_t>(m + kEdgePad);
+    g->d_val = dev_alloc<uint32_t>(m + kEdgePad);
     d_index = dev_alloc<unsigned long long>(static_cast<size_t>(n) + 1);
+    d_uthr = dev_alloc<uint32_t>(n);
     RIP_HIP(hipMemcpy(d_index, index, sizeof(uint64_t) * (static_cast<size_t>(n) + 1),
                       hipMemcpyHostToDevice));
-    RIP_HIP(hipMemcpy(g->d_adj, adj.data(), sizeof(uint2) * (m + 2),
+    RIP_HIP(hipMemcpy(d_uthr, uthr.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    RIP_HIP(hipMemcpy(g->d_ids, ids.data(), sizeof(uint32_t) * (m + kEdgePad),
+                      hipMemcpyHostToDevice));
+    RIP_HIP(hipMemcpy(g->d_val, val.data(), sizeof(uint32_t) * (m + kEdgePad),
                       hipMemcpyHostToDevice));
     dim3 grid((n + 255) / 256);
-    hipLaunchKernelGGL(build_vrec_kernel, grid, dim3(256), 0, 0, d_index, n,
+    hipLaunchKernelGGL(build_vrec_kernel, grid, dim3(256), 0, 0, d_index, d_uthr, n,
                        g->step, g->d_vrec);
     RIP_HIP(hipGetLastError());
     RIP_HIP(hipDeviceSynchronize());
   } catch (...) {
     dev_free(d_index);
+    dev_free(d_uthr);
     graph_destroy(g);
     throw;
   }
   dev_free(d_index);
+  dev_free(d_uthr);
 }
 
 void graph_destroy(rip_graph *g) {
   if (!g) return;
   (void)hipSetDevice(g->device);
   dev_free(g->d_vrec);
-  dev_free(g->d_adj);
+  dev_free(g->d_ids);
+  dev_free(g->d_val);
   dev_free(g->ic.d_vp);
   dev_free(g->ic.d_bm0);
   dev_free(g->ic.d_bm1);

@@ -65,7 +65,8 @@ struct rip_graph {
   uint32_t max_degree = 0;
   rip::Affine step{1, 0};           // one step of a sample stream
   rip::VertexRec *d_vrec = nullptr;  // n records
-  uint2 *d_adj = nullptr;            // m x (source, IC threshold | LT weight bits)
+  uint32_t *d_ids = nullptr;         // m + kEdgePad: in-neighbour of each edge
+  uint32_t *d_val = nullptr;         // m + kEdgePad: IC threshold | LT weight bits
   uint64_t deg_ge_pow2[33] = {};     // vertices with in-degree >= 2^b
   // Sampler scratch shared by every pool of this graph; pool_sample holds
   // sampler_mu while a launch uses it.

@@ -39,7 +39,8 @@ namespace {
 
 struct IcParams {
   const VertexRec *vrec;
-  const uint2 *adj;
+  const uint32_t *ids;
+  const uint32_t *val;
   uint32_t n;
   Affine step;
   uint64_t seed;
@@ -62,11 +63,11 @@ struct IcParams {
   uint32_t *hub;        // slots x 2 x hub_cap deferred-hub lists
   uint64_t hub_cap;
   uint32_t jlevels;     // 6-bit digits of the run index covered by the jump tables
-  const uint32_t *ids;  // optional: local index -> sample index (minus first)
+  const uint32_t *sample_ids;  // optional: local index -> sample index (minus first)
 };
 
 __device__ inline uint64_t sample_id(const IcParams &P, uint64_t local) {
-  return P.first + (P.ids ? P.ids[local] : local);
+  return P.first + (P.sample_ids ? P.sample_ids[local] : local);
 }
 
 constexpr int kJumpLevels = 5;
@@ -113,6 +114,7 @@ ic_fused_kernel(IcParams P) {
   __shared__ Affine s_jt[kJumpLevels][64];  // step^(RUN * d * 64^l)
   __shared__ uint32_t s_ebeg[BT];
   __shared__ uint32_t s_edeg[BT];
+  __shared__ uint32_t s_eu[BT];   // uniform row threshold or 0xFFFFFFFF
   __shared__ uint32_t s_epre[BT];
   __shared__ unsigned long long s_ea[BT];
   __shared__ unsigned long long s_eb[BT];
@@ -212,6 +214,7 @@ ic_fused_kernel(IcParams P) {
           }
           s_ebeg[tid] = r.begin;
           s_edeg[tid] = deg;
+          s_eu[tid] = (r.flags & kRowUniform) ? r.uthr : 0xFFFFFFFFu;
           s_ea[tid] = r.ja;
           s_eb[tid] = r.jb;
           s_em[tid] = M;
@@ -252,16 +255,11 @@ ic_fused_kernel(IcParams P) {
             Affine E{s_ea[ent], s_eb[ent]};
             for (uint32_t l = 0; l < P.jlevels; ++l)
               E = aff_then(E, s_jt[l][(k >> (6 * l)) & 63]);
+            const uint32_t uthr = s_eu[ent];
             uint32_t thr[RUN];
 #pragma unroll
-            for (int i = 0; i < RUN; ++i) {
-              thr[i] = 0;
-              if (j0 + i < deg) {
-                const uint2 a = P.adj[e0 + i];
-                src[i] = a.x;
-                thr[i] = a.y;
-              }
-            }
+            for (int i = 0; i < RUN; ++i)
+              thr[i] = j0 + i < deg ? (uthr != 0xFFFFFFFFu ? uthr : P.val[e0 + i]) : 0u;
             for (unsigned long long mm = M; mm; mm &= mm - 1) {
               const int c = __ffsll(mm) - 1;
               uint64_t x = E.a * s_z[c] + E.b;
@@ -271,6 +269,10 @@ ic_fused_kernel(IcParams P) {
                 if (i + 1 < RUN) x = P.step.a * x + P.step.b;
               }
             }
+            // in-neighbour ids only for edges live for some colour
+#pragma unroll
+            for (int i = 0; i < RUN; ++i)
+              if (acc[i]) src[i] = P.ids[e0 + i];
           }
 #pragma unroll
           for (int i = 0; i < RUN; ++i)
@@ -459,9 +461,11 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
     size_t free_b = 0, total_b = 0;
     RIP_HIP(hipMemGetInfo(&free_b, &total_b));
     const size_t per = slot_bytes_for(p->g, c.heavy_degree);
-    // keep the sampler scratch within an eighth of the free memory, 24 GiB max
-    size_t budget = std::min(free_b / 8, static_cast<size_t>(24) << 30);
-    if (p->g->ic.slots && p->g->ic.n == p->g->n) budget += p->g->ic.slot_bytes * p->g->ic.slots;
+    // keep the sampler scratch within an eighth of the memory it may use
+    // (free plus what it already holds), 24 GiB at most
+    size_t held = 0;
+    if (p->g->ic.slots && p->g->ic.n == p->g->n) held = p->g->ic.slot_bytes * p->g->ic.slots;
+    const size_t budget = std::min((free_b + held) / 8, static_cast<size_t>(24) << 30);
     uint64_t by_mem = std::max<uint64_t>(1, budget / per);
     uint64_t by_cu = static_cast<uint64_t>(prop.multiProcessorCount) *
                      resident_blocks(c.block, c.edge_run);
@@ -499,11 +503,15 @@ void ic_ensure_slots(rip_pool *p, uint32_t slots, uint32_t heavy_deg) {
 
 uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
                    const IcLaunch &cfg, const uint32_t *ids, bool record_events) {
-  ic_ensure_slots(p, cfg.slots, cfg.heavy_degree);
+  // no more resident groups than there are groups; scratch only ever grows
+  const uint64_t groups = (count + cfg.colors - 1) / cfg.colors;
+  ic_ensure_slots(p, static_cast<uint32_t>(std::min<uint64_t>(cfg.slots, groups)),
+                  cfg.heavy_degree);
   const rip_graph *g = p->g;
   IcParams P;
   P.vrec = g->d_vrec;
-  P.adj = g->d_adj;
+  P.ids = g->d_ids;
+  P.val = g->d_val;
   P.n = g->n;
   P.step = g->step;
   P.seed = p->seed;
@@ -533,7 +541,7 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
   while (levels < kJumpLevels && (max_runs >> (6 * levels)) != 0) ++levels;
   RIP_REQUIRE((max_runs >> (6 * levels)) == 0, -1, "graph: row too long for the jump tables");
   P.jlevels = levels;
-  P.ids = ids;
+  P.sample_ids = ids;
   const uint32_t slots =
       static_cast<uint32_t>(std::min<uint64_t>(cfg.slots, P.ngroups));
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
@@ -544,7 +552,7 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
     default: launch_run<512>(P, slots, cfg.edge_run, p->stream); break;
   }
   RIP_HIP(hipGetLastError());
-  RIP_HIP(hipEventRecord(p->ev1, p->stream));
+  if (record_events) RIP_HIP(hipEventRecord(p->ev1, p->stream));
   return 2;
 }
 

@@ -31,7 +31,8 @@ constexpr uint32_t kSortCap = 1024;
 
 struct WaveParams {
   const VertexRec *vrec;
-  const uint2 *adj;
+  const uint32_t *ids;
+  const uint32_t *val;
   uint32_t n;
   Affine step;
   uint64_t seed;
@@ -147,16 +148,19 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
       st_edges += deg;
       const uint64_t x0 = r.ja * z + r.jb;          // coin of the row's first edge
       uint64_t x = jl.a * x0 + jl.b;                  // coin of edge begin + lane
+      const bool uniform = (r.flags & kRowUniform) != 0;  // same for the whole wave
       for (uint32_t j0 = 0; j0 < deg && !big; j0 += 64 * U) {
-        uint2 a[U];
+        // thresholds: from the vertex record for uniform rows, else one
+        // coalesced 4-byte load per edge; thr 0 past the row end
+        uint32_t thr[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const uint32_t j = j0 + lane + 64 * u;
-          a[u] = j < deg ? P.adj[r.begin + j] : make_uint2(0, 0);
+          thr[u] = j < deg ? (uniform ? r.uthr : P.val[r.begin + j]) : 0u;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const bool live = static_cast<uint32_t>(x >> 40) < a[u].y;  // thr 0 when j >= deg
+          const bool live = static_cast<uint32_t>(x >> 40) < thr[u];
           x = j64.a * x + j64.b;
           const uint64_t lm = __ballot(live);
           if (lm == 0) continue;
@@ -164,9 +168,11 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
             big = true;
             break;
           }
-          const bool fresh = live && table_insert(table, tmask, tag, a[u].x);
+          // the in-neighbour id is read only for live edges
+          const uint32_t src = live ? P.ids[r.begin + j0 + lane + 64 * u] : 0u;
+          const bool fresh = live && table_insert(table, tmask, tag, src);
           const uint64_t fm = __ballot(fresh);
-          if (fresh) list[tail + __popcll(fm & lanes_below())] = a[u].x;
+          if (fresh) list[tail + __popcll(fm & lanes_below())] = src;
           tail += __popcll(fm);
         }
       }
@@ -268,7 +274,8 @@ uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count) {
   }
   WaveParams P;
   P.vrec = g->d_vrec;
-  P.adj = g->d_adj;
+  P.ids = g->d_ids;
+  P.val = g->d_val;
   P.n = g->n;
   P.step = g->step;
   P.seed = p->seed;

@@ -27,7 +27,8 @@ constexpr uint32_t kLenLong = 0xFFFFFFFEu;
 
 struct LtParams {
   const VertexRec *vrec;
-  const uint2 *adj;
+  const uint32_t *ids;
+  const uint32_t *val;
   uint32_t n;
   uint64_t seed;
   uint64_t first;
@@ -51,14 +52,14 @@ struct LtParams {
 // lt_long_kernel.
 template <int BT, int CAP, int SPILL, int UNROLL>
 __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
-  static_assert(UNROLL % 2 == 0, "pairs of edges per load");
+  static_assert(UNROLL % 4 == 0, "groups of four weights per load");
   __shared__ uint32_t s_list[CAP * BT];
   uint32_t *list = s_list + threadIdx.x;  // element i < CAP at list[i * BT]
   uint32_t *spill = P.spill + (static_cast<uint64_t>(blockIdx.x) * BT + threadIdx.x) * SPILL;
   auto at = [&](uint32_t i) -> uint32_t & {
     return i < static_cast<uint32_t>(CAP) ? list[i * BT] : spill[i - CAP];
   };
-  const uint4 *adj4 = reinterpret_cast<const uint4 *>(P.adj);
+  const uint4 *val4 = reinterpret_cast<const uint4 *>(P.val);
   const uint32_t lane = lane_id();
   bool busy = false, exhausted = false;
   uint64_t gid = 0, z = 0;
@@ -103,25 +104,25 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
       if (e >= end) {
         done = true;  // no in-edge crosses the threshold: the walk stops
       } else {
-        const uint32_t base = e & ~1u;      // pair-aligned start
-        const uint32_t last_pair = (end - 1) >> 1;
-        uint4 q[UNROLL / 2];
+        const uint32_t base = e & ~3u;      // 16-byte aligned group of 4 weights
+        const uint32_t last_quad = (end - 1) >> 2;
+        uint4 q[UNROLL / 4];
 #pragma unroll
-        for (int kk = 0; kk < UNROLL / 2; ++kk)
-          q[kk] = adj4[min((base >> 1) + kk, last_pair)];
-        const uint32_t lo = e - base;       // 0 or 1
+        for (int kk = 0; kk < UNROLL / 4; ++kk)
+          q[kk] = val4[min((base >> 2) + kk, last_quad)];
+        const uint32_t lo = e - base;       // 0..3
         const uint32_t hi = min(static_cast<uint32_t>(UNROLL), end - base);
         bool found = false;
-        uint32_t u = 0, stop = hi;
+        uint32_t stop = hi;
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
-          const uint32_t wbits = (k & 1) ? q[k >> 1].w : q[k >> 1].y;
-          const uint32_t src = (k & 1) ? q[k >> 1].z : q[k >> 1].x;
+          const uint4 &g4 = q[k >> 2];
+          const uint32_t wbits = (k & 3) == 0 ? g4.x : (k & 3) == 1 ? g4.y
+                                 : (k & 3) == 2 ? g4.z : g4.w;
           if (!found && static_cast<uint32_t>(k) >= lo && static_cast<uint32_t>(k) < hi) {
             t -= __uint_as_float(wbits);
             if (!(t > 0)) {
               found = true;
-              u = src;
               stop = k + 1;
             }
           }
@@ -131,6 +132,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         edges += scanned;
         e = base + stop;
         if (found) {
+          const uint32_t u = P.ids[e - 1];  // the chosen in-neighbour
           bool seen = false;
           for (uint32_t i = 0; i < len; ++i)
             if (at(i) == u) { seen = true; break; }
@@ -255,10 +257,10 @@ __global__ void lt_long_kernel(LtParams P, uint32_t num_long, uint32_t lo,
     float t = u01(r.ja * z + r.jb);
     uint32_t next = kLenFail;
     for (uint32_t e = r.begin; e < r.end; ++e) {
-      const uint2 a = P.adj[e];
-      t -= __uint_as_float(a.y);
+      t -= __uint_as_float(P.val[e]);
       if (t > 0) continue;
-      if (!((bm[a.x >> 5] >> (a.x & 31)) & 1u)) next = a.x;
+      const uint32_t u = P.ids[e];
+      if (!((bm[u >> 5] >> (u & 31)) & 1u)) next = u;
       break;
     }
     if (next == kLenFail) break;
@@ -299,7 +301,8 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   const rip_graph *g = p->g;
   LtParams P;
   P.vrec = g->d_vrec;
-  P.adj = g->d_adj;
+  P.ids = g->d_ids;
+  P.val = g->d_val;
   P.n = g->n;
   P.seed = p->seed;
   P.first = first;
