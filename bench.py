@@ -142,7 +142,13 @@ def measure(name, seed, steps, warmup, batch, tune, skip_imm):
         roof["kernel"] = "lt_walk_kernel"
         roof["algo_bytes_per_launch"] = per_launch
         roof["algo_bytes_formula"] = ("sum over walks of 8 per visited vertex + 8 per "
-                                      "scanned in-edge + 4 per output id")
+                                      "in-edge the subtraction loop scans + 4 per output id")
+        # what the breakpoint search itself reads: the 32-byte vertex record,
+        # one 64-byte node per index level or breakpoint block (the `coins`
+        # counter), 4 per chosen id, 4 per output id
+        search = 32 * n_elem + 64 * stats["coins"] + 8 * n_elem
+        roof["search_form_bytes_per_launch"] = search
+        roof["search_form_achieved"] = search / (kernel_ms * 1e-3) / 1e9
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9
     roof["achieved"] = achieved
     roof["frac"] = achieved / HBM_PEAK_GBS

@@ -94,7 +94,10 @@ int rip_device_count(int *count);
  * (include/ripples/graph.h:107-130, 364-1086; neighbors() at :616-618).
  * index: uint64[num_nodes+1] (graph.h index_type), edges: uint32[num_edges],
  * weights: float[num_edges].  model selects the edge encoding (RIP_IC or
- * RIP_LT).  The caller keeps ownership of the host arrays. */
+ * RIP_LT); for RIP_LT the call also derives each row's threshold
+ * breakpoints on the device (4 bytes per edge), which the LT walk searches
+ * instead of subtracting weights edge by edge.  The caller keeps ownership of
+ * the host arrays. */
 int rip_graph_create(int device, int model, uint32_t num_nodes,
                      uint64_t num_edges, const uint64_t *index,
                      const uint32_t *edges, const float *weights,
@@ -147,11 +150,13 @@ int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
  * stats[0 .. count) in this order (no counterpart in the reference; used for
  * roofline accounting):
  *   RIP_STAT_ALGO_BYTES  LT: sum over walks of 8 per visited vertex (its CSR
- *                        offsets), 8 per scanned in-edge (id + weight) and
- *                        4 per output id; 0 for IC, whose per-launch figure
- *                        depends on the union of the sets
- *   RIP_STAT_EDGES       in-edges read
- *   RIP_STAT_COINS       IC: (edge, sample) coin evaluations
+ *                        offsets), 8 per in-edge the subtraction loop scans
+ *                        (id + weight) and 4 per output id; 0 for IC, whose
+ *                        per-launch figure depends on the union of the sets
+ *   RIP_STAT_EDGES       IC: in-edges read; LT: in-edges the subtraction
+ *                        loop scans (up to and including the chosen one)
+ *   RIP_STAT_COINS       IC: (edge, sample) coin evaluations;
+ *                        LT: row breakpoints read
  *   RIP_STAT_EXPANSIONS  IC: frontier entries expanded (vertex, colour group)
  *   RIP_STAT_LEVELS      IC: BFS levels summed over colour groups */
 #define RIP_STAT_ALGO_BYTES 0
@@ -193,6 +198,54 @@ int rip_csr_from_edges(uint32_t num_nodes, uint64_t num_edges,
                        const uint32_t *src, const uint32_t *dst, int scheme,
                        float scale, uint64_t seed, uint64_t *index,
                        uint32_t *edges, float *weights);
+
+/* ---------------------------------------------------------------- files */
+/* Host transposed CSR read from a file: the arrays rip_graph_create takes,
+ * plus the map from internal to original vertex ids.  Replaces loadGraph /
+ * loadEdgeList (include/ripples/loaders.h:322-418) and Graph::load_binary /
+ * dump_binary (include/ripples/graph.h:700-771, 854-995). */
+typedef struct rip_host_graph rip_host_graph;
+
+/* Weight distributions for unweighted edge lists (--distribution). */
+#define RIP_DIST_UNIFORM 0 /* u01 * scale_factor */
+#define RIP_DIST_NORMAL 1  /* normal(mean, scale = variance) truncated to [0,1],
+                              times scale_factor */
+#define RIP_DIST_CONST 2   /* mean (in [0, 1]) */
+
+typedef struct {
+  int model;          /* RIP_IC or RIP_LT: under LT the drawn weights of each
+                         source are divided by (their sum + one more draw),
+                         loaders.h:169-191 */
+  int weighted;       /* the third column of each line is the weight (-w) */
+  int undirected;     /* each line also gives the reverse edge (-u) */
+  int renumber;       /* ids mapped to their rank (default); 0 keeps ids and
+                         n = max id + 1 (--disable-renumbering) */
+  int distribution;   /* RIP_DIST_* for unweighted inputs */
+  float mean;         /* --mean (0.5) */
+  float variance;     /* --variance (1.0) */
+  float scale_factor; /* --scale-factor, in (0, 1] */
+  uint64_t seed;      /* weights stream: lcg64(seed).split(2, 0), draw i =
+                         master output 2i (tools/imm.cc:172-174 use 0) */
+} rip_load_options;
+
+/* Edge list in TSV form ("src dst" or, weighted, "src dst w" per line; empty
+ * lines and lines containing '%' or '#' are skipped; a malformed line is an
+ * error).  Rows list the sources of the edges into each vertex in input
+ * order, duplicates kept (graph.h:506-599 with one thread). */
+int rip_load_edge_list(const char *path, const rip_load_options *opts,
+                       rip_host_graph **out);
+/* Reference binary CSR (--reload-binary); must be the backward (transposed)
+ * direction.  Weights are multiplied by scale_factor (in (0, 1]). */
+int rip_load_binary(const char *path, float scale_factor, rip_host_graph **out);
+/* Writes the reference binary CSR (backward direction, float weights). */
+int rip_dump_binary(const rip_host_graph *g, const char *path);
+/* Sizes and arrays of a host graph; the pointers stay valid until
+ * rip_host_graph_destroy.  reverse_map[v] is v's original id. */
+int rip_host_graph_info(const rip_host_graph *g, uint32_t *num_nodes, uint64_t *num_edges);
+int rip_host_graph_arrays(const rip_host_graph *g, const uint64_t **index,
+                          const uint32_t **edges, const float **weights,
+                          const uint32_t **reverse_map);
+int rip_host_graph_destroy(rip_host_graph *g);
 
 #ifdef __cplusplus
 }

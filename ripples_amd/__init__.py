@@ -21,6 +21,7 @@ __all__ = [
 
 IC, LT = 0, 1
 W_UNIFORM, W_CASCADE, W_LT_NORM, W_CONST = 0, 1, 2, 3
+DIST_UNIFORM, DIST_NORMAL, DIST_CONST = 0, 1, 2
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libripples_amd.so")
@@ -41,6 +42,12 @@ class Tune(C.Structure):
                 ("slots", C.c_uint32), ("edge_run", C.c_uint32),
                 ("heavy_degree", C.c_uint32), ("sampler", C.c_uint32),
                 ("reserved", C.c_uint32 * 2)]
+
+
+class LoadOptions(C.Structure):
+    _fields_ = [("model", C.c_int), ("weighted", C.c_int), ("undirected", C.c_int),
+                ("renumber", C.c_int), ("distribution", C.c_int), ("mean", C.c_float),
+                ("variance", C.c_float), ("scale_factor", C.c_float), ("seed", C.c_uint64)]
 
 
 class ImmRecord(C.Structure):
@@ -111,6 +118,16 @@ def lib():
                                  C.c_uint64]
     L.rip_csr_from_edges.argtypes = [C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_int,
                                      C.c_float, C.c_uint64, _u64p, _u32p, _f32p]
+    hp = C.POINTER(C.c_void_p)
+    L.rip_load_edge_list.argtypes = [C.c_char_p, C.POINTER(LoadOptions), hp]
+    L.rip_load_binary.argtypes = [C.c_char_p, C.c_float, hp]
+    L.rip_dump_binary.argtypes = [vp, C.c_char_p]
+    L.rip_host_graph_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
+    L.rip_host_graph_arrays.argtypes = [vp, C.POINTER(C.POINTER(C.c_uint64)),
+                                        C.POINTER(C.POINTER(C.c_uint32)),
+                                        C.POINTER(C.POINTER(C.c_float)),
+                                        C.POINTER(C.POINTER(C.c_uint32))]
+    L.rip_host_graph_destroy.argtypes = [vp]
     _LIB = L
     return L
 
@@ -159,6 +176,66 @@ def csr_from_edges(n, src, dst, scheme=W_UNIFORM, scale=1.0, seed=0):
                                     dst if m else np.zeros(1, np.uint32), scheme,
                                     scale, seed, index, edges, weights))
     return index, edges[:m], weights[:m]
+
+
+def _host_graph_arrays(h):
+    n, m = C.c_uint32(0), C.c_uint64(0)
+    _check(lib().rip_host_graph_info(h, C.byref(n), C.byref(m)))
+    pi, pe = C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint32)()
+    pw, pr = C.POINTER(C.c_float)(), C.POINTER(C.c_uint32)()
+    _check(lib().rip_host_graph_arrays(h, C.byref(pi), C.byref(pe), C.byref(pw), C.byref(pr)))
+    n, m = n.value, m.value
+    cp = lambda ptr, k, dt: np.ctypeslib.as_array(ptr, (k,)).astype(dt) if k else np.zeros(0, dt)
+    return cp(pi, n + 1, np.uint64), cp(pe, m, np.uint32), cp(pw, m, np.float32), cp(pr, n, np.uint32)
+
+
+class HostGraph:
+    """Host transposed CSR read from a file: index, edges, weights and the
+    reverse map (internal -> original ids), as numpy copies."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.index, self.edges, self.weights, self.reverse_map = _host_graph_arrays(handle)
+
+    @property
+    def num_nodes(self):
+        return len(self.index) - 1
+
+    def dump_binary(self, path):
+        """Reference binary CSR, backward direction (graph.h:700-771)."""
+        _check(lib().rip_dump_binary(self._h, os.fsencode(path)))
+
+    def close(self):
+        if self._h:
+            lib().rip_host_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_edge_list(path, model=IC, weighted=False, undirected=False, renumber=True,
+                   distribution=DIST_UNIFORM, mean=0.5, variance=1.0, scale_factor=1.0,
+                   seed=0):
+    """Edge-list file -> HostGraph, with the reference loader's weight and
+    renumbering semantics (include/ripples/loaders.h:138-418,
+    graph.h:506-599)."""
+    opts = LoadOptions(model, int(weighted), int(undirected), int(renumber), distribution,
+                       mean, variance, scale_factor, seed)
+    h = C.c_void_p()
+    _check(lib().rip_load_edge_list(os.fsencode(path), C.byref(opts), C.byref(h)))
+    return HostGraph(h)
+
+
+def load_binary(path, scale_factor=1.0):
+    """Reference binary CSR (backward direction) -> HostGraph
+    (include/ripples/graph.h:854-995)."""
+    h = C.c_void_p()
+    _check(lib().rip_load_binary(os.fsencode(path), scale_factor, C.byref(h)))
+    return HostGraph(h)
 
 
 class Graph:

@@ -229,4 +229,64 @@ int rip_csr_from_edges(uint32_t num_nodes, uint64_t num_edges, const uint32_t *s
   });
 }
 
+int rip_load_edge_list(const char *path, const rip_load_options *opts, rip_host_graph **out) {
+  return guarded([&] {
+    RIP_REQUIRE(out && opts, RIP_EINVAL, "opts or out is NULL");
+    *out = nullptr;
+    auto *g = new rip_host_graph();
+    try {
+      rip::load_edge_list(path, *opts, g);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int rip_load_binary(const char *path, float scale_factor, rip_host_graph **out) {
+  return guarded([&] {
+    RIP_REQUIRE(out, RIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    auto *g = new rip_host_graph();
+    try {
+      rip::load_binary(path, scale_factor, g);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int rip_dump_binary(const rip_host_graph *g, const char *path) {
+  return guarded([&] {
+    RIP_REQUIRE(g, RIP_EINVAL, "graph is NULL");
+    rip::dump_binary(g, path);
+  });
+}
+
+int rip_host_graph_info(const rip_host_graph *g, uint32_t *num_nodes, uint64_t *num_edges) {
+  return guarded([&] {
+    RIP_REQUIRE(g, RIP_EINVAL, "graph is NULL");
+    if (num_nodes) *num_nodes = static_cast<uint32_t>(g->reverse_map.size());
+    if (num_edges) *num_edges = g->edges.size();
+  });
+}
+
+int rip_host_graph_arrays(const rip_host_graph *g, const uint64_t **index, const uint32_t **edges,
+                          const float **weights, const uint32_t **reverse_map) {
+  return guarded([&] {
+    RIP_REQUIRE(g, RIP_EINVAL, "graph is NULL");
+    if (index) *index = g->index.data();
+    if (edges) *edges = g->edges.data();
+    if (weights) *weights = g->weights.data();
+    if (reverse_map) *reverse_map = g->reverse_map.data();
+  });
+}
+
+int rip_host_graph_destroy(rip_host_graph *g) {
+  return guarded([&] { delete g; });
+}
+
 }  // extern "C"

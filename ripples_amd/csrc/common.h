@@ -82,11 +82,13 @@ inline uint32_t ic_threshold(float w) {
 struct alignas(32) VertexRec {
   uint64_t ja, jb;       // P^(1 + begin) as an affine map
   uint32_t begin, end;   // transposed-CSR row [begin, end)
-  uint32_t uthr;         // IC: the row's threshold when all its edges share it
-  uint32_t flags;        // kRowUniform when uthr is valid
+  uint32_t uthr;         // IC: the row's threshold when all its edges share it;
+                         // LT: offset of the row's breakpoint index
+  uint32_t flags;        // kRowUniform, kRowScalar
 };
 
-constexpr uint32_t kRowUniform = 1u;
+constexpr uint32_t kRowUniform = 1u;  // IC: uthr holds the row's threshold
+constexpr uint32_t kRowScalar = 2u;   // LT: no breakpoints, scan the float chain
 // Extra zero elements after the per-edge arrays, so 16-byte loads of an
 // aligned group of four never leave the allocation.
 constexpr uint64_t kEdgePad = 4;

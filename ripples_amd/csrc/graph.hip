@@ -6,17 +6,21 @@
 //               threshold (weighted cascade), that threshold
 //   d_val[e]  : IC integer threshold | LT float weight bits
 //   d_ids[e]  : in-neighbour (read only for live / chosen edges)
-// Both per-edge arrays carry kEdgePad zero elements for 16-byte loads.
+//   d_bp[e]   : LT row breakpoints of the float chain (lt_scan.h)
+//   d_bidx    : LT fanout-16 index over the breakpoints of rows longer than 16
+// The id and value arrays carry kEdgePad zero elements for 16-byte loads.
 #include <cstring>
 #include <vector>
 
 #include "internal.h"
+#include "lt_scan.h"
 
 namespace rip {
 namespace {
 
 __global__ void build_vrec_kernel(const unsigned long long *index, const uint32_t *uthr,
-                                  uint32_t n, Affine step, VertexRec *vrec) {
+                                  const uint32_t *flags, uint32_t n, Affine step,
+                                  VertexRec *vrec) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   uint64_t b = index[v], e = index[v + 1];
@@ -27,8 +31,40 @@ __global__ void build_vrec_kernel(const unsigned long long *index, const uint32_
   r.begin = static_cast<uint32_t>(b);
   r.end = static_cast<uint32_t>(e);
   r.uthr = uthr[v];
-  r.flags = uthr[v] != 0xFFFFFFFFu ? kRowUniform : 0u;
+  r.flags = flags[v];
   vrec[v] = r;
+}
+
+// One thread per edge e of an LT row without irregular weights: the row
+// breakpoint B[e - begin] of lt_scan.h.  Threads of a wave take consecutive
+// edges, so they read the same weights in step.
+__global__ void build_bp_kernel(const unsigned long long *index, const uint32_t *val,
+                                const VertexRec *vrec, uint32_t n, uint64_t m, uint32_t *bp) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  uint32_t lo = 0, hi = n;  // row of e: the last v with index[v] <= e
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (index[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const VertexRec r = vrec[lo];
+  if (r.flags & kRowScalar) {
+    bp[e] = kLtNever;
+    return;
+  }
+  bp[e] = lt_breakpoint(val + r.begin, static_cast<uint32_t>(e) - r.begin);
+}
+
+// One thread per indexed LT row: the fanout-16 index over its breakpoints.
+__global__ void build_bidx_kernel(const uint32_t *rows, uint32_t nrows, const VertexRec *vrec,
+                                  const uint32_t *bp, uint32_t *bidx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  const VertexRec r = vrec[rows[i]];
+  lt_index_build(bp + r.begin, r.end - r.begin, bidx + r.uthr);
 }
 
 }  // namespace
@@ -62,15 +98,41 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
       std::memcpy(&val[e], &weights[e], 4);
     }
   }
-  // IC rows whose edges all share one threshold keep it in the vertex record
-  std::vector<uint32_t> uthr(n, 0xFFFFFFFFu);
-  if (model == RIP_IC) {
-    for (uint32_t v = 0; v < n; ++v) {
-      const uint64_t b = index[v], e = index[v + 1];
-      if (b == e) continue;
+  // IC rows whose edges all share one threshold keep it in the vertex record;
+  // LT rows holding a weight without a breakpoint form are marked
+  std::vector<uint32_t> uthr(n, 0xFFFFFFFFu), flags(n, 0u);
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint64_t b = index[v], e = index[v + 1];
+    if (b == e) continue;
+    if (model == RIP_IC) {
       bool same = true;
       for (uint64_t i = b + 1; i < e && same; ++i) same = val[i] == val[b];
-      if (same) uthr[v] = val[b];
+      if (same) {
+        uthr[v] = val[b];
+        flags[v] |= kRowUniform;
+      }
+    } else {
+      for (uint64_t i = b; i < e; ++i)
+        if (lt_weight_irregular(val[i])) {
+          flags[v] |= kRowScalar;
+          break;
+        }
+    }
+  }
+
+  // LT rows longer than 16 edges (without irregular weights) get a breakpoint
+  // index; its offset takes the place of the IC threshold in the record
+  std::vector<uint32_t> index_rows;
+  uint64_t index_size = 0;
+  if (model == RIP_LT) {
+    for (uint32_t v = 0; v < n; ++v) {
+      const uint64_t d = index[v + 1] - index[v];
+      if (d <= kLtFan || (flags[v] & kRowScalar)) continue;
+      RIP_REQUIRE(index_size + lt_index_size(static_cast<uint32_t>(d)) < (1ull << 32), -1,
+                  "graph: LT breakpoint index exceeds 2^32 entries");
+      index_rows.push_back(v);
+      uthr[v] = static_cast<uint32_t>(index_size);
+      index_size += lt_index_size(static_cast<uint32_t>(d));
     }
   }
   int ndev = 0;
@@ -87,33 +149,57 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
   for (int b = 0; b < 33; ++b) g->deg_ge_pow2[b] = deg_ge[b];
 
   unsigned long long *d_index = nullptr;
-  uint32_t *d_uthr = nullptr;
+  uint32_t *d_uthr = nullptr, *d_flags = nullptr, *d_rows = nullptr;
   try {
     g->d_vrec = dev_alloc<VertexRec>(n);
     g->d_ids = dev_alloc<uint32_t>(m + kEdgePad);
     g->d_val = dev_alloc<uint32_t>(m + kEdgePad);
     d_index = dev_alloc<unsigned long long>(static_cast<size_t>(n) + 1);
     d_uthr = dev_alloc<uint32_t>(n);
+    d_flags = dev_alloc<uint32_t>(n);
     RIP_HIP(hipMemcpy(d_index, index, sizeof(uint64_t) * (static_cast<size_t>(n) + 1),
                       hipMemcpyHostToDevice));
     RIP_HIP(hipMemcpy(d_uthr, uthr.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    RIP_HIP(hipMemcpy(d_flags, flags.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     RIP_HIP(hipMemcpy(g->d_ids, ids.data(), sizeof(uint32_t) * (m + kEdgePad),
                       hipMemcpyHostToDevice));
     RIP_HIP(hipMemcpy(g->d_val, val.data(), sizeof(uint32_t) * (m + kEdgePad),
                       hipMemcpyHostToDevice));
     dim3 grid((n + 255) / 256);
-    hipLaunchKernelGGL(build_vrec_kernel, grid, dim3(256), 0, 0, d_index, d_uthr, n,
-                       g->step, g->d_vrec);
+    hipLaunchKernelGGL(build_vrec_kernel, grid, dim3(256), 0, 0, d_index, d_uthr, d_flags,
+                       n, g->step, g->d_vrec);
     RIP_HIP(hipGetLastError());
+    if (model == RIP_LT && m > 0) {
+      // padded so a search may read 16-byte groups past the last row
+      g->d_bp = dev_alloc<uint32_t>(m + kBpPad);
+      RIP_HIP(hipMemset(g->d_bp + m, 0xFF, sizeof(uint32_t) * kBpPad));
+      hipLaunchKernelGGL(build_bp_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)),
+                         dim3(256), 0, 0, d_index, g->d_val, g->d_vrec, n, m, g->d_bp);
+      RIP_HIP(hipGetLastError());
+      if (!index_rows.empty()) {
+        const uint32_t nrows = static_cast<uint32_t>(index_rows.size());
+        g->d_bidx = dev_alloc<uint32_t>(index_size);
+        d_rows = dev_alloc<uint32_t>(nrows);
+        RIP_HIP(hipMemcpy(d_rows, index_rows.data(), sizeof(uint32_t) * nrows,
+                          hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(build_bidx_kernel, dim3((nrows + 255) / 256), dim3(256), 0, 0, d_rows,
+                           nrows, g->d_vrec, g->d_bp, g->d_bidx);
+        RIP_HIP(hipGetLastError());
+      }
+    }
     RIP_HIP(hipDeviceSynchronize());
   } catch (...) {
     dev_free(d_index);
     dev_free(d_uthr);
+    dev_free(d_flags);
+    dev_free(d_rows);
     graph_destroy(g);
     throw;
   }
   dev_free(d_index);
   dev_free(d_uthr);
+  dev_free(d_flags);
+  dev_free(d_rows);
 }
 
 void graph_destroy(rip_graph *g) {
@@ -133,6 +219,8 @@ void graph_destroy(rip_graph *g) {
   dev_free(g->wave.d_big);
   dev_free(g->lt.d_long_ids);
   dev_free(g->lt.d_spill);
+  dev_free(g->d_bp);
+  dev_free(g->d_bidx);
 }
 
 }  // namespace rip

@@ -8,7 +8,16 @@
 #include <vector>
 
 #include "common.h"
+#include "lt_scan.h"
 #include "ripples_amd.h"
+
+// Host transposed CSR read from a file (loaders.cpp).
+struct rip_host_graph {
+  std::vector<uint64_t> index;
+  std::vector<uint32_t> edges;
+  std::vector<float> weights;
+  std::vector<uint32_t> reverse_map;
+};
 
 namespace rip {
 
@@ -38,10 +47,12 @@ struct WaveScratch {
 
 struct LtScratch {
   uint64_t cap_ids = 0;
-  uint32_t *d_long_ids = nullptr;      // walks that outgrew the LDS + spill list
+  uint32_t *d_long_ids = nullptr;      // walks that outgrew the LDS + spill list (from
+                                       // the front), unsorted long lists (from the back)
   uint64_t cap_spill = 0;
   uint32_t *d_spill = nullptr;         // per-lane visited-list spill area
 };
+
 
 struct SelectScratch {
   uint32_t n = 0;
@@ -67,6 +78,8 @@ struct rip_graph {
   rip::VertexRec *d_vrec = nullptr;  // n records
   uint32_t *d_ids = nullptr;         // m + kEdgePad: in-neighbour of each edge
   uint32_t *d_val = nullptr;         // m + kEdgePad: IC threshold | LT weight bits
+  uint32_t *d_bp = nullptr;          // LT, m: row breakpoints of the float chain
+  uint32_t *d_bidx = nullptr;        // LT: fanout-16 index of rows longer than 16
   uint64_t deg_ge_pow2[33] = {};     // vertices with in-degree >= 2^b
   // Sampler scratch shared by every pool of this graph; pool_sample holds
   // sampler_mu while a launch uses it.
@@ -110,6 +123,7 @@ enum : int {
   kCtlStatCoins = 7,
   kCtlStatExpansions = 8,
   kCtlStatLevels = 9,
+  kCtlSortCount = 10,    // LT: finished lists longer than the LDS part, to sort
   kCtlWords = 16,
 };
 
@@ -196,5 +210,10 @@ void dev_free(T *&p) {
   if (p) (void)hipFree(p);
   p = nullptr;
 }
+
+// loaders.cpp
+void load_edge_list(const char *path, const rip_load_options &o, rip_host_graph *g);
+void load_binary(const char *path, float scale_factor, rip_host_graph *g);
+void dump_binary(const rip_host_graph *g, const char *path);
 
 }  // namespace rip

@@ -6,7 +6,16 @@
 // arithmetic, and take the first in-neighbour at which the threshold drops
 // to <= 0; stop when no edge is taken or the taken vertex was already
 // visited.  The threshold of (sample s, vertex v) is element 1 + begin(v)
-// of sample s's stream, reached with the vertex record's jump.
+// of sample s's stream, reached with the vertex record's jump; it is
+// U = k * 2^-24 with k = (x >> 40) + 1.
+//
+// The float chain's crossing edge is a non-decreasing function of k, so the
+// graph holds per-row breakpoints and a fanout-16 index over them (lt_scan.h,
+// built once with the graph), and a step is one search for the first
+// breakpoint above k -- a 64-byte node per index level and one block of
+// breakpoints -- giving the same edge as the subtraction loop.  Rows holding negative, infinite or NaN
+// weights have no breakpoints and are scanned with the float chain itself,
+// UNROLL edges per round.
 //
 // Walks up to CAP vertices keep their visited list in LDS (column-major, so
 // lanes hit distinct banks) and are sorted in place; the reference's GPU
@@ -18,6 +27,7 @@
 
 #include "device_util.h"
 #include "internal.h"
+#include "lt_scan.h"
 
 namespace rip {
 namespace {
@@ -29,6 +39,8 @@ struct LtParams {
   const VertexRec *vrec;
   const uint32_t *ids;
   const uint32_t *val;
+  const uint32_t *bp;
+  const uint32_t *bidx;
   uint32_t n;
   uint64_t seed;
   uint64_t first;
@@ -42,17 +54,112 @@ struct LtParams {
   uint32_t *spill;
 };
 
-// Persistent walkers: every lane runs its own walk and, when it ends, claims
-// the next sample id from a device counter (one atomic per wave and round),
-// so short walks never wait for the long ones of the same wave.  Each round
-// scans up to UNROLL in-edges of the lane's current row with 16-byte loads
-// (two edges each) issued together, then reduces the float threshold in CSR
-// order.  A walk's visited list lives in LDS for its first CAP vertices and
-// in a per-lane HBM spill area for the next SPILL; longer walks are handed to
-// lt_long_kernel.
+// Sample ids a wave claims from the work counter at a time.
+constexpr unsigned long long kClaim = 256;
+
+// LT threshold of the walk at a vertex: U = k * 2^-24.
+__device__ inline uint32_t lt_threshold(const VertexRec &r, uint64_t z) {
+  return static_cast<uint32_t>((r.ja * z + r.jb) >> 40) + 1u;
+}
+
+__device__ inline uint32_t count_le(uint4 q, uint32_t k) {
+  return (q.x <= k ? 1u : 0u) + (q.y <= k ? 1u : 0u) + (q.z <= k ? 1u : 0u) + (q.w <= k ? 1u : 0u);
+}
+
+// Crossing edge of threshold k in row [begin, begin + d) (first j with
+// B[j] > k, or d) through the row's fanout-16 breakpoint index at bidx + off
+// (lt_index_cross of lt_scan.h with 16-byte loads): one aligned 64-byte node
+// per level, then one block of at most 16 breakpoints.  *probes counts the
+// nodes and blocks read.
+__device__ inline uint32_t lt_cross(const uint32_t *bp, const uint32_t *bidx, uint32_t begin,
+                                    uint32_t d, uint32_t off, uint32_t k, uint32_t *probes) {
+  // level l holds ceil(d / 16^(l+1)) entries; the top is the first level of
+  // at most 16
+  int nl = 0;
+  while (d > kLtFan && ((static_cast<uint64_t>(d) + (1ull << (4 * nl + 4)) - 1) >> (4 * nl + 4)) >
+                           kLtFan)
+    ++nl;
+  if (d > kLtFan) ++nl;
+  uint32_t node = 0;
+  uint32_t start = off;  // the top level comes first
+  for (int l = nl - 1; l >= 0; --l) {
+    const uint32_t sz =
+        static_cast<uint32_t>((static_cast<uint64_t>(d) + (1ull << (4 * l + 4)) - 1) >> (4 * l + 4));
+    const uint4 *q = reinterpret_cast<const uint4 *>(bidx + start + 16u * node);
+    const uint4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    const uint32_t p = count_le(q0, k) + count_le(q1, k) + count_le(q2, k) + count_le(q3, k);
+    ++*probes;
+    const uint32_t child = 16u * node + p;
+    if (child >= sz) return d;  // every breakpoint is <= k
+    node = child;
+    start += lt_pad16(sz);
+  }
+  // the block [first, first + cnt) of at most 16 breakpoints, read as five
+  // aligned 16-byte groups from first & ~3 (the array is padded past its end)
+  const uint32_t lo = 16u * node;
+  const uint32_t cnt = d - lo < 16u ? d - lo : 16u;
+  const uint32_t first = begin + lo, a = first & ~3u, skip = first - a;
+  const uint4 *q = reinterpret_cast<const uint4 *>(bp + a);
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t t = 0; t < 5; ++t) {
+    const uint4 v = q[t];
+    const uint32_t e0 = 4 * t - skip;  // position of v.x within the block (wraps below 0)
+    c += (e0 < cnt && v.x <= k) ? 1u : 0u;
+    c += (e0 + 1 < cnt && v.y <= k) ? 1u : 0u;
+    c += (e0 + 2 < cnt && v.z <= k) ? 1u : 0u;
+    c += (e0 + 3 < cnt && v.w <= k) ? 1u : 0u;
+  }
+  ++*probes;
+  return lo + c;
+}
+
+// Bit of a vertex in a walk's 128-bit visited filter.
+__device__ inline uint32_t lt_filter_bit(uint32_t v) { return (v * 0x9E3779B1u) >> 25; }
+
+// Ascending bitonic sort of 128 keys held two per lane (a[lane], a[lane + 64])
+// across a wave.
+__device__ inline void wave_sort128(uint32_t &x0, uint32_t &x1) {
+  const uint32_t lane = lane_id();
+#pragma unroll 1
+  for (uint32_t k = 2; k <= 128; k <<= 1) {
+#pragma unroll 1
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j == 64) {
+        // partners a[lane] and a[lane + 64] sit in one lane; k == 128 is ascending
+        const uint32_t lo = min(x0, x1), hi = max(x0, x1);
+        x0 = lo;
+        x1 = hi;
+      } else {
+        const bool lower = (lane & j) == 0;
+        const uint32_t p0 = __shfl_xor(x0, static_cast<int>(j), 64);
+        const uint32_t p1 = __shfl_xor(x1, static_cast<int>(j), 64);
+        const bool asc0 = (lane & k) == 0, asc1 = ((lane + 64) & k) == 0;
+        x0 = (lower == asc0) ? min(x0, p0) : max(x0, p0);
+        x1 = (lower == asc1) ? min(x1, p1) : max(x1, p1);
+      }
+    }
+  }
+}
+
+// Persistent walkers: every lane runs its own walk and, when it ends, takes
+// the next sample id from its wave's batch (claimed kClaim at a time from a
+// device counter), so short walks never wait for the long ones of the wave.  Each round
+// takes one step of every busy lane (a breakpoint search), or scans up to
+// UNROLL in-edges of a row without breakpoints with 16-byte loads issued
+// together.  A walk's visited list lives in LDS for its first CAP vertices
+// and in a per-lane HBM spill area for the next SPILL, with a 128-bit filter
+// in registers so the list is searched only when the filter has the vertex;
+// longer walks are handed to lt_long_kernel.  Finished lists up to CAP are
+// sorted four at a time in 16-lane register groups; longer ones are copied
+// by the whole wave and sorted by lt_sort_kernel.
+// Eight waves per SIMD: the walk is latency-bound, and at 64 registers the
+// kernel needs no scratch.
 template <int BT, int CAP, int SPILL, int UNROLL>
-__global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
+__global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8)))
+lt_walk_kernel(LtParams P) {
   static_assert(UNROLL % 4 == 0, "groups of four weights per load");
+  static_assert(CAP <= 16 && CAP + SPILL <= 128, "16-lane group sort and 128-key wave sort");
   __shared__ uint32_t s_list[CAP * BT];
   uint32_t *list = s_list + threadIdx.x;  // element i < CAP at list[i * BT]
   uint32_t *spill = P.spill + (static_cast<uint64_t>(blockIdx.x) * BT + threadIdx.x) * SPILL;
@@ -63,33 +170,70 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   const uint32_t lane = lane_id();
   bool busy = false, exhausted = false;
   uint64_t gid = 0, z = 0;
-  uint32_t len = 0, e = 0, end = 0;
-  float t = 0.0f;
-  unsigned long long bytes = 0;  // 8 per visited vertex, 8 per scanned edge, 4 per output
-  unsigned long long edges = 0;
+  // sample ids claimed by this wave and not yet handed out (wave-uniform);
+  // ids are handed out in increasing order, so a wave only stops once every
+  // id it still holds is past `count`
+  uint64_t claim_lo = 0, claim_hi = 0;
+  uint32_t len = 0, e = 0, end = 0, rflags = 0, kthr = 0, ridx = 0;
+  uint64_t filt0 = 0, filt1 = 0;  // visited filter, bits 0..63 and 64..127
+  auto filter_add = [&](uint32_t v) {
+    const uint32_t b = lt_filter_bit(v);
+    if (b < 64)
+      filt0 |= 1ull << b;
+    else
+      filt1 |= 1ull << (b - 64);
+  };
+  auto filter_has = [&](uint32_t v) -> bool {
+    const uint32_t b = lt_filter_bit(v);
+    return ((b < 64 ? filt0 >> b : filt1 >> (b - 64)) & 1ull) != 0;
+  };
+  float t = 0.0f;                 // float chain of rows without breakpoints
+  // work counters: in-edges the subtraction loop scans, index nodes and
+  // breakpoint blocks read, vertices written out (every visited vertex is)
+  uint32_t edges = 0, probes = 0, outs = 0;
+
+  auto enter = [&](uint32_t v) {
+    const VertexRec r = P.vrec[v];
+    kthr = lt_threshold(r, z);
+    t = static_cast<float>(kthr) * 0x1p-24f;
+    e = r.begin;
+    end = r.end;
+    rflags = r.flags;
+    ridx = r.uthr;
+  };
 
   for (;;) {
-    // ---- refill idle lanes
+    // ---- refill idle lanes from the wave's batch of claimed sample ids
     const bool want = !busy && !exhausted;
     const uint64_t wm = __ballot(want);
     if (wm) {
-      const uint32_t leader = __ffsll(static_cast<unsigned long long>(wm)) - 1;
-      unsigned long long base = 0;
-      if (lane == leader)
-        base = atomicAdd(&P.ctl[kCtlGroupCounter], static_cast<unsigned long long>(__popcll(wm)));
-      base = __shfl(base, static_cast<int>(leader), 64);
+      const uint32_t need = __popcll(wm);
+      const uint32_t rank = __popcll(wm & lanes_below());
+      const uint64_t have = claim_hi - claim_lo;
+      uint64_t id;
+      if (have >= need) {
+        id = claim_lo + rank;
+        claim_lo += need;
+      } else {
+        // a new batch; the rest of the current one is handed out first
+        const uint32_t leader = __ffsll(static_cast<unsigned long long>(wm)) - 1;
+        unsigned long long nb = 0;
+        if (lane == leader) nb = atomicAdd(&P.ctl[kCtlGroupCounter], kClaim);
+        nb = __shfl(nb, static_cast<int>(leader), 64);
+        id = rank < have ? claim_lo + rank : nb + (rank - have);
+        claim_lo = nb + (need - have);
+        claim_hi = nb + kClaim;
+      }
       if (want) {
-        gid = base + __popcll(wm & lanes_below());
+        gid = id;
         if (gid < P.count) {
           z = sample_z(P.seed, P.first + gid);
           const uint32_t root = uniform_index(z, P.n);
           list[0] = root;
           len = 1;
-          const VertexRec r = P.vrec[root];
-          t = u01(r.ja * z + r.jb);
-          e = r.begin;
-          end = r.end;
-          bytes += 8;
+          filt0 = filt1 = 0;
+          filter_add(root);
+          enter(root);
           busy = true;
         } else {
           exhausted = true;
@@ -98,21 +242,31 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
     }
     if (__ballot(busy) == 0) break;
 
-    // ---- one scan round
+    // ---- one step (or one scan round of a row without breakpoints)
     bool done = false, is_long = false;
     if (busy) {
+      bool found = false;
       if (e >= end) {
         done = true;  // no in-edge crosses the threshold: the walk stops
+      } else if (!(rflags & kRowScalar)) {
+        const uint32_t d = end - e;  // e is the row start
+        const uint32_t c = lt_cross(P.bp, P.bidx, e, d, ridx, kthr, &probes);
+        edges += c < d ? c + 1 : d;
+        if (c < d) {
+          found = true;
+          e += c + 1;
+        } else {
+          done = true;
+        }
       } else {
-        const uint32_t base = e & ~3u;      // 16-byte aligned group of 4 weights
+        const uint32_t base = e & ~3u;  // 16-byte aligned group of 4 weights
         const uint32_t last_quad = (end - 1) >> 2;
         uint4 q[UNROLL / 4];
 #pragma unroll
         for (int kk = 0; kk < UNROLL / 4; ++kk)
           q[kk] = val4[min((base >> 2) + kk, last_quad)];
-        const uint32_t lo = e - base;       // 0..3
+        const uint32_t lo = e - base;  // 0..3
         const uint32_t hi = min(static_cast<uint32_t>(UNROLL), end - base);
-        bool found = false;
         uint32_t stop = hi;
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
@@ -127,49 +281,36 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
             }
           }
         }
-        const uint32_t scanned = stop - lo;
-        bytes += 8ull * scanned;
-        edges += scanned;
+        edges += stop - lo;
         e = base + stop;
-        if (found) {
-          const uint32_t u = P.ids[e - 1];  // the chosen in-neighbour
-          bool seen = false;
+        if (!found && e >= end) done = true;
+      }
+      if (found) {
+        const uint32_t u = P.ids[e - 1];  // the chosen in-neighbour
+        bool seen = false;
+        if (filter_has(u))
           for (uint32_t i = 0; i < len; ++i)
-            if (at(i) == u) { seen = true; break; }
-          if (seen) {
-            done = true;
-          } else if (len == CAP + SPILL) {
-            done = true;
-            is_long = true;
-          } else {
-            at(len) = u;
-            ++len;
-            const VertexRec r = P.vrec[u];
-            t = u01(r.ja * z + r.jb);
-            e = r.begin;
-            end = r.end;
-            bytes += 8;
-          }
-        } else if (e >= end) {
+            if (at(i) == u) {
+              seen = true;
+              break;
+            }
+        if (seen) {
           done = true;
+        } else if (len == CAP + SPILL) {
+          done = true;
+          is_long = true;
+        } else {
+          at(len) = u;
+          ++len;
+          filter_add(u);
+          enter(u);
         }
       }
     }
 
-    // ---- retire finished walks: sort, reserve pool space per wave, write
+    // ---- retire finished walks: reserve pool space per wave, sort, write
     if (__ballot(done)) {
       const bool emit = done && !is_long;
-      if (emit) {
-        for (uint32_t i = 1; i < len; ++i) {
-          const uint32_t key = at(i);
-          uint32_t j = i;
-          while (j > 0 && at(j - 1) > key) {
-            at(j) = at(j - 1);
-            --j;
-          }
-          at(j) = key;
-        }
-      }
       const uint32_t size = emit ? len : 0;
       const uint32_t incl = wave_inclusive_scan(size);
       const uint32_t total = __shfl(incl, 63, 64);
@@ -179,6 +320,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
       base = __shfl(base, 0, 64);
       const bool fail = total && base + total > P.cap_data;
       if (fail && lane == 0) atomicOr(&P.ctl[kCtlOverflow], 1ull);
+      const unsigned long long off = base + incl - size;
       if (done) {
         const uint64_t s = P.first + gid;
         if (is_long) {
@@ -188,23 +330,73 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         } else if (fail) {
           P.set_len[s] = kLenFail;
         } else {
-          const unsigned long long off = base + incl - size;
-          for (uint32_t i = 0; i < len; ++i) P.data[off + i] = at(i);
+          if (len > static_cast<uint32_t>(CAP)) {
+            // written unsorted below; lt_sort_kernel sorts it in place
+            const unsigned long long idx = atomicAdd(&P.ctl[kCtlSortCount], 1ull);
+            P.long_ids[P.count - 1 - idx] = static_cast<uint32_t>(gid);
+          }
           P.set_off[s] = off;
           P.set_len[s] = len;
-          bytes += 4ull * len;
+          outs += len;
         }
+      }
+      const uint32_t tid0 = threadIdx.x & ~63u;
+      // lists of at most CAP ids: four at a time, one per 16-lane group,
+      // sorted in registers and written in order
+      uint64_t sm = __ballot(emit && !fail && len <= static_cast<uint32_t>(CAP));
+      const uint32_t g = lane >> 4, gi = lane & 15u;
+      while (sm) {
+        uint32_t pick = 64;  // this group's lane, 64 = none
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+          const uint32_t l = sm ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(sm)) - 1) : 64u;
+          if (sm) sm &= sm - 1;
+          if (q == g) pick = l;
+        }
+        const uint32_t src = pick < 64 ? pick : 0u;
+        const uint32_t n_g = static_cast<uint32_t>(__shfl(static_cast<int>(len), static_cast<int>(src), 64));
+        const unsigned long long off_g = __shfl(off, static_cast<int>(src), 64);
+        const bool have = pick < 64 && gi < n_g;
+        uint32_t x = have ? s_list[gi * BT + tid0 + pick] : 0xFFFFFFFFu;
+#pragma unroll
+        for (uint32_t k = 2; k <= 16; k <<= 1) {
+#pragma unroll
+          for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t y = __shfl_xor(x, static_cast<int>(j), 64);
+            const bool lower = (gi & j) == 0, asc = (gi & k) == 0;
+            x = (lower == asc) ? min(x, y) : max(x, y);
+          }
+        }
+        if (have) P.data[off_g + gi] = x;
+      }
+      // longer lists: copied by the whole wave, sorted by lt_sort_kernel
+      uint64_t lm = __ballot(emit && !fail && len > static_cast<uint32_t>(CAP));
+      while (lm) {
+        const uint32_t L = __ffsll(static_cast<unsigned long long>(lm)) - 1;
+        lm &= lm - 1;
+        const uint32_t n_l = static_cast<uint32_t>(__shfl(static_cast<int>(len), static_cast<int>(L), 64));
+        const unsigned long long off_l = __shfl(off, static_cast<int>(L), 64);
+        const uint32_t *spill_l = P.spill + (static_cast<uint64_t>(blockIdx.x) * BT + tid0 + L) * SPILL;
+        for (uint32_t i = lane; i < n_l; i += 64)
+          P.data[off_l + i] = i < static_cast<uint32_t>(CAP) ? s_list[i * BT + tid0 + L] : spill_l[i - CAP];
+      }
+      if (done) {
         busy = false;
         len = 0;
       }
     }
   }
+  unsigned long long e64 = edges, p64 = probes, o64 = outs;
   for (int d = 32; d > 0; d >>= 1) {
-    bytes += __shfl_down(bytes, d, 64);
-    edges += __shfl_down(edges, d, 64);
+    e64 += __shfl_down(e64, d, 64);
+    p64 += __shfl_down(p64, d, 64);
+    o64 += __shfl_down(o64, d, 64);
   }
-  if (lane == 0 && bytes) atomicAdd(&P.ctl[kCtlStatBytes], bytes);
-  if (lane == 0 && edges) atomicAdd(&P.ctl[kCtlStatEdges], edges);
+  // algorithmic bytes of the subtraction loop: 8 per visited vertex (its CSR
+  // offsets), 8 per scanned in-edge (id + weight), 4 per output id
+  if (lane == 0 && (e64 || o64)) atomicAdd(&P.ctl[kCtlStatBytes], 12ull * o64 + 8ull * e64);
+  if (lane == 0 && e64) atomicAdd(&P.ctl[kCtlStatEdges], e64);
+  if (lane == 0 && p64) atomicAdd(&P.ctl[kCtlStatCoins], p64);
 }
 
 template <int BT, int CAP, int SPILL, int UNROLL>
@@ -250,23 +442,31 @@ __global__ void lt_long_kernel(LtParams P, uint32_t num_long, uint32_t lo,
   const uint64_t z = sample_z(P.seed, s);
   uint32_t v = uniform_index(z, P.n);
   uint32_t len = 0;
+  uint32_t probes = 0;
   list[len++] = v;
   bm[v >> 5] |= 1u << (v & 31);
   for (;;) {
     const VertexRec r = P.vrec[v];
-    float t = u01(r.ja * z + r.jb);
-    uint32_t next = kLenFail;
-    for (uint32_t e = r.begin; e < r.end; ++e) {
-      t -= __uint_as_float(P.val[e]);
-      if (t > 0) continue;
-      const uint32_t u = P.ids[e];
-      if (!((bm[u >> 5] >> (u & 31)) & 1u)) next = u;
-      break;
+    const uint32_t k = lt_threshold(r, z);
+    uint32_t e = r.end;
+    if (!(r.flags & kRowScalar)) {
+      e = r.begin + lt_cross(P.bp, P.bidx, r.begin, r.end - r.begin, r.uthr, k, &probes);
+    } else {
+      float t = static_cast<float>(k) * 0x1p-24f;
+      for (uint32_t i = r.begin; i < r.end; ++i) {
+        t -= __uint_as_float(P.val[i]);
+        if (!(t > 0)) {
+          e = i;
+          break;
+        }
+      }
     }
-    if (next == kLenFail) break;
-    list[len++] = next;
-    bm[next >> 5] |= 1u << (next & 31);
-    v = next;
+    if (e >= r.end) break;
+    const uint32_t u = P.ids[e];
+    if ((bm[u >> 5] >> (u & 31)) & 1u) break;
+    list[len++] = u;
+    bm[u >> 5] |= 1u << (u & 31);
+    v = u;
   }
   for (uint32_t i = 0; i < len; ++i) bm[list[i] >> 5] = 0;
   for (uint32_t i = len / 2; i-- > 0;) sift_down(list, i, len);
@@ -288,8 +488,30 @@ __global__ void lt_long_kernel(LtParams P, uint32_t num_long, uint32_t lo,
   P.set_len[s] = len;
 }
 
+// Sorts the long lists the walk kernel recorded, one wave per list, in
+// registers (lists hold at most 128 ids).  Waves stride over the recorded
+// lists, whose count the walk kernel left in the control block.
+__global__ void lt_sort_kernel(LtParams P) {
+  const uint32_t lane = lane_id();
+  const uint64_t nsort = P.ctl[kCtlSortCount];
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x >> 6);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       i < nsort; i += waves) {
+    const uint64_t s = P.first + P.long_ids[P.count - 1 - i];
+    const unsigned long long off = P.set_off[s];
+    const uint32_t len = P.set_len[s];
+    uint32_t x0 = 0xFFFFFFFFu, x1 = 0xFFFFFFFFu;
+    if (lane < len) x0 = P.data[off + lane];
+    if (lane + 64 < len) x1 = P.data[off + lane + 64];
+    wave_sort128(x0, x1);
+    if (lane < len) P.data[off + lane] = x0;
+    if (lane + 64 < len) P.data[off + lane + 64] = x1;
+  }
+}
+
 __global__ void reset_long_kernel(unsigned long long *ctl) {
   ctl[kCtlLongCount] = 0;
+  ctl[kCtlSortCount] = 0;
 }
 
 }  // namespace
@@ -297,12 +519,14 @@ __global__ void reset_long_kernel(unsigned long long *ctl) {
 uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
                    const rip_tune *tune) {
   (void)tune;
-  constexpr int BT = 256, CAP = 32, SPILL = 96, UNROLL = 32;
+  constexpr int BT = 256, CAP = 16, SPILL = 112, UNROLL = 16;
   const rip_graph *g = p->g;
   LtParams P;
   P.vrec = g->d_vrec;
   P.ids = g->d_ids;
   P.val = g->d_val;
+  P.bp = g->d_bp;
+  P.bidx = g->d_bidx;
   P.n = g->n;
   P.seed = p->seed;
   P.first = first;
@@ -332,8 +556,10 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   hipLaunchKernelGGL((lt_walk_kernel<BT, CAP, SPILL, UNROLL>), dim3(blocks), dim3(BT), 0,
                      p->stream, P);
   RIP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(lt_sort_kernel, dim3(256), dim3(256), 0, p->stream, P);
+  RIP_HIP(hipGetLastError());
   RIP_HIP(hipEventRecord(p->ev1, p->stream));
-  uint32_t launches = 1;
+  uint32_t launches = 2;
   unsigned long long nlong = 0;
   RIP_HIP(hipMemcpyAsync(&nlong, p->d_ctl + kCtlLongCount, sizeof(nlong),
                          hipMemcpyDeviceToHost, p->stream));

@@ -158,6 +158,63 @@ def test_lt_long_walks(orc):
     g.close()
 
 
+def lt_hub_graph(kind, n=2500, hubs=8, seed=7):
+    """Hubs that are in- and out-neighbours of every vertex, so walks keep
+    scanning rows of ~n in-edges, plus sparse random edges.  Weight kinds:
+    normalised uniform, dyadic k*2^-q per row (exact grid values and halfway
+    ties), a wide exponent range with subnormals, zeros and -0, and rows with
+    inf / NaN weights that must be scanned one edge at a time."""
+    rng = np.random.default_rng(seed)
+    allv = np.arange(n)
+    src, dst = [], []
+    for h in range(hubs):
+        others = allv[allv != h]
+        src += [others, np.full(len(others), h)]
+        dst += [np.full(len(others), h), others]
+    s = rng.integers(0, n, 6 * n)
+    d = rng.integers(0, n, 6 * n)
+    src.append(s)
+    dst.append(d)
+    s, d = np.concatenate(src), np.concatenate(dst)
+    keep = s != d
+    key = np.unique(s[keep] * n + d[keep])
+    s, d = key // n, key % n
+    deg = np.bincount(d, minlength=n)
+    if kind == "dyadic":
+        q = np.floor(np.log2(deg[d] * 4.0)).astype(np.int64)
+        w = np.ldexp(rng.integers(0, 8, len(s)).astype(np.float64), -q).astype(np.float32)
+    elif kind == "wide":
+        w = np.ldexp(rng.random(len(s)), -rng.integers(0, 40, len(s))).astype(np.float32)
+        w = lt_normalise(n, s, d, w, 0.95)
+        r = rng.random(len(s))
+        w[r < 0.01] = np.ldexp(np.float32(1.0), -130 - rng.integers(0, 19, int((r < 0.01).sum())))
+        w[(r >= 0.01) & (r < 0.03)] = 0.0
+        w[(r >= 0.03) & (r < 0.04)] = -0.0
+    else:
+        w = lt_normalise(n, s, d, rng.random(len(s)).astype(np.float32), 0.97)
+    index, edges, w = transposed_csr(n, s, d, w)
+    if kind == "scalar":
+        for h, bad in ((0, np.inf), (1, np.nan)):
+            b, e = int(index[h]), int(index[h + 1])
+            w[b + (e - b) // 2] = bad
+    return index, edges, w
+
+
+@pytest.mark.parametrize("kind", ["uniform", "dyadic", "wide", "scalar"])
+def test_lt_hub_rows(kind, orc):
+    """Breakpoint searches on long rows give the float chain's exact choice;
+    rows with inf / NaN weights keep the subtraction loop."""
+    index, edges, w = lt_hub_graph(kind)
+    g = R.Graph(index, edges, w, model=R.LT)
+    p = g.pool(9)
+    p.generate_rrr_sets(6000)
+    stats = p.last_stats()
+    assert_same_sets(p.export(), orc.sample(R.LT, index, edges, w, 9, 0, 6000))
+    assert stats["coins"] > 0          # breakpoints were searched
+    p.close()
+    g.close()
+
+
 @pytest.mark.parametrize("model,k,eps", [(R.IC, 5, 0.5), (R.LT, 10, 0.3)])
 def test_imm_matches_oracle(model, k, eps, orc):
     if model == R.IC:
