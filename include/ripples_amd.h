@@ -199,6 +199,22 @@ int rip_csr_from_edges(uint32_t num_nodes, uint64_t num_edges,
                        float scale, uint64_t seed, uint64_t *index,
                        uint32_t *edges, float *weights);
 
+/* --------------------------------------------------------------- spread */
+/* Monte Carlo spread of a seed set: replicas [first, first + count) of the
+ * forward IC or LT process of the graph's model from seeds[0 .. num_seeds)
+ * (internal ids; duplicates allowed).  Replaces simulate() of
+ * include/ripples/diffusion_simulation.h:73-178 as driven by
+ * tools/simulator.cc:100-171.  Replica r draws from sample stream r of
+ * `seed` (the samplers' per-sample construction): element 1 + f is the IC
+ * coin of forward edge f, element 1 + v the LT threshold of vertex v; the
+ * forward edges of u are the transposed entries with source u, in transposed
+ * order.  activated[i] = vertices active at the end of replica first + i,
+ * levels[i] = its steps (IC: breadth-first layers including the seeds'; LT:
+ * iterations including the last, which activates nothing).  The first call
+ * on a graph builds its forward CSR on the device. */
+int rip_spread(rip_graph *g, const uint32_t *seeds, uint32_t num_seeds, uint64_t seed,
+               uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels);
+
 /* ---------------------------------------------------------------- files */
 /* Host transposed CSR read from a file: the arrays rip_graph_create takes,
  * plus the map from internal to original vertex ids.  Replaces loadGraph /

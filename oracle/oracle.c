@@ -21,6 +21,13 @@
  *       values recomputed with Python's math module in the tests.
  *   orc_imm           round structure checked against the closed forms and
  *       against orc_select/orc_sample run step by step.
+ *   orc_spread (IC)   w = 1 -> forward reachability; w = 0 -> the seeds;
+ *       exact expected spread by live-edge enumeration on tiny graphs;
+ *       equals the reverse formulation (v is reached iff v's RRR set, drawn
+ *       with the same coins, meets the seeds) on graphs where both are run.
+ *   orc_spread (LT)   path graphs with unit weights -> deterministic chains;
+ *       exact activation probabilities of single in-edges; threshold rule
+ *       checked against a direct Python restatement on small graphs.
  */
 #include "oracle.h"
 
@@ -544,4 +551,112 @@ uint32_t orc_imm(int model, uint32_t n, const uint64_t *index,
   free(p.offsets);
   free(p.data);
   return got;
+}
+
+/* ------------------------------------------------------------------------ */
+/* forward Monte Carlo spread: diffusion_simulation.h:73-178                 */
+/* ------------------------------------------------------------------------ */
+void orc_spread(int model, uint32_t n, const uint64_t *index, const uint32_t *src,
+                const float *w, const uint32_t *seeds, uint32_t nseeds, uint64_t seed,
+                uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels,
+                int nthreads) {
+  const uint64_t m = index[n];
+  /* forward CSR: row u = targets of the transposed entries with source u */
+  uint64_t *fidx = (uint64_t *)calloc((size_t)n + 1, sizeof(uint64_t));
+  uint32_t *fdst = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(m ? m : 1));
+  float *fw = (float *)malloc(sizeof(float) * (size_t)(m ? m : 1));
+  for (uint64_t i = 0; i < m; ++i) fidx[src[i] + 1]++;
+  for (uint32_t u = 0; u < n; ++u) fidx[u + 1] += fidx[u];
+  {
+    uint64_t *cur = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n ? n : 1));
+    memcpy(cur, fidx, sizeof(uint64_t) * n);
+    for (uint32_t v = 0; v < n; ++v)
+      for (uint64_t i = index[v]; i < index[v + 1]; ++i) {
+        const uint64_t f = cur[src[i]]++;
+        fdst[f] = v;
+        fw[f] = w[i];
+      }
+    free(cur);
+  }
+  const aff P = aff_pow(MASTER, 2 * ORC_SAMPLE_STRIDE);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel
+  {
+    uint8_t *act = (uint8_t *)calloc((size_t)n ? n : 1, 1);
+    uint8_t *cand = (uint8_t *)calloc((size_t)n ? n : 1, 1);
+    uint32_t *queue = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n ? n : 1));
+    uint32_t *cl = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n ? n : 1));
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t ri = 0; ri < (int64_t)count; ++ri) {
+      const uint64_t z = sample_z(seed, first + (uint64_t)ri);
+      uint64_t len = 0;
+      for (uint32_t i = 0; i < nseeds; ++i)
+        if (!act[seeds[i]]) {
+          act[seeds[i]] = 1;
+          queue[len++] = seeds[i];
+        }
+      uint64_t lo = 0, hi = len;
+      uint32_t lev = 0;
+      if (model == ORC_IC) {
+        while (lo < hi) {
+          ++lev;
+          for (uint64_t q = lo; q < hi; ++q) {
+            const uint32_t u = queue[q];
+            uint64_t x = aff_apply(aff_pow(P, 1 + fidx[u]), z);
+            for (uint64_t f = fidx[u]; f < fidx[u + 1]; ++f, x = aff_apply(P, x)) {
+              const uint32_t v = fdst[f];
+              if (!act[v] && orc_u01(x) <= fw[f]) {
+                act[v] = 1;
+                queue[len++] = v;
+              }
+            }
+          }
+          lo = hi;
+          hi = len;
+        }
+      } else {
+        do {
+          ++lev;
+          uint64_t nc = 0;
+          for (uint64_t q = lo; q < hi; ++q) {
+            const uint32_t u = queue[q];
+            for (uint64_t f = fidx[u]; f < fidx[u + 1]; ++f) {
+              const uint32_t v = fdst[f];
+              if (!act[v] && !cand[v]) {
+                cand[v] = 1;
+                cl[nc++] = v;
+              }
+            }
+          }
+          const uint64_t before = len;
+          for (uint64_t c = 0; c < nc; ++c) {
+            const uint32_t v = cl[c];
+            float total = 0.0f;
+            for (uint64_t i = index[v]; i < index[v + 1]; ++i)
+              if (act[src[i]]) total += w[i];
+            const float th = orc_u01(aff_apply(aff_pow(P, 1 + (uint64_t)v), z));
+            if (total >= th) queue[len++] = v;
+          }
+          for (uint64_t c = 0; c < nc; ++c) cand[cl[c]] = 0;
+          for (uint64_t q = before; q < len; ++q) act[queue[q]] = 1;
+          lo = before;
+          hi = len;
+        } while (lo < hi);
+      }
+      activated[ri] = len;
+      levels[ri] = lev;
+      for (uint64_t q = 0; q < len; ++q) act[queue[q]] = 0;
+    }
+    free(act);
+    free(cand);
+    free(queue);
+    free(cl);
+  }
+  free(fidx);
+  free(fdst);
+  free(fw);
 }

@@ -107,6 +107,24 @@ uint32_t orc_imm(int model, uint32_t n, const uint64_t *index,
                  uint64_t seed, int nthreads, uint32_t *seeds,
                  orc_imm_record *rec);
 
+/* ---- forward Monte Carlo spread (diffusion_simulation.h:73-178) ----
+ * Replica r of a seed set draws from the sample stream r of `seed` (element j
+ * = master output 1 + 2(r + jS)): element 1 + f is the IC coin of forward
+ * edge f, element 1 + v the LT threshold of vertex v.  The forward CSR lists,
+ * for each vertex u, the targets v of the transposed entries with source u,
+ * in transposed-position order.
+ *   IC: breadth-first from the seeds; edge f is live when u01 <= w; levels =
+ *       breadth-first layers processed (the seed layer included).
+ *   LT: per level, every inactive out-neighbour of the vertices activated in
+ *       the previous level sums, in transposed CSR order, the weights of its
+ *       active in-neighbours and activates when the sum >= its threshold;
+ *       levels = iterations, the last one (activating nothing) included.
+ * activated[i] / levels[i] belong to replica first + i. */
+void orc_spread(int model, uint32_t n, const uint64_t *index, const uint32_t *src,
+                const float *w, const uint32_t *seeds, uint32_t nseeds, uint64_t seed,
+                uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels,
+                int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

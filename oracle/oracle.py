@@ -84,6 +84,8 @@ def lib():
                           C.c_double, C.c_uint64, C.c_int, _u32p,
                           C.POINTER(ImmRecord)]
     L.orc_imm.restype = C.c_uint32
+    L.orc_spread.argtypes = [C.c_int, C.c_uint32, _u64p, _u32p, _f32p, _u32p, C.c_uint32,
+                             C.c_uint64, C.c_uint64, C.c_uint64, _u64p, _u32p, C.c_int]
     _LIB = L
     return L
 
@@ -132,6 +134,22 @@ def count(n, offsets, vertices):
     out = np.zeros(n, np.uint32)
     lib().orc_count(n, len(offsets) - 1, offsets, vertices, out)
     return out
+
+
+def spread(model, index, src, w, seeds, seed, first, count, threads=None):
+    """Forward Monte Carlo spread of `seeds`, replicas [first, first+count):
+    (activated u64[count], levels u32[count])."""
+    index, src, w = _csr(index, src, w)
+    n = len(index) - 1
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    if len(src) == 0:
+        src, w = np.zeros(1, np.uint32), np.zeros(1, np.float32)
+    act = np.zeros(max(count, 1), np.uint64)
+    lev = np.zeros(max(count, 1), np.uint32)
+    threads = threads or min(8, os.cpu_count() or 1)
+    lib().orc_spread(model, n, index, src, w, seeds if len(seeds) else np.zeros(1, np.uint32),
+                     len(seeds), seed, first, count, act, lev, threads)
+    return act[:count], lev[:count]
 
 
 def imm(model, index, src, w, k, eps, seed, threads=None):

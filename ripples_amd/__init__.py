@@ -118,6 +118,8 @@ def lib():
                                  C.c_uint64]
     L.rip_csr_from_edges.argtypes = [C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_int,
                                      C.c_float, C.c_uint64, _u64p, _u32p, _f32p]
+    L.rip_spread.argtypes = [vp, _u32p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64,
+                             _u64p, _u32p]
     hp = C.POINTER(C.c_void_p)
     L.rip_load_edge_list.argtypes = [C.c_char_p, C.POINTER(LoadOptions), hp]
     L.rip_load_binary.argtypes = [C.c_char_p, C.c_float, hp]
@@ -269,6 +271,16 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+    def spread(self, seeds, replicas, seed=0, first=0):
+        """Monte Carlo spread of `seeds` (diffusion_simulation.h:73-178):
+        (activated u64[replicas], levels u32[replicas])."""
+        seeds = np.ascontiguousarray(seeds, np.uint32)
+        act = np.zeros(max(replicas, 1), np.uint64)
+        lev = np.zeros(max(replicas, 1), np.uint32)
+        _check(lib().rip_spread(self._h, seeds if len(seeds) else np.zeros(1, np.uint32),
+                                len(seeds), seed, first, replicas, act, lev))
+        return act[:replicas], lev[:replicas]
 
     def pool(self, seed=0):
         return Pool(self, seed)

@@ -229,6 +229,14 @@ int rip_csr_from_edges(uint32_t num_nodes, uint64_t num_edges, const uint32_t *s
   });
 }
 
+int rip_spread(rip_graph *g, const uint32_t *seeds, uint32_t num_seeds, uint64_t seed,
+               uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels) {
+  return guarded([&] {
+    RIP_REQUIRE(g, RIP_EINVAL, "graph is NULL");
+    rip::spread_run(g, seeds, num_seeds, seed, first, count, activated, levels);
+  });
+}
+
 int rip_load_edge_list(const char *path, const rip_load_options *opts, rip_host_graph **out) {
   return guarded([&] {
     RIP_REQUIRE(out && opts, RIP_EINVAL, "opts or out is NULL");

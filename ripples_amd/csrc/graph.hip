@@ -221,6 +221,10 @@ void graph_destroy(rip_graph *g) {
   dev_free(g->lt.d_spill);
   dev_free(g->d_bp);
   dev_free(g->d_bidx);
+  dev_free(g->fwd.d_rec);
+  dev_free(g->fwd.d_dst);
+  dev_free(g->fwd.d_val);
+  dev_free(g->fwd.d_scratch);
 }
 
 }  // namespace rip

@@ -45,6 +45,16 @@ struct WaveScratch {
   uint32_t *d_big = nullptr;              // sample indices left over
 };
 
+// Forward CSR and per-workgroup scratch of the spread simulator, built on
+// first use (spread.hip).
+struct FwdGraph {
+  uint8_t *d_rec = nullptr;       // n forward records
+  uint32_t *d_dst = nullptr;      // m + 1: target of each forward edge
+  uint32_t *d_val = nullptr;      // m + 1: IC threshold of each forward edge
+  uint32_t *d_scratch = nullptr;  // wgs x per_wg words
+  uint64_t wgs = 0, per_wg = 0;
+};
+
 struct LtScratch {
   uint64_t cap_ids = 0;
   uint32_t *d_long_ids = nullptr;      // walks that outgrew the LDS + spill list (from
@@ -87,6 +97,7 @@ struct rip_graph {
   rip::IcSlots ic;
   rip::WaveScratch wave;
   rip::LtScratch lt;
+  rip::FwdGraph fwd;
 };
 
 struct rip_pool {
@@ -210,6 +221,10 @@ void dev_free(T *&p) {
   if (p) (void)hipFree(p);
   p = nullptr;
 }
+
+// spread.hip
+void spread_run(rip_graph *g, const uint32_t *seeds, uint32_t nseeds, uint64_t seed,
+                uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels);
 
 // loaders.cpp
 void load_edge_list(const char *path, const rip_load_options &o, rip_host_graph *g);

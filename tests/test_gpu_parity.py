@@ -245,3 +245,32 @@ def test_scale16_properties(orc):
     assert_same_sets(p.export(17000, 3000), exp)
     p.close()
     g.close()
+
+
+@pytest.mark.parametrize("name", ["karate_ic", "karate_lt", "random_ic", "random_lt",
+                                  "rmat12_wc", "rmat12_lt"])
+def test_spread_matches_oracle(name, orc):
+    """Monte Carlo spread (diffusion_simulation.h:73-178) replica by replica."""
+    model, index, edges, w = GRAPHS[name]()
+    g = R.Graph(index, edges, w, model=model)
+    n = len(index) - 1
+    deg = np.diff(index.astype(np.int64))
+    seeds = np.argsort(-deg)[:5].astype(np.uint32)
+    seeds = np.concatenate([seeds, seeds[:2]])        # duplicates are allowed
+    act, lev = g.spread(seeds, 400, seed=3, first=50)
+    ea, el = orc.spread(model, index, edges, w, seeds, 3, 50, 400)
+    assert np.array_equal(act, ea) and np.array_equal(lev, el)
+    act2, lev2 = g.spread(seeds, 100, seed=3, first=150)   # replica ids are stable
+    assert np.array_equal(act2, ea[100:200]) and np.array_equal(lev2, el[100:200])
+    assert act.min() >= len(set(seeds.tolist())) and act.max() <= n
+    g.close()
+
+
+def test_spread_unit_weights_and_errors():
+    index, edges, w = rmat_csr(12, R.W_CONST, 1.0)
+    g = R.Graph(index, edges, w, model=R.IC)
+    act, lev = g.spread([0], 20)
+    assert len(set(act.tolist())) == 1 and len(set(lev.tolist())) == 1   # w = 1: deterministic
+    with pytest.raises(R.RipplesError, match="out of range"):
+        g.spread([1 << 12], 5)
+    g.close()
