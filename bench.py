@@ -9,6 +9,14 @@ k = 100, epsilon = 0.13.  A step is one sampling pass of `batch` RRR sets
 is resident in HBM before the timed region starts.  IMM wall time is one full
 rip_imm call (theta estimation + final sampling + selection).
 
+Serving (the "serving" object): IMM queries on resident R-MAT graphs (scales
+18/20/22, IC U[0,0.1] and LT-0.9 weightings), k in {10,50,100,200}, epsilon in
+{0.5,0.2,0.13}, seed = arrival counter, answered by the continuous-batching
+server, against the same queries solved back to back with rip_imm: queries/s
+with every query submitted at once, and p50/p99 latency under Poisson arrivals
+(the back-to-back figures are those of a FIFO queue in front of the measured
+solve times).  Every served answer is checked against its solo solve.
+
 The CPU baseline is the oracle's OpenMP restatement of the reference's CPU
 path, timed on a bounded sample of the same workload on this host.
 """
@@ -177,6 +185,114 @@ def measure(name, seed, steps, warmup, batch, tune, skip_imm):
     return res
 
 
+SERVE_K = (10, 50, 100, 200)
+SERVE_EPS = (0.5, 0.2, 0.13)
+
+
+def serve_graphs(scales, seed):
+    """Resident graphs of the serving workload: R-MAT at each scale with the
+    IC U[0,0.1] and the LT normalised-to-0.9 in-weights."""
+    out = []
+    for sc in scales:
+        t = time.time()
+        s, d = R.rmat_edges(sc)
+        n = 1 << sc
+        for model, scheme, ws in ((R.IC, R.W_UNIFORM, 0.1), (R.LT, R.W_LT_NORM, 0.9)):
+            index, edges, w = R.csr_from_edges(n, s, d, scheme, ws, seed)
+            out.append(("rmat%d_%s" % (sc, "ic" if model == R.IC else "lt"),
+                        R.Graph(index, edges, w, model=model)))
+        log("serving graphs scale %d built in %.1fs" % (sc, time.time() - t))
+    return out
+
+
+def fifo_latencies(arrivals, service):
+    """Latencies of one FIFO server that solves the queries back to back."""
+    free, lat = 0.0, []
+    for t, d in zip(arrivals, service):
+        start = max(t, free)
+        free = start + d
+        lat.append(free - t)
+    return lat
+
+
+def served_run(graphs, queries, arrivals, slots, workers, budget):
+    """Submits `queries` at `arrivals` (seconds from the start; None = all at
+    once) and returns (makespan s, latencies s, server counters)."""
+    srv = R.Server(slots=slots, workers=workers, budget_bytes=budget)
+    gids = [srv.add_graph(g) for _, g in graphs]
+    t0 = time.time()
+    ids = []
+    for i, (gi, k, eps, qseed) in enumerate(queries):
+        if arrivals is not None:
+            dt = arrivals[i] - (time.time() - t0)
+            if dt > 0:
+                time.sleep(dt)
+        ids.append(srv.submit(gids[gi], k, eps, qseed))
+    lat, results = [], []
+    for q in ids:
+        seeds, _, ms = srv.poll(q)
+        lat.append(ms * 1e-3)
+        results.append(seeds)
+    span = time.time() - t0
+    st = srv.stats()
+    srv.close()
+    return span, lat, st, results
+
+
+def pct(x, p):
+    return float(np.percentile(np.asarray(x), p))
+
+
+def serve_bench(scales, n_queries, seed, slots, workers, load, budget):
+    """Continuous-batching serving against back-to-back plain solves.
+    Queries: graph uniform over the resident graphs, k in {10,50,100,200},
+    eps in {0.5,0.2,0.13}, seed = arrival counter; Poisson arrivals."""
+    graphs = serve_graphs(scales, seed)
+    rng = np.random.default_rng(seed + 1)
+    queries = [(int(rng.integers(len(graphs))), int(rng.choice(SERVE_K)),
+                float(rng.choice(SERVE_EPS)), i) for i in range(n_queries)]
+    for gi in range(len(graphs)):      # first-touch costs outside the timing
+        R.imm(graphs[gi][1], 10, 0.5, seed=10 ** 6)
+    service, plain = [], []
+    for gi, k, eps, qseed in queries:
+        t = time.time()
+        seeds, _ = R.imm(graphs[gi][1], k, eps, seed=qseed)
+        service.append(time.time() - t)
+        plain.append(seeds)
+    seq_qps = n_queries / sum(service)
+    log("serving: %d plain solves back to back in %.2fs (%.2f queries/s)" % (
+        n_queries, sum(service), seq_qps))
+    span, _, st, res = served_run(graphs, queries, None, slots, workers, budget)
+    assert all(np.array_equal(a, b) for a, b in zip(res, plain)), "served != solo"
+    sat_qps = n_queries / span
+    log("serving: all %d submitted at once, done in %.2fs (%.2f queries/s); %s" % (
+        n_queries, span, sat_qps, st))
+    out = {"workload": "C4: R-MAT scales %s, IC U[0,0.1] and LT 0.9 graphs, k in %s, eps in %s, "
+                       "seed = arrival counter" % (list(scales), list(SERVE_K),
+                                                    list(SERVE_EPS)),
+           "queries": n_queries, "slots": slots, "workers": workers, "budget_bytes": budget,
+           "plain_queries_per_s": seq_qps, "served_saturated_queries_per_s": sat_qps,
+           "saturated_speedup": sat_qps / seq_qps, "saturated_counters": st,
+           "results_equal_solo": True, "poisson": []}
+    for f in load:
+        lam = f * seq_qps
+        arr = np.cumsum(rng.exponential(1.0 / lam, n_queries)).tolist()
+        span, lat, st, res = served_run(graphs, queries, arr, slots, workers, budget)
+        assert all(np.array_equal(a, b) for a, b in zip(res, plain)), "served != solo"
+        fifo = fifo_latencies(arr, service)
+        row = {"rate_per_s": lam, "load_vs_plain_capacity": f,
+               "served_p50_ms": pct(lat, 50) * 1e3, "served_p99_ms": pct(lat, 99) * 1e3,
+               "served_queries_per_s": n_queries / span,
+               "plain_fifo_p50_ms": pct(fifo, 50) * 1e3, "plain_fifo_p99_ms": pct(fifo, 99) * 1e3}
+        log("serving at %.2f x plain capacity: served p50/p99 %.1f/%.1f ms, plain FIFO %.1f/%.1f ms"
+            % (f, row["served_p50_ms"], row["served_p99_ms"], row["plain_fifo_p50_ms"],
+               row["plain_fifo_p99_ms"]))
+        out["poisson"].append(row)
+    for _, g in graphs:
+        g.close()
+    return out
+
+
 def copy_probe(gib):
     """One device-to-device copy of `gib` GiB: the reference kernel for scaling
     rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes."""
@@ -210,6 +326,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--tune", default="", help='sampler tuning, e.g. "sampler=2,block=256"')
     ap.add_argument("--skip-imm", action="store_true")
+    ap.add_argument("--serve-scales", default="18,20,22",
+                    help="R-MAT scales of the serving workload ('' skips it)")
+    ap.add_argument("--serve-queries", type=int, default=48)
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0,
+                    help="HBM budget of the resident pools")
+    ap.add_argument("--serve-load", default="0.8,2.0",
+                    help="Poisson rates as multiples of the plain back-to-back capacity")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device copy of this many GiB (counter calibration)")
     a = ap.parse_args()
@@ -259,6 +384,11 @@ def main():
         sec.pop("graph")
         out["secondary"] = {k: sec[k] for k in ("value", "ms_per_step", "config", "imm_wall_s",
                                                 "imm", "pool", "kernel_counters", "roofline")}
+    if a.serve_scales:
+        out["serving"] = serve_bench([int(x) for x in a.serve_scales.split(",")],
+                                     a.serve_queries, a.seed, a.serve_slots, a.serve_workers,
+                                     [float(x) for x in a.serve_load.split(",")],
+                                     int(a.serve_budget_gib * (1 << 30)))
     print(json.dumps(out), flush=True)
     return 0
 

@@ -215,6 +215,46 @@ int rip_csr_from_edges(uint32_t num_nodes, uint64_t num_edges,
 int rip_spread(rip_graph *g, const uint32_t *seeds, uint32_t num_seeds, uint64_t seed,
                uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels);
 
+/* -------------------------------------------------------------- serving */
+/* IMM queries served against resident graphs with continuous batching (no
+ * counterpart in the reference, which runs one IMM per process).  A query
+ * (graph, k, epsilon, seed) returns the seeds rip_imm(g, k, epsilon, seed)
+ * returns alone.  RRR pools stay resident, one per (graph, seed): a query
+ * selects only on the pool prefixes a solo run draws (theta'(x) per
+ * estimation round, then theta), so prefixes drawn for earlier or concurrent
+ * queries with the same key serve it unchanged and such queries share each
+ * sampling launch.  Queries are admitted from the arrival order while fewer
+ * than `slots` are active and, when a new pool is needed, while the resident
+ * pools stay under `budget_bytes` (0 = no limit; idle pools are evicted least
+ * recently used first).  `workers` threads step different pools concurrently,
+ * each pool on its own stream.  In a served query's record, rrr_elements
+ * counts the shared pool and the timings are the query's own; num_sets is the
+ * pool size a solo run would have drawn.  Graphs must outlive the server. */
+typedef struct rip_server rip_server;
+int rip_server_create(uint32_t slots, uint32_t workers, uint64_t budget_bytes, rip_server **out);
+int rip_server_add_graph(rip_server *s, rip_graph *g, uint32_t *graph_id);
+int rip_server_submit(rip_server *s, uint32_t graph_id, uint32_t k, double epsilon,
+                      uint64_t seed, uint64_t *query_id);
+/* Waits up to wait_ms for the query; *done = 1 when its result was written
+ * (seeds[0 .. *num_seeds), the record, the latency from submission), after
+ * which the query is forgotten; *done = 0 on time-out. */
+int rip_server_poll(rip_server *s, uint64_t query_id, double wait_ms, int *done, uint32_t *seeds,
+                    uint32_t cap, uint32_t *num_seeds, rip_imm_record *rec, double *latency_ms);
+/* Counters in this order (rip_server_stats copies the first `count`). */
+enum {
+  RIP_SERVER_LAUNCHES = 0,   /* sampling launches */
+  RIP_SERVER_SETS_DRAWN,     /* RRR sets drawn */
+  RIP_SERVER_SELECTIONS,     /* selector runs */
+  RIP_SERVER_POOL_BYTES,     /* device bytes of the resident pools */
+  RIP_SERVER_POOLS,          /* resident pools */
+  RIP_SERVER_REUSED,         /* queries admitted onto an existing pool */
+  RIP_SERVER_EVICTED,        /* pools evicted for the budget */
+  RIP_SERVER_STAT_COUNT
+};
+int rip_server_stats(rip_server *s, uint64_t *out, uint32_t count);
+/* Finishes every submitted query, then frees the pools. */
+int rip_server_destroy(rip_server *s);
+
 /* ---------------------------------------------------------------- files */
 /* Host transposed CSR read from a file: the arrays rip_graph_create takes,
  * plus the map from internal to original vertex ids.  Replaces loadGraph /

@@ -16,7 +16,7 @@ import numpy as np
 __all__ = [
     "IC", "LT", "Tune", "ImmRecord", "Graph", "Pool", "RipplesError", "lib",
     "device_count", "rmat_edges", "csr_from_edges", "imm",
-    "W_UNIFORM", "W_CASCADE", "W_LT_NORM", "W_CONST",
+    "W_UNIFORM", "W_CASCADE", "W_LT_NORM", "W_CONST", "Server", "SERVER_STATS",
 ]
 
 IC, LT = 0, 1
@@ -130,6 +130,15 @@ def lib():
                                         C.POINTER(C.POINTER(C.c_float)),
                                         C.POINTER(C.POINTER(C.c_uint32))]
     L.rip_host_graph_destroy.argtypes = [vp]
+    L.rip_server_create.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(vp)]
+    L.rip_server_add_graph.argtypes = [vp, vp, C.POINTER(C.c_uint32)]
+    L.rip_server_submit.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint64,
+                                    C.POINTER(C.c_uint64)]
+    L.rip_server_poll.argtypes = [vp, C.c_uint64, C.c_double, C.POINTER(C.c_int), _u32p,
+                                  C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(ImmRecord),
+                                  C.POINTER(C.c_double)]
+    L.rip_server_stats.argtypes = [vp, _u64p, C.c_uint32]
+    L.rip_server_destroy.argtypes = [vp]
     _LIB = L
     return L
 
@@ -383,3 +392,59 @@ def imm(graph, k, epsilon, seed=0, tune=None):
     _check(lib().rip_imm(graph.handle, k, epsilon, seed, _tune(tune), seeds,
                          C.byref(rec)))
     return seeds[:rec.num_seeds], rec.as_dict()
+
+
+SERVER_STATS = ["launches", "sets_drawn", "selections", "pool_bytes", "pools", "reused",
+                "evicted"]
+
+
+class Server:
+    """IMM queries against resident graphs with continuous batching.  A query
+    (graph, k, epsilon, seed) returns the seeds of imm(graph, k, epsilon, seed)."""
+
+    def __init__(self, slots=8, workers=4, budget_bytes=0):
+        h = C.c_void_p()
+        _check(lib().rip_server_create(slots, workers, budget_bytes, C.byref(h)))
+        self._h = h
+        self._graphs = []
+
+    def add_graph(self, graph):
+        gid = C.c_uint32()
+        _check(lib().rip_server_add_graph(self._h, graph.handle, C.byref(gid)))
+        self._graphs.append(graph)  # the graph must outlive the server
+        return gid.value
+
+    def submit(self, graph_id, k, epsilon, seed=0):
+        qid = C.c_uint64()
+        _check(lib().rip_server_submit(self._h, graph_id, k, epsilon, seed, C.byref(qid)))
+        return qid.value
+
+    def poll(self, query_id, wait_ms=-1.0, cap=None):
+        """(seeds, record dict, latency ms) once the query is done, else None.
+        wait_ms < 0 waits without limit."""
+        cap = cap or 1 << 16
+        seeds = np.zeros(cap, np.uint32)
+        done, ns, lat = C.c_int(), C.c_uint32(), C.c_double()
+        rec = ImmRecord()
+        wait = 1e12 if wait_ms < 0 else wait_ms
+        _check(lib().rip_server_poll(self._h, query_id, wait, C.byref(done), seeds, cap,
+                                     C.byref(ns), C.byref(rec), C.byref(lat)))
+        if not done.value:
+            return None
+        return seeds[:ns.value].copy(), rec.as_dict(), lat.value
+
+    def stats(self):
+        out = np.zeros(len(SERVER_STATS), np.uint64)
+        _check(lib().rip_server_stats(self._h, out, len(out)))
+        return dict(zip(SERVER_STATS, (int(v) for v in out)))
+
+    def close(self):
+        if self._h:
+            lib().rip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

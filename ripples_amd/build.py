@@ -1,4 +1,4 @@
-"""Build libripples_amd.so (HIP for gfx950 + host C++) and the imm tool in-tree.
+"""Build libripples_amd.so (HIP for gfx950 + host C++) and the command-line tools in-tree.
 
 Usage: python -m ripples_amd.build [--jobs N]
 """
@@ -65,13 +65,12 @@ def build(jobs=8, verbose=False):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-lpthread"])
     os.replace(tmp, LIB)
-    # command-line tool, linked against the C ABI only
-    tool = os.path.join(TOOLS, "imm.cpp")
-    if os.path.exists(tool):
+    # command-line tools, linked against the C ABI only
+    for tool in sorted(f for f in os.listdir(TOOLS) if f.endswith(".cpp")) if os.path.isdir(TOOLS) else []:
         os.makedirs(BIN_DIR, exist_ok=True)
-        exe = os.path.join(BIN_DIR, "imm")
+        exe = os.path.join(BIN_DIR, tool[:-4])
         _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
-              tool, "-o", exe + ".tmp", "-L", LIB_DIR, "-lripples_amd",
+              os.path.join(TOOLS, tool), "-o", exe + ".tmp", "-L", LIB_DIR, "-lripples_amd",
               "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"])
         os.replace(exe + ".tmp", exe)
     return LIB

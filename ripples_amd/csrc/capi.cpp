@@ -237,6 +237,51 @@ int rip_spread(rip_graph *g, const uint32_t *seeds, uint32_t num_seeds, uint64_t
   });
 }
 
+int rip_server_create(uint32_t slots, uint32_t workers, uint64_t budget_bytes,
+                      rip_server **out) {
+  return guarded([&] {
+    RIP_REQUIRE(out, RIP_EINVAL, "out is NULL");
+    *out = rip::server_create(slots, workers, budget_bytes);
+  });
+}
+
+int rip_server_add_graph(rip_server *s, rip_graph *g, uint32_t *graph_id) {
+  return guarded([&] {
+    RIP_REQUIRE(s, RIP_EINVAL, "server is NULL");
+    const uint32_t id = rip::server_add_graph(s, g);
+    if (graph_id) *graph_id = id;
+  });
+}
+
+int rip_server_submit(rip_server *s, uint32_t graph_id, uint32_t k, double epsilon,
+                      uint64_t seed, uint64_t *query_id) {
+  return guarded([&] {
+    RIP_REQUIRE(s && query_id, RIP_EINVAL, "server or query_id is NULL");
+    *query_id = rip::server_submit(s, graph_id, k, epsilon, seed);
+  });
+}
+
+int rip_server_poll(rip_server *s, uint64_t query_id, double wait_ms, int *done, uint32_t *seeds,
+                    uint32_t cap, uint32_t *num_seeds, rip_imm_record *rec, double *latency_ms) {
+  return guarded([&] {
+    RIP_REQUIRE(s && done, RIP_EINVAL, "server or done is NULL");
+    *done = rip::server_poll(s, query_id, wait_ms, seeds, cap, num_seeds, rec, latency_ms) ? 1 : 0;
+  });
+}
+
+int rip_server_stats(rip_server *s, uint64_t *out, uint32_t count) {
+  return guarded([&] {
+    RIP_REQUIRE(s && (out || count == 0), RIP_EINVAL, "server or out is NULL");
+    rip::server_stats(s, out, count);
+  });
+}
+
+int rip_server_destroy(rip_server *s) {
+  return guarded([&] {
+    if (s) rip::server_destroy(s);
+  });
+}
+
 int rip_load_edge_list(const char *path, const rip_load_options *opts, rip_host_graph **out) {
   return guarded([&] {
     RIP_REQUIRE(out && opts, RIP_EINVAL, "opts or out is NULL");

@@ -226,6 +226,15 @@ void dev_free(T *&p) {
 void spread_run(rip_graph *g, const uint32_t *seeds, uint32_t nseeds, uint64_t seed,
                 uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels);
 
+// server.cpp
+rip_server *server_create(uint32_t slots, uint32_t workers, uint64_t budget);
+uint32_t server_add_graph(rip_server *s, rip_graph *g);
+uint64_t server_submit(rip_server *s, uint32_t graph, uint32_t k, double eps, uint64_t seed);
+bool server_poll(rip_server *s, uint64_t id, double wait_ms, uint32_t *seeds, uint32_t cap,
+                 uint32_t *num_seeds, rip_imm_record *rec, double *latency_ms);
+void server_stats(rip_server *s, uint64_t *out, uint32_t count);
+void server_destroy(rip_server *s);
+
 // loaders.cpp
 void load_edge_list(const char *path, const rip_load_options &o, rip_host_graph *g);
 void load_binary(const char *path, float scale_factor, rip_host_graph *g);

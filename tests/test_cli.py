@@ -12,6 +12,7 @@ import ripples_amd as R
 from graphs import GOLDEN
 
 IMM = os.path.join(os.path.dirname(R.__file__), "bin", "imm")
+SIM = os.path.join(os.path.dirname(R.__file__), "bin", "simulator")
 KEYS = ["Algorithm", "Input", "Output", "DiffusionModel", "Epsilon", "K", "L", "NumThreads",
         "NumWalkWorkers", "NumCPUTeams", "NumGPUWalkWorkers", "PauseThreshold", "Total",
         "ThetaPrimeDeltas", "ThetaEstimation", "ThetaEstimationGenerateRRR",
@@ -20,14 +21,14 @@ KEYS = ["Algorithm", "Input", "Output", "DiffusionModel", "Epsilon", "K", "L", "
         "GenerateRRRSets", "FindMostInfluentialSet", "Seeds"]
 
 
-def run(*args):
-    return subprocess.run([IMM, *args], capture_output=True, text=True, timeout=300)
+def run(*args, tool=None):
+    return subprocess.run([tool or IMM, *args], capture_output=True, text=True, timeout=300)
 
 
 @pytest.fixture(scope="module", autouse=True)
 def built():
-    if not os.path.exists(IMM):
-        pytest.skip("imm tool not built")
+    if not (os.path.exists(IMM) and os.path.exists(SIM)):
+        pytest.skip("command-line tools not built")
 
 
 @pytest.mark.parametrize("args,msg", [
@@ -84,3 +85,47 @@ def test_rrsets_mode_and_binary_reload(tmp_path):
     assert rec["Theta"] == 5000 and rec["ThetaPrimeDeltas"] == [5000]
     assert rec["Seeds"] == [int(hg.reverse_map[1])] * 3
     assert rec["RRRSetSizeBytes"] > 4 * 5000
+
+
+@pytest.mark.parametrize("args,msg", [
+    ([], "--input-graph is required"),
+    (["-i", "g.tsv", "--replicas", "5"], "--experiment-file is required"),
+    (["-i", "g.tsv", "-e", "x.json"], "--replicas is required"),
+])
+def test_simulator_usage(args, msg):
+    r = run(*args, tool=SIM)
+    assert r.returncode == 2 and msg in r.stderr
+
+
+def test_simulator_bad_record(tmp_path):
+    bad = tmp_path / "bad.json"
+    bad.write_text('[{"DiffusionModel": "XX", "Seeds": [1]}]')
+    r = run("-i", GOLDEN + "/karate.tsv", "-e", str(bad), "--replicas", "3", tool=SIM)
+    assert r.returncode == 1 and "IC or LT" in r.stderr
+    bad.write_text('[{"DiffusionModel": "IC", "Seeds": [1,]')
+    r = run("-i", GOLDEN + "/karate.tsv", "-e", str(bad), "--replicas", "3", tool=SIM)
+    assert r.returncode == 1 and "bad experiment file" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,extra", [("IC", ["-w"]), ("LT", [])])
+def test_imm_then_simulator(tmp_path, model, extra, orc):
+    """imm -> simulator round trip; the simulations equal the oracle's spread
+    of the same seeds (tools/simulator.cc:100-171)."""
+    rec_path, out = str(tmp_path / "imm.json"), str(tmp_path / "sim.json")
+    r = run("-i", GOLDEN + "/karate.tsv", "-k", "3", "-e", "0.5", "-d", model, "-o", rec_path,
+            *extra)
+    assert r.returncode == 0, r.stderr
+    r = run("-i", GOLDEN + "/karate.tsv", "-e", rec_path, "--replicas", "300", "-o", out, *extra,
+            tool=SIM)
+    assert r.returncode == 0, r.stderr
+    imm_rec = json.load(open(rec_path))[0]
+    sim = json.load(open(out))
+    assert len(sim) == 1 and list(sim[0]) == ["Input", "Algorithm", "DiffusionModel", "Epsilon",
+                                               "K", "Seeds", "Simulations"]
+    assert sim[0]["Seeds"] == imm_rec["Seeds"] and sim[0]["DiffusionModel"] == model
+    mdl = R.IC if model == "IC" else R.LT
+    hg = R.load_edge_list(GOLDEN + "/karate.tsv", model=mdl, weighted=bool(extra))
+    internal = [int(np.searchsorted(hg.reverse_map, s)) for s in imm_rec["Seeds"]]
+    act, lev = orc.spread(mdl, hg.index, hg.edges, hg.weights, internal, 0, 0, 300)
+    assert sim[0]["Simulations"] == [[int(a), int(b)] for a, b in zip(act, lev)]
