@@ -82,6 +82,28 @@ typedef struct {
   double round_select_ms[64];    /* ThetaEstimationMostInfluential */
 } rip_imm_record;
 
+/* OPIMCExecutionRecord (include/ripples/opimc_execution_record.h) plus the
+ * per-round bound values.  Round i: rrr_sets_generated[i] sets drawn into
+ * R1 and R2 before it, covered sets of R1 (as OPIMC passes it to UpperBound)
+ * and of R2, the two bounds and their ratio alpha. */
+typedef struct rip_opimc_record {
+  uint64_t theta_zero;           /* ThetaZero */
+  uint64_t theta_max;            /* ThetaMax */
+  uint64_t theta;                /* |R1| = |R2| in the last round */
+  uint64_t rrr_elements;         /* elements of R1 and R2 */
+  uint32_t rounds;
+  uint32_t num_seeds;
+  double total_ms;               /* Total */
+  uint64_t rrr_sets_generated[64];  /* RRRSetsGenerated */
+  double generate_ms[64];        /* GenerateRRRSets */
+  double select_ms[64];          /* FindMostInfluentialSet */
+  uint64_t coverage1[64];
+  uint64_t coverage2[64];
+  float lower[64];
+  float upper[64];
+  float alpha[64];
+} rip_opimc_record;
+
 /* ---------------------------------------------------------------- basics */
 const char *rip_last_error(void);
 int rip_version(void);
@@ -143,6 +165,10 @@ int rip_pool_select(rip_pool *p, uint64_t num_sets, uint32_t k,
 /* Occurrence counts of every vertex over sets [0, num_sets) --
  * CountOccurrencies (include/ripples/counting.h:65-97). counts: uint32[n]. */
 int rip_pool_count(rip_pool *p, uint64_t num_sets, uint32_t *counts);
+/* Sets among [0, num_sets) that contain at least one of seeds[0 ..
+ * num_seeds) -- FindCoverage (include/ripples/opimc.h:100-118). */
+int rip_pool_coverage(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
+                      uint32_t num_seeds, uint64_t *covered);
 /* Device time of the last rip_pool_sample call's sampling kernels, measured
  * with HIP events on the pool's stream, and the number of kernel launches. */
 int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
@@ -174,6 +200,16 @@ int rip_pool_last_stats(const rip_pool *p, uint64_t *stats, uint32_t count);
  * seeds: uint32[k]. */
 int rip_imm(rip_graph *g, uint32_t k, double epsilon, uint64_t seed,
             const rip_tune *tune, uint32_t *seeds, rip_imm_record *rec);
+
+/* OPIM-C -- OPIMC(G, CFG, 1, gen, record, model, omp) (include/ripples/
+ * opimc.h:150-241) with CFG.delta = delta (the reference's default is 0.05).
+ * R1 is sample ids [0, theta) of the seed's sampling stream, R2 ids
+ * [2^31, 2^31 + theta): the pool of seed rip_opimc_r2_seed(seed) draws them.
+ * seeds: uint32[k]; *num_seeds in the record. */
+int rip_opimc(rip_graph *g, uint32_t k, double epsilon, double delta, uint64_t seed,
+              const rip_tune *tune, uint32_t *seeds, rip_opimc_record *rec);
+/* The seed whose pool draws sample ids 2^31, 2^31 + 1, ... of `seed`'s stream. */
+uint64_t rip_opimc_r2_seed(uint64_t seed);
 
 /* ------------------------------------------------ synthetic input (host) */
 /* Directed R-MAT edge list with 2^scale vertices and edge_factor * 2^scale

@@ -505,6 +505,17 @@ static void pool_grow_to(pool *p, int model, uint32_t n, const uint64_t *index,
   orc_sets_free(h);
 }
 
+/* sets [base + p->count, base + target) appended: R2 of OPIM-C lives at base 2^31 */
+static void pool_grow_at(pool *p, int model, uint32_t n, const uint64_t *index,
+                         const uint32_t *src, const float *w, uint64_t seed, uint64_t base,
+                         uint64_t target, int nthreads) {
+  if (target <= p->count) return;
+  orc_sets *h = orc_sample(model, 0, n, index, src, w, seed, base + p->count,
+                           target - p->count, nthreads);
+  pool_append(p, h);
+  orc_sets_free(h);
+}
+
 uint32_t orc_imm(int model, uint32_t n, const uint64_t *index,
                  const uint32_t *src, const float *w, uint32_t k, double eps,
                  uint64_t seed, int nthreads, uint32_t *seeds,
@@ -659,4 +670,125 @@ void orc_spread(int model, uint32_t n, const uint64_t *index, const uint32_t *sr
   free(fidx);
   free(fdst);
   free(fw);
+}
+
+/* ------------------------------------------------------------------------ */
+/* OPIM-C (opimc.h)                                                          */
+/* ------------------------------------------------------------------------ */
+uint64_t orc_coverage(uint32_t n, uint64_t nsets, const uint64_t *offsets,
+                      const uint32_t *vertices, const uint32_t *seeds, uint32_t nseeds) {
+  /* opimc.h:100-118: a set counts once if any of its vertices is a seed */
+  uint8_t *is_seed = (uint8_t *)calloc(n ? n : 1, 1);
+  for (uint32_t i = 0; i < nseeds; ++i) is_seed[seeds[i]] = 1;
+  uint64_t hits = 0;
+  for (uint64_t s = 0; s < nsets; ++s) {
+    for (uint64_t i = offsets[s]; i < offsets[s + 1]; ++i)
+      if (is_seed[vertices[i]]) {
+        ++hits;
+        break;
+      }
+  }
+  free(is_seed);
+  return hits;
+}
+
+/* 1 - 1/e as the reference's float constant */
+#define ORC_OPIMC_A 0.6321203113733684f
+
+/* opimc.h:26-33.  C's log/sqrt are the double functions; the float
+ * variables round on assignment as in the reference. */
+float orc_opimc_theta_helper(uint64_t n, float epsilon, uint64_t k, float delta) {
+  (void)epsilon;
+  float B = ORC_OPIMC_A * sqrt(log(6.0 / delta));
+  float C = sqrt(ORC_OPIMC_A * (orc_log_binomial(n, k) + log(6.0 / delta)));
+  float D = (B + C) * (B + C);
+  return 2 * D;
+}
+
+/* Equation (16), opimc.h:36-38 */
+uint64_t orc_opimc_theta_max(uint64_t n, float epsilon, uint64_t k, float delta) {
+  float top = (float)n * orc_opimc_theta_helper(n, epsilon, k, delta);
+  float bottom = epsilon * epsilon * (float)k;
+  return (uint64_t)ceil(top / bottom);
+}
+
+/* Equation (17), opimc.h:41-43 */
+uint64_t orc_opimc_theta_zero(uint64_t n, float epsilon, uint64_t k, float delta) {
+  return (uint64_t)ceil(orc_opimc_theta_helper(n, epsilon, k, delta));
+}
+
+/* Equation (5), opimc.h:46-54 */
+float orc_opimc_lower_bound(uint64_t coverage, float delta2, uint64_t n, uint64_t theta) {
+  float a = log(1 / delta2);
+  float b = sqrt((float)coverage + 2 * a / 9);
+  float c = sqrt(a / 2);
+  float d = (b - c) * (b - c);
+  float e = a / 18;
+  return ((float)n / (float)theta) * (d - e);
+}
+
+/* Equation (8), opimc.h:57-65 */
+float orc_opimc_upper_bound(uint64_t coverage, float delta1, uint64_t n, uint64_t theta) {
+  float b = (float)coverage / ORC_OPIMC_A;
+  float c = log(1 / delta1) / 2;
+  float d = sqrt(b + c);
+  float e = (d + sqrt(c)) * (d + sqrt(c));
+  return ((float)n / (float)theta) * e;
+}
+
+/* OPIMC (opimc.h:150-241): R1 = sample ids [0, theta) of `seed`, R2 = ids
+ * [2^31, 2^31 + theta); both double each round until alpha >= 1-1/e-eps. */
+uint32_t orc_opimc(int model, uint32_t n, const uint64_t *index, const uint32_t *src,
+                   const float *w, uint32_t k, double eps, double delta_cfg, uint64_t seed,
+                   int nthreads, uint32_t *seeds, orc_opimc_record *rec) {
+  orc_opimc_record r;
+  memset(&r, 0, sizeof(r));
+  float eps_f = (float)eps, delta_f = (float)delta_cfg;
+  uint64_t theta = orc_opimc_theta_zero(n, eps_f, k, delta_f);
+  r.theta_zero = theta;
+  r.theta_max = orc_opimc_theta_max(n, eps_f, k, delta_f);
+  uint64_t imax = (uint64_t)ceil(log2((double)(r.theta_max / theta)));
+  float delta = delta_f / (float)(3 * imax);
+  pool r1, r2;
+  memset(&r1, 0, sizeof(r1));
+  memset(&r2, 0, sizeof(r2));
+  r1.cap_sets = r2.cap_sets = 1;
+  r1.offsets = (uint64_t *)calloc(1, sizeof(uint64_t));
+  r2.offsets = (uint64_t *)calloc(1, sizeof(uint64_t));
+  uint32_t *cur = (uint32_t *)malloc(sizeof(uint32_t) * (k ? k : 1));
+  uint32_t got = 0;
+  for (uint32_t round = 0;; ++round) {
+    uint64_t before = r1.count;
+    pool_grow_at(&r1, model, n, index, src, w, seed, 0, theta, nthreads);
+    pool_grow_at(&r2, model, n, index, src, w, seed, 1ull << 31, theta, nthreads);
+    uint64_t cov = 0;
+    got = orc_select(n, r1.count, r1.offsets, r1.data, k, cur, &cov);
+    double coverage1 = (double)cov / (double)r1.count;
+    uint64_t c1 = (uint64_t)(coverage1 * (double)r1.count);
+    uint64_t c2 = orc_coverage(n, r2.count, r2.offsets, r2.data, cur, got);
+    float upper = orc_opimc_upper_bound(c1, delta, n, r1.count);
+    float lower = orc_opimc_lower_bound(c2, delta, n, r2.count);
+    float alpha = lower / upper;
+    if (round < 64) {
+      r.rrr_sets_generated[round] = 2 * (r1.count - before);
+      r.coverage1[round] = c1;
+      r.coverage2[round] = c2;
+      r.lower[round] = lower;
+      r.upper[round] = upper;
+      r.alpha[round] = alpha;
+    }
+    r.rounds = round + 1;
+    r.theta = theta;
+    if (alpha >= (ORC_OPIMC_A - eps)) break;
+    theta *= 2;
+  }
+  memcpy(seeds, cur, sizeof(uint32_t) * got);
+  r.num_seeds = got;
+  if (rec) *rec = r;
+  free(cur);
+  free(r1.offsets);
+  free(r1.data);
+  free(r2.offsets);
+  free(r2.data);
+  return got;
 }

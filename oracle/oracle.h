@@ -125,6 +125,29 @@ void orc_spread(int model, uint32_t n, const uint64_t *index, const uint32_t *sr
                 uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels,
                 int nthreads);
 
+/* ---- OPIM-C (opimc.h:26-241) ---- */
+/* Sets among [0, nsets) containing any of the seeds (FindCoverage). */
+uint64_t orc_coverage(uint32_t n, uint64_t nsets, const uint64_t *offsets,
+                      const uint32_t *vertices, const uint32_t *seeds, uint32_t nseeds);
+float orc_opimc_theta_helper(uint64_t n, float epsilon, uint64_t k, float delta);
+uint64_t orc_opimc_theta_max(uint64_t n, float epsilon, uint64_t k, float delta);
+uint64_t orc_opimc_theta_zero(uint64_t n, float epsilon, uint64_t k, float delta);
+float orc_opimc_lower_bound(uint64_t coverage, float delta2, uint64_t n, uint64_t theta);
+float orc_opimc_upper_bound(uint64_t coverage, float delta1, uint64_t n, uint64_t theta);
+
+typedef struct {
+  uint64_t theta_zero, theta_max, theta;
+  uint32_t rounds, num_seeds;
+  uint64_t rrr_sets_generated[64];
+  uint64_t coverage1[64], coverage2[64];
+  float lower[64], upper[64], alpha[64];
+} orc_opimc_record;
+
+/* R1 = sample ids [0, theta) of `seed`, R2 = ids [2^31, 2^31 + theta). */
+uint32_t orc_opimc(int model, uint32_t n, const uint64_t *index, const uint32_t *src,
+                   const float *w, uint32_t k, double eps, double delta, uint64_t seed,
+                   int nthreads, uint32_t *seeds, orc_opimc_record *rec);
+
 #ifdef __cplusplus
 }
 #endif

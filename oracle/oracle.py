@@ -35,6 +35,14 @@ class Lcg64(C.Structure):
     _fields_ = [("a", C.c_uint64), ("b", C.c_uint64), ("r", C.c_uint64)]
 
 
+class OpimcRecord(C.Structure):
+    _fields_ = [("theta_zero", C.c_uint64), ("theta_max", C.c_uint64), ("theta", C.c_uint64),
+                ("rounds", C.c_uint32), ("num_seeds", C.c_uint32),
+                ("rrr_sets_generated", C.c_uint64 * 64), ("coverage1", C.c_uint64 * 64),
+                ("coverage2", C.c_uint64 * 64), ("lower", C.c_float * 64),
+                ("upper", C.c_float * 64), ("alpha", C.c_float * 64)]
+
+
 def lib():
     global _LIB
     if _LIB is not None:
@@ -86,6 +94,19 @@ def lib():
     L.orc_imm.restype = C.c_uint32
     L.orc_spread.argtypes = [C.c_int, C.c_uint32, _u64p, _u32p, _f32p, _u32p, C.c_uint32,
                              C.c_uint64, C.c_uint64, C.c_uint64, _u64p, _u32p, C.c_int]
+    L.orc_coverage.argtypes = [C.c_uint32, C.c_uint64, _u64p, _u32p, _u32p, C.c_uint32]
+    L.orc_coverage.restype = C.c_uint64
+    L.orc_opimc_theta_helper.argtypes = [C.c_uint64, C.c_float, C.c_uint64, C.c_float]
+    L.orc_opimc_theta_helper.restype = C.c_float
+    for f in ("orc_opimc_theta_max", "orc_opimc_theta_zero"):
+        getattr(L, f).argtypes = [C.c_uint64, C.c_float, C.c_uint64, C.c_float]
+        getattr(L, f).restype = C.c_uint64
+    for f in ("orc_opimc_lower_bound", "orc_opimc_upper_bound"):
+        getattr(L, f).argtypes = [C.c_uint64, C.c_float, C.c_uint64, C.c_uint64]
+        getattr(L, f).restype = C.c_float
+    L.orc_opimc.argtypes = [C.c_int, C.c_uint32, _u64p, _u32p, _f32p, C.c_uint32, C.c_double,
+                            C.c_double, C.c_uint64, C.c_int, _u32p, C.POINTER(OpimcRecord)]
+    L.orc_opimc.restype = C.c_uint32
     _LIB = L
     return L
 
@@ -159,6 +180,47 @@ def imm(model, index, src, w, k, eps, seed, threads=None):
     seeds = np.zeros(max(k, 1), np.uint32)
     rec = ImmRecord()
     got = lib().orc_imm(model, n, index, src, w, k, eps, seed, threads, seeds, C.byref(rec))
+    return seeds[:got], rec
+
+
+def coverage(n, offsets, vertices, seeds):
+    """Sets of the pool containing any of `seeds` (opimc.h:100-118)."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    vertices = np.ascontiguousarray(vertices, np.uint32)
+    seeds = np.ascontiguousarray(seeds, np.uint32)
+    return int(lib().orc_coverage(n, len(offsets) - 1, offsets,
+                                  vertices if len(vertices) else np.zeros(1, np.uint32),
+                                  seeds if len(seeds) else np.zeros(1, np.uint32), len(seeds)))
+
+
+def opimc_theta_helper(n, eps, k, delta):
+    return lib().orc_opimc_theta_helper(n, eps, k, delta)
+
+
+def opimc_theta_max(n, eps, k, delta):
+    return int(lib().orc_opimc_theta_max(n, eps, k, delta))
+
+
+def opimc_theta_zero(n, eps, k, delta):
+    return int(lib().orc_opimc_theta_zero(n, eps, k, delta))
+
+
+def opimc_lower_bound(cov, delta, n, theta):
+    return lib().orc_opimc_lower_bound(cov, delta, n, theta)
+
+
+def opimc_upper_bound(cov, delta, n, theta):
+    return lib().orc_opimc_upper_bound(cov, delta, n, theta)
+
+
+def opimc(model, index, src, w, k, eps, delta, seed, threads=None):
+    index, src, w = _csr(index, src, w)
+    n = len(index) - 1
+    threads = threads or min(8, os.cpu_count() or 1)
+    seeds = np.zeros(max(k, 1), np.uint32)
+    rec = OpimcRecord()
+    got = lib().orc_opimc(model, n, index, src, w, k, eps, delta, seed, threads, seeds,
+                          C.byref(rec))
     return seeds[:got], rec
 
 

@@ -16,7 +16,7 @@ import numpy as np
 __all__ = [
     "IC", "LT", "Tune", "ImmRecord", "Graph", "Pool", "RipplesError", "lib",
     "device_count", "rmat_edges", "csr_from_edges", "imm",
-    "W_UNIFORM", "W_CASCADE", "W_LT_NORM", "W_CONST", "Server", "SERVER_STATS",
+    "W_UNIFORM", "W_CASCADE", "W_LT_NORM", "W_CONST", "Server", "SERVER_STATS", "opimc", "OpimcRecord",
 ]
 
 IC, LT = 0, 1
@@ -77,6 +77,31 @@ class ImmRecord(C.Structure):
         }
 
 
+class OpimcRecord(C.Structure):
+    """rip_opimc_record (OPIMCExecutionRecord plus per-round bounds)."""
+    _fields_ = [("theta_zero", C.c_uint64), ("theta_max", C.c_uint64), ("theta", C.c_uint64),
+                ("rrr_elements", C.c_uint64), ("rounds", C.c_uint32),
+                ("num_seeds", C.c_uint32), ("total_ms", C.c_double),
+                ("rrr_sets_generated", C.c_uint64 * 64), ("generate_ms", C.c_double * 64),
+                ("select_ms", C.c_double * 64), ("coverage1", C.c_uint64 * 64),
+                ("coverage2", C.c_uint64 * 64), ("lower", C.c_float * 64),
+                ("upper", C.c_float * 64), ("alpha", C.c_float * 64)]
+
+    def as_dict(self):
+        r = min(self.rounds, 64)
+        return {
+            "ThetaZero": self.theta_zero, "ThetaMax": self.theta_max, "Theta": self.theta,
+            "Rounds": self.rounds, "NumSeeds": self.num_seeds, "Total": self.total_ms,
+            "RRRElements": self.rrr_elements,
+            "RRRSetsGenerated": list(self.rrr_sets_generated[:r]),
+            "GenerateRRRSets": list(self.generate_ms[:r]),
+            "FindMostInfluentialSet": list(self.select_ms[:r]),
+            "Coverage1": list(self.coverage1[:r]), "Coverage2": list(self.coverage2[:r]),
+            "Lower": list(self.lower[:r]), "Upper": list(self.upper[:r]),
+            "Alpha": list(self.alpha[:r]),
+        }
+
+
 _LIB = None
 
 
@@ -130,6 +155,11 @@ def lib():
                                         C.POINTER(C.POINTER(C.c_float)),
                                         C.POINTER(C.POINTER(C.c_uint32))]
     L.rip_host_graph_destroy.argtypes = [vp]
+    L.rip_pool_coverage.argtypes = [vp, C.c_uint64, _u32p, C.c_uint32, _u64p]
+    L.rip_opimc.argtypes = [vp, C.c_uint32, C.c_double, C.c_double, C.c_uint64,
+                            C.POINTER(Tune), _u32p, C.POINTER(OpimcRecord)]
+    L.rip_opimc_r2_seed.argtypes = [C.c_uint64]
+    L.rip_opimc_r2_seed.restype = C.c_uint64
     L.rip_server_create.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(vp)]
     L.rip_server_add_graph.argtypes = [vp, vp, C.POINTER(C.c_uint32)]
     L.rip_server_submit.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint64,
@@ -368,6 +398,15 @@ class Pool:
         _check(lib().rip_pool_count(self._h, num_sets, out))
         return out
 
+    def coverage(self, seeds, num_sets=None):
+        """Sets among the first num_sets containing any seed (FindCoverage)."""
+        seeds = np.ascontiguousarray(seeds, np.uint32)
+        ns = self.size()[0] if num_sets is None else num_sets
+        out = np.zeros(1, np.uint64)
+        _check(lib().rip_pool_coverage(self._h, ns, seeds if len(seeds) else
+                                       np.zeros(1, np.uint32), len(seeds), out))
+        return int(out[0])
+
     def last_kernel_ms(self):
         ms, launches = C.c_double(0), C.c_uint32(0)
         _check(lib().rip_pool_last_kernel_ms(self._h, C.byref(ms), C.byref(launches)))
@@ -448,3 +487,17 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def opimc(graph, k, epsilon, delta=0.05, seed=0, tune=None):
+    """OPIM-C (opimc.h:150-241): returns (seeds, record dict)."""
+    seeds = np.zeros(max(k, 1), np.uint32)
+    rec = OpimcRecord()
+    _check(lib().rip_opimc(graph.handle, k, epsilon, delta, seed, _tune(tune), seeds,
+                           C.byref(rec)))
+    return seeds[:rec.num_seeds], rec.as_dict()
+
+
+def opimc_r2_seed(seed):
+    """Seed whose pool draws sample ids 2^31, 2^31 + 1, ... of `seed`'s stream."""
+    return int(lib().rip_opimc_r2_seed(seed))

@@ -176,6 +176,16 @@ int rip_pool_count(rip_pool *p, uint64_t num_sets, uint32_t *counts) {
   });
 }
 
+int rip_pool_coverage(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
+                      uint32_t num_seeds, uint64_t *covered) {
+  return guarded([&] {
+    RIP_REQUIRE(p && covered, RIP_EINVAL, "pool or covered is NULL");
+    RIP_REQUIRE(num_seeds == 0 || seeds, RIP_EINVAL, "seeds is NULL");
+    RIP_REQUIRE(num_sets <= p->num_sets, RIP_EINVAL, "num_sets beyond the pool");
+    *covered = rip::coverage_run(p, num_sets, seeds, num_seeds);
+  });
+}
+
 int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches) {
   return guarded([&] {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
@@ -201,6 +211,16 @@ int rip_imm(rip_graph *g, uint32_t k, double epsilon, uint64_t seed,
     rip::imm_run(g, k, epsilon, seed, tune, seeds, rec);
   });
 }
+
+int rip_opimc(rip_graph *g, uint32_t k, double epsilon, double delta, uint64_t seed,
+              const rip_tune *tune, uint32_t *seeds, rip_opimc_record *rec) {
+  return guarded([&] {
+    RIP_REQUIRE(g && seeds, RIP_EINVAL, "graph or seeds is NULL");
+    rip::opimc_run(g, k, epsilon, delta, seed, tune, seeds, rec);
+  });
+}
+
+uint64_t rip_opimc_r2_seed(uint64_t seed) { return rip::opimc_r2_seed(seed); }
 
 int rip_rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double b,
                    double c, uint64_t seed, uint64_t *num_edges, uint32_t *src,

@@ -181,6 +181,8 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
 uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k,
                     uint32_t *seeds, uint64_t *covered);
 void count_run(rip_pool *p, uint64_t num_sets, uint32_t *counts_host);
+uint64_t coverage_run(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
+                      uint32_t num_seeds);
 
 // scan.hip: exclusive scan of uint32 values into uint64 prefixes (n + 1 out)
 void exclusive_scan_u32(const uint32_t *d_in, unsigned long long *d_out,
@@ -225,6 +227,16 @@ void dev_free(T *&p) {
 // spread.hip
 void spread_run(rip_graph *g, const uint32_t *seeds, uint32_t nseeds, uint64_t seed,
                 uint64_t first, uint64_t count, uint64_t *activated, uint32_t *levels);
+
+// opimc.cpp
+float opimc_theta_helper(uint64_t n, float epsilon, uint64_t k, float delta);
+uint64_t opimc_theta_max(uint64_t n, float epsilon, uint64_t k, float delta);
+uint64_t opimc_theta_zero(uint64_t n, float epsilon, uint64_t k, float delta);
+float opimc_lower_bound(uint64_t coverage, float delta2, uint64_t n, uint64_t theta);
+float opimc_upper_bound(uint64_t coverage, float delta1, uint64_t n, uint64_t theta);
+uint64_t opimc_r2_seed(uint64_t seed);
+void opimc_run(rip_graph *g, uint32_t k, double epsilon, double delta, uint64_t seed,
+               const rip_tune *tune, uint32_t *seeds, rip_opimc_record *rec);
 
 // server.cpp
 rip_server *server_create(uint32_t slots, uint32_t workers, uint64_t budget);

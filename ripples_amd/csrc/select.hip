@@ -166,6 +166,52 @@ kill_kernel(const unsigned long long *key_in, const unsigned long long *inv_star
   }
 }
 
+// Sets [0, num_sets) that contain a flagged vertex (FindCoverage,
+// include/ripples/opimc.h:100-118): each wave walks 64 sets, marks hits in
+// LDS and adds their popcount to *out once per wave.
+__global__ void __launch_bounds__(kBT)
+coverage_kernel(const unsigned long long *set_off, const uint32_t *set_len,
+                const uint32_t *data, uint64_t num_sets, const uint32_t *flags,
+                unsigned long long *out) {
+  __shared__ uint32_t s_pre[kBT / 64][65];
+  __shared__ unsigned long long s_off[kBT / 64][64];
+  __shared__ uint32_t s_hit[kBT / 64][64];
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBT / 64);
+  unsigned long long acc = 0;
+  for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * (kBT / 64) + wid; w * 64 < num_sets;
+       w += nwaves) {
+    const uint64_t s = w * 64 + lane;
+    const uint32_t len = s < num_sets ? set_len[s] : 0u;
+    s_off[wid][lane] = s < num_sets ? set_off[s] : 0ull;
+    s_hit[wid][lane] = 0;
+    const uint32_t incl = wave_inclusive_scan(len);
+    s_pre[wid][lane + 1] = incl;
+    if (lane == 0) s_pre[wid][0] = 0;
+    const uint32_t total = __shfl(incl, 63, 64);
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < total; i += 64) {
+      uint32_t lo = 0, hi = 64;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[wid][mid + 1] <= i) lo = mid + 1;
+        else hi = mid;
+      }
+      const uint32_t v = data[s_off[wid][lo] + (i - s_pre[wid][lo])];
+      if (flags[v]) s_hit[wid][lo] = 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    acc += __popcll(__ballot(s_hit[wid][lane] != 0));
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
+__global__ void flag_seeds_kernel(const uint32_t *seeds, uint32_t k, uint32_t *flags) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x)
+    flags[seeds[i]] = 1;
+}
+
 void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t total,
                            uint32_t k) {
   SelectScratch &s = p->sel;
@@ -269,6 +315,38 @@ uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
   }
   if (covered) *covered = cov;
   return got;
+}
+
+uint64_t coverage_run(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
+                      uint32_t num_seeds) {
+  RIP_REQUIRE(num_sets <= p->num_sets, -1, "coverage: num_sets beyond the pool");
+  const uint32_t n = p->g->n;
+  for (uint32_t i = 0; i < num_seeds; ++i)
+    RIP_REQUIRE(seeds[i] < n, -1, "coverage: seed vertex out of range");
+  hipStream_t st = p->stream;
+  RIP_HIP(hipSetDevice(p->g->device));
+  ensure_select_scratch(p, 0, 0, std::max<uint32_t>(num_seeds, 1));
+  SelectScratch &s = p->sel;
+  // the seeds travel through the key buffer (k >= num_seeds words of 8 bytes)
+  if (num_seeds)
+    RIP_HIP(hipMemcpyAsync(s.d_keys, seeds, sizeof(uint32_t) * num_seeds,
+                           hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, kBT, 2048)), dim3(kBT), 0, st,
+                     s.d_counts, static_cast<uint64_t>(n), 0u);
+  if (num_seeds)
+    hipLaunchKernelGGL(flag_seeds_kernel, dim3(grid_for(num_seeds, kBT, 64)), dim3(kBT), 0, st,
+                       reinterpret_cast<const uint32_t *>(s.d_keys), num_seeds, s.d_counts);
+  unsigned long long *out = p->d_ctl + kCtlWords - 1;
+  hipLaunchKernelGGL(zero_word_kernel, dim3(1), dim3(1), 0, st, out);
+  if (num_sets && num_seeds)
+    hipLaunchKernelGGL(coverage_kernel, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
+                       dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets, s.d_counts,
+                       out);
+  RIP_HIP(hipGetLastError());
+  unsigned long long cov = 0;
+  RIP_HIP(hipMemcpyAsync(&cov, out, sizeof(cov), hipMemcpyDeviceToHost, st));
+  RIP_HIP(hipStreamSynchronize(st));
+  return cov;
 }
 
 void count_run(rip_pool *p, uint64_t num_sets, uint32_t *counts_host) {

@@ -129,3 +129,41 @@ def test_imm_then_simulator(tmp_path, model, extra, orc):
     internal = [int(np.searchsorted(hg.reverse_map, s)) for s in imm_rec["Seeds"]]
     act, lev = orc.spread(mdl, hg.index, hg.edges, hg.weights, internal, 0, 0, 300)
     assert sim[0]["Simulations"] == [[int(a), int(b)] for a, b in zip(act, lev)]
+
+
+OPIMC = os.path.join(os.path.dirname(R.__file__), "bin", "opimc")
+OPIMC_KEYS = ["Algorithm", "Input", "Output", "DiffusionModel", "Epsilon", "K", "Delta",
+              "NumThreads", "NumWalkWorkers", "NumCPUTeams", "NumGPUWalkWorkers",
+              "PauseThreshold", "Total", "CPUBatchSize", "GPUBatchSize", "RRRSetSizeBytes",
+              "GenerateRRRSets", "FindMostInfluentialSet", "Microbenchmarking", "ThetaZero",
+              "ThetaMax", "RRRSetsGenerated", "Seeds"]
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-i", "g.tsv", "-k", "5", "-e", "0.5"], "--diffusion-model is required"),
+    (["-i", "g.tsv", "-k", "5", "-e", "0.5", "-d", "IC", "--rrsets", "9"], "unknown option"),
+])
+def test_opimc_usage_errors(args, msg):
+    r = run(*args, tool=OPIMC)
+    assert r.returncode == 2 and msg in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,extra", [("IC", ["-w"]), ("LT", [])])
+def test_opimc_record_matches_library(tmp_path, model, extra):
+    """tools/opimc.cc:63-97 record; the seeds are rip_opimc's in original ids."""
+    out = str(tmp_path / "o.json")
+    r = run("-i", GOLDEN + "/karate.tsv", "-k", "3", "-e", "0.3", "-d", model, "--delta", "0.1",
+            "-o", out, *extra, tool=OPIMC)
+    assert r.returncode == 0, r.stderr
+    rec = json.load(open(out))[0]
+    assert list(rec) == OPIMC_KEYS and rec["Algorithm"] == "OPIMC"
+    mdl = R.IC if model == "IC" else R.LT
+    hg = R.load_edge_list(GOLDEN + "/karate.tsv", model=mdl, weighted=bool(extra))
+    g = R.Graph(hg.index, hg.edges, hg.weights, model=mdl)
+    seeds, lib_rec = R.opimc(g, 3, 0.3, 0.1, seed=0)
+    assert rec["Seeds"] == hg.reverse_map[seeds].tolist()
+    assert rec["ThetaZero"] == lib_rec["ThetaZero"] and rec["ThetaMax"] == lib_rec["ThetaMax"]
+    assert rec["RRRSetsGenerated"] == lib_rec["RRRSetsGenerated"]
+    assert abs(rec["Delta"] - 0.1) < 1e-7
+    g.close()

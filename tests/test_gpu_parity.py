@@ -274,3 +274,45 @@ def test_spread_unit_weights_and_errors():
     with pytest.raises(R.RipplesError, match="out of range"):
         g.spread([1 << 12], 5)
     g.close()
+
+
+@pytest.mark.parametrize("model", [R.IC, R.LT])
+@pytest.mark.parametrize("k,eps", [(5, 0.3), (10, 0.15)])
+def test_opimc_matches_oracle(model, k, eps, orc):
+    """OPIM-C (opimc.h:150-241): seeds, rounds and every round's coverages and
+    bounds equal the oracle's."""
+    if model == R.IC:
+        index, edges, w = rmat_csr(10, R.W_UNIFORM, 0.1)
+    else:
+        index, edges, w = rmat_csr(10, R.W_LT_NORM, 0.9)
+    g = R.Graph(index, edges, w, model=model)
+    seeds, rec = R.opimc(g, k, eps, 0.05, seed=3)
+    es, erec = orc.opimc(model, index, edges, w, k, eps, 0.05, 3)
+    assert seeds.tolist() == es.tolist()
+    r = erec.rounds
+    assert rec["Rounds"] == r and rec["Theta"] == erec.theta
+    assert (rec["ThetaZero"], rec["ThetaMax"]) == (erec.theta_zero, erec.theta_max)
+    assert rec["Coverage1"] == list(erec.coverage1[:r])
+    assert rec["Coverage2"] == list(erec.coverage2[:r])
+    assert rec["RRRSetsGenerated"] == list(erec.rrr_sets_generated[:r])
+    assert rec["Alpha"] == list(erec.alpha[:r])
+    assert rec["Lower"] == list(erec.lower[:r]) and rec["Upper"] == list(erec.upper[:r])
+    g.close()
+
+
+def test_pool_coverage(orc):
+    index, edges, w = rmat_csr(10, R.W_UNIFORM, 0.1)
+    g = R.Graph(index, edges, w, model=R.IC)
+    p = g.pool(4)
+    p.generate_rrr_sets(5000)
+    off, ver = p.export()
+    n = len(index) - 1
+    rng = np.random.default_rng(0)
+    for seeds in ([], [0], rng.choice(n, 7, replace=False), rng.choice(n, 300, replace=False)):
+        for ns in (5000, 1234, 0):
+            want = orc.coverage(n, off[:ns + 1], ver[:int(off[ns])], seeds)
+            assert p.coverage(seeds, ns) == want
+    with pytest.raises(R.RipplesError, match="out of range"):
+        p.coverage([n], 10)
+    p.close()
+    g.close()
