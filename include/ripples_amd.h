@@ -169,8 +169,10 @@ int rip_pool_count(rip_pool *p, uint64_t num_sets, uint32_t *counts);
  * num_seeds) -- FindCoverage (include/ripples/opimc.h:100-118). */
 int rip_pool_coverage(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
                       uint32_t num_seeds, uint64_t *covered);
-/* Device time of the last rip_pool_sample call's sampling kernels, measured
- * with HIP events on the pool's stream, and the number of kernel launches. */
+/* Device time of the last rip_pool_sample call's main sampling kernel (the
+ * IC wave kernel or the LT walk kernel; the colour fallback, the LT list
+ * sort and the long-walk kernel are not included), measured with HIP events
+ * on the pool's stream, and the number of kernel launches of the call. */
 int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
 /* Work counters of the last rip_pool_sample call's main kernel, written to
  * stats[0 .. count) in this order (no counterpart in the reference; used for
@@ -261,8 +263,10 @@ int rip_spread(rip_graph *g, const uint32_t *seeds, uint32_t num_seeds, uint64_t
  * queries with the same key serve it unchanged and such queries share each
  * sampling launch.  Queries are admitted from the arrival order while fewer
  * than `slots` are active and, when a new pool is needed, while the resident
- * pools stay under `budget_bytes` (0 = no limit; idle pools are evicted least
- * recently used first).  `workers` threads step different pools concurrently,
+ * pools stay under `budget_bytes` (0 = no limit).  A new key re-uses the
+ * buffers of the least recently used idle pool of its graph once the graph
+ * holds `slots` pools or the budget is reached; idle pools of other graphs
+ * are freed, least recently used first, while over budget.  `workers` threads step different pools concurrently,
  * each pool on its own stream.  In a served query's record, rrr_elements
  * counts the shared pool and the timings are the query's own; num_sets is the
  * pool size a solo run would have drawn.  Graphs must outlive the server. */
@@ -284,7 +288,8 @@ enum {
   RIP_SERVER_POOL_BYTES,     /* device bytes of the resident pools */
   RIP_SERVER_POOLS,          /* resident pools */
   RIP_SERVER_REUSED,         /* queries admitted onto an existing pool */
-  RIP_SERVER_EVICTED,        /* pools evicted for the budget */
+  RIP_SERVER_EVICTED,        /* pools freed for the budget */
+  RIP_SERVER_RECYCLED,       /* idle pools re-keyed to a new seed */
   RIP_SERVER_STAT_COUNT
 };
 int rip_server_stats(rip_server *s, uint64_t *out, uint32_t count);

@@ -434,7 +434,7 @@ def imm(graph, k, epsilon, seed=0, tune=None):
 
 
 SERVER_STATS = ["launches", "sets_drawn", "selections", "pool_bytes", "pools", "reused",
-                "evicted"]
+                "evicted", "recycled"]
 
 
 class Server:

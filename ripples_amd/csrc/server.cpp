@@ -14,7 +14,10 @@
 //
 // A query is admitted from the arrival queue while fewer than `slots` queries
 // are active and, if it needs a new pool, while the resident pools fit the HBM
-// budget (idle pools are evicted, least recently used first, to make room).
+// budget.  A new key takes over the least recently used idle pool of its graph
+// (buffers kept, contents dropped) once the graph holds `slots` pools or the
+// budget is reached; idle pools of other graphs are freed, least recently used
+// first, to make room.
 // `workers` threads repeat one scheduling step each on an idle pool that has
 // active queries: extend the pool once to the largest prefix any of them waits
 // for, then run the selection of every query whose prefix is drawn and move it
@@ -71,6 +74,7 @@ struct Entry {
   uint64_t seed = 0;
   rip_pool pool;
   bool created = false;  // pool_create ran (by the first worker that steps it)
+  bool reseed = false;   // re-keyed to `seed`: the worker empties the pool first
   bool busy = false;     // a worker is stepping it
   std::vector<Query *> active;
   uint64_t bytes = 0;    // device bytes, refreshed after every step
@@ -93,7 +97,7 @@ struct rip_server {
   std::deque<rip::Query *> inbox;
   std::map<uint64_t, std::unique_ptr<rip::Query>> queries;
   std::vector<std::thread> workers;
-  uint64_t launches = 0, sets_drawn = 0, selections = 0, reused = 0, evicted = 0;
+  uint64_t launches = 0, sets_drawn = 0, selections = 0, reused = 0, evicted = 0, recycled = 0;
 };
 
 namespace rip {
@@ -127,20 +131,49 @@ void evict_idle(rip_server *s, std::vector<std::unique_ptr<Entry>> *out) {
   }
 }
 
+// A pool for a new key: an idle pool of the same graph is re-keyed (its
+// buffers are kept, so no device allocation or free) once the graph holds
+// `slots` pools or the budget is reached; otherwise a new one is made, after
+// evicting idle pools while over budget.  End of the map when the query has
+// to wait for memory.
+std::map<std::pair<uint32_t, uint64_t>, std::unique_ptr<Entry>>::iterator new_pool(
+    rip_server *s, Query *q, std::vector<std::unique_ptr<Entry>> *evicted) {
+  const auto key = std::make_pair(q->graph, q->seed);
+  auto victim = s->pools.end();
+  uint32_t of_graph = 0;
+  for (auto it = s->pools.begin(); it != s->pools.end(); ++it) {
+    const Entry &e = *it->second;
+    if (e.graph != q->graph) continue;
+    ++of_graph;
+    if (e.busy || !e.active.empty()) continue;
+    if (victim == s->pools.end() || e.last_used < victim->second->last_used) victim = it;
+  }
+  const bool over = s->budget != 0 && resident_bytes(s) >= s->budget;
+  if (victim != s->pools.end() && (of_graph >= s->slots || over)) {
+    auto node = s->pools.extract(victim);
+    node.key() = key;
+    node.mapped()->seed = q->seed;
+    node.mapped()->reseed = true;
+    s->recycled++;
+    return s->pools.insert(std::move(node)).position;
+  }
+  if (over) {
+    evict_idle(s, evicted);
+    if (s->active != 0 && resident_bytes(s) >= s->budget) return s->pools.end();
+  }
+  auto e = std::make_unique<Entry>();
+  e->graph = q->graph;
+  e->seed = q->seed;
+  return s->pools.emplace(key, std::move(e)).first;
+}
+
 void admit(rip_server *s, std::vector<std::unique_ptr<Entry>> *evicted) {
   while (!s->inbox.empty() && s->active < s->slots) {
     Query *q = s->inbox.front();
-    const auto key = std::make_pair(q->graph, q->seed);
-    auto it = s->pools.find(key);
+    auto it = s->pools.find(std::make_pair(q->graph, q->seed));
     if (it == s->pools.end()) {
-      if (s->budget != 0) {
-        evict_idle(s, evicted);
-        if (s->active != 0 && resident_bytes(s) >= s->budget) return;
-      }
-      auto e = std::make_unique<Entry>();
-      e->graph = q->graph;
-      e->seed = q->seed;
-      it = s->pools.emplace(key, std::move(e)).first;
+      it = new_pool(s, q, evicted);
+      if (it == s->pools.end()) return;
     } else {
       s->reused++;
     }
@@ -271,7 +304,12 @@ void worker_loop(rip_server *s) {
       if (!e->created) {
         pool_create(&e->pool, g, e->seed);
         e->created = true;
+      } else if (e->reseed) {
+        e->pool.seed = e->seed;
+        e->pool.num_sets = 0;
+        e->pool.used_data = 0;
       }
+      e->reseed = false;
       // one shared launch extends the pool to the largest prefix waited for
       uint64_t want = 0;
       for (Query *q : batch) want = std::max(want, q->need);
@@ -387,7 +425,7 @@ void server_stats(rip_server *s, uint64_t *out, uint32_t count) {
   std::lock_guard<std::mutex> lk(s->mu);
   const uint64_t v[RIP_SERVER_STAT_COUNT] = {
       s->launches, s->sets_drawn, s->selections, resident_bytes(s),
-      static_cast<uint64_t>(s->pools.size()), s->reused, s->evicted};
+      static_cast<uint64_t>(s->pools.size()), s->reused, s->evicted, s->recycled};
   for (uint32_t i = 0; i < count && i < RIP_SERVER_STAT_COUNT; ++i) out[i] = v[i];
 }
 

@@ -62,7 +62,8 @@ def test_mixed_queries_equal_solo_runs():
     assert st["sets_drawn"] == sum(need.values())
     assert st["pools"] == len(need) and st["reused"] == len(plan) - len(need)
     assert st["selections"] == sum(r[1]["Rounds"] + 1 for r in results)
-    assert st["launches"] < st["selections"] and st["pool_bytes"] > 0 and st["evicted"] == 0
+    assert st["launches"] < st["selections"] and st["pool_bytes"] > 0
+    assert st["evicted"] == 0 and st["recycled"] == 0
     with pytest.raises(R.RipplesError, match="no such query"):
         srv.poll(qids[0], wait_ms=0)
     srv.close()
@@ -82,12 +83,14 @@ def test_budget_and_single_slot_give_same_results():
         qids = [srv.submit(gid, k, eps, seed) for k, eps, seed in plan]
         out.append([srv.poll(q)[0].tolist() for q in qids])
         srv.close()
-    # one query at a time under a 1-byte budget: every new key evicts the idle pool
+    # one query at a time under a 1-byte budget: every new key takes over the
+    # idle pool's buffers
     srv = R.Server(slots=4, workers=2, budget_bytes=1)
     gid = srv.add_graph(g)
     out.append([srv.poll(srv.submit(gid, k, eps, seed))[0].tolist() for k, eps, seed in plan])
     st = srv.stats()
-    assert st["evicted"] == 4 and st["pools"] == 1 and st["reused"] == 0
+    assert st["recycled"] == 4 and st["evicted"] == 0 and st["pools"] == 1
+    assert st["reused"] == 0
     srv.close()
     assert out[0] == out[1] == out[2]
     assert out[0] == [R.imm(g, k, eps, seed=seed)[0].tolist() for k, eps, seed in plan]
@@ -108,5 +111,22 @@ def test_query_errors_are_reported_per_query():
     with pytest.raises(R.RipplesError, match="too small"):
         srv.poll(q, wait_ms=-1, cap=1)
     assert srv.poll(q, wait_ms=0, cap=3) is not None  # still there after the refusal
+    srv.close()
+    g.close()
+
+
+@pytest.mark.gpu
+def test_many_seeds_recycle_pools():
+    """More distinct seeds than slots on one graph: pools are re-keyed, at most
+    `slots` stay resident, and every answer is still the solo one."""
+    g = R.Graph(*rmat_csr(10, R.W_LT_NORM, 0.9, 1), model=R.LT)
+    srv = R.Server(slots=3, workers=3)
+    gid = srv.add_graph(g)
+    plan = [(k, eps, seed) for seed in range(12) for k, eps in [(4, 0.5)] ] + [(7, 0.3, 2)]
+    qids = [srv.submit(gid, k, eps, seed) for k, eps, seed in plan]
+    got = [srv.poll(q)[0].tolist() for q in qids]
+    st = srv.stats()
+    assert st["pools"] <= 3 and st["recycled"] >= 9
+    assert got == [R.imm(g, k, eps, seed=seed)[0].tolist() for k, eps, seed in plan]
     srv.close()
     g.close()
