@@ -17,7 +17,10 @@ outgrow HBM), k in {10,50,100,200}, epsilon in
 server, against the same queries solved back to back with rip_imm: queries/s
 with every query submitted at once, and p50/p99 latency under Poisson arrivals
 (the back-to-back figures are those of a FIFO queue in front of the measured
-solve times).  Every served answer is checked against its solo solve.
+solve times), once with seed = arrival counter (no stream shared) and once
+with seed 0 for every query (the reference imm tool's fixed seeding, so
+queries on a graph share its RRR sets).  Every served answer is checked
+against its solo solve.
 
 The CPU baseline is the oracle's OpenMP restatement of the reference's CPU
 path, timed on a bounded sample of the same workload on this host.
@@ -258,37 +261,29 @@ def pct(x, p):
     return float(np.percentile(np.asarray(x), p))
 
 
-def serve_bench(scales, n_queries, seed, slots, workers, load, budget):
-    """Continuous-batching serving against back-to-back plain solves.
-    Queries: graph uniform over the resident graphs, k in {10,50,100,200},
-    eps in {0.5,0.2,0.13}, seed = arrival counter; Poisson arrivals."""
-    graphs = serve_graphs(scales, seed)
-    rng = np.random.default_rng(seed + 1)
-    queries = [(int(rng.integers(len(graphs))), int(rng.choice(SERVE_K)),
-                float(rng.choice(SERVE_EPS)), i) for i in range(n_queries)]
-    for gi in range(len(graphs)):      # first-touch costs outside the timing
-        R.imm(graphs[gi][1], 10, 0.5, seed=10 ** 6)
+def serve_traffic(graphs, queries, rng, slots, workers, load, budget):
+    """One traffic mix: back-to-back plain solves, everything submitted at
+    once, and Poisson arrivals at `load` x the plain capacity."""
+    n_queries = len(queries)
     service, plain = [], []
+    per_graph = {}
     for gi, k, eps, qseed in queries:
         t = time.time()
         seeds, _ = R.imm(graphs[gi][1], k, eps, seed=qseed)
         service.append(time.time() - t)
         plain.append(seeds)
+        per_graph[graphs[gi][0]] = per_graph.get(graphs[gi][0], 0.0) + service[-1]
     seq_qps = n_queries / sum(service)
-    log("serving: %d plain solves back to back in %.2fs (%.2f queries/s)" % (
-        n_queries, sum(service), seq_qps))
+    log("serving: %d plain solves back to back in %.2fs (%.2f queries/s); seconds per graph %s"
+        % (n_queries, sum(service), seq_qps, {k: round(v, 2) for k, v in per_graph.items()}))
     span, _, st, res = served_run(graphs, queries, None, slots, workers, budget)
     assert all(np.array_equal(a, b) for a, b in zip(res, plain)), "served != solo"
     sat_qps = n_queries / span
     log("serving: all %d submitted at once, done in %.2fs (%.2f queries/s); %s" % (
         n_queries, span, sat_qps, st))
-    out = {"workload": "IMM queries on resident R-MAT graphs %s, k in %s, eps in %s, "
-                       "graph uniform, seed = arrival counter" % (
-                           [name for name, _ in graphs], list(SERVE_K), list(SERVE_EPS)),
-           "queries": n_queries, "slots": slots, "workers": workers, "budget_bytes": budget,
-           "plain_queries_per_s": seq_qps, "served_saturated_queries_per_s": sat_qps,
-           "saturated_speedup": sat_qps / seq_qps, "saturated_counters": st,
-           "results_equal_solo": True, "poisson": []}
+    out = {"plain_queries_per_s": seq_qps, "plain_seconds_per_graph": per_graph,
+           "served_saturated_queries_per_s": sat_qps, "saturated_speedup": sat_qps / seq_qps,
+           "saturated_counters": st, "results_equal_solo": True, "poisson": []}
     for f in load:
         lam = f * seq_qps
         arr = np.cumsum(rng.exponential(1.0 / lam, n_queries)).tolist()
@@ -303,6 +298,29 @@ def serve_bench(scales, n_queries, seed, slots, workers, load, budget):
             % (f, row["served_p50_ms"], row["served_p99_ms"], row["plain_fifo_p50_ms"],
                row["plain_fifo_p99_ms"]))
         out["poisson"].append(row)
+    return out
+
+
+def serve_bench(scales, n_queries, seed, slots, workers, load, budget, modes):
+    """Continuous-batching serving against back-to-back plain solves.
+    Queries: graph uniform over the resident graphs, k in {10,50,100,200},
+    eps in {0.5,0.2,0.13}, Poisson arrivals.  Seed modes: "arrival" gives
+    query i the sampling seed i (no two queries share a stream); "shared"
+    gives every query seed 0, the fixed seeding of the reference's imm tool,
+    so queries on one graph reuse each other's RRR sets."""
+    graphs = serve_graphs(scales, seed)
+    rng = np.random.default_rng(seed + 1)
+    mix = [(int(rng.integers(len(graphs))), int(rng.choice(SERVE_K)),
+            float(rng.choice(SERVE_EPS))) for _ in range(n_queries)]
+    for gi in range(len(graphs)):      # first-touch costs outside the timing
+        R.imm(graphs[gi][1], 10, 0.5, seed=10 ** 6)
+    out = {"graphs": [name for name, _ in graphs], "k": list(SERVE_K), "eps": list(SERVE_EPS),
+           "queries": n_queries, "slots": slots, "workers": workers, "budget_bytes": budget}
+    for mode in modes:
+        log("serving traffic with %s seeds" % mode)
+        queries = [(gi, k, eps, i if mode == "arrival" else 0) for i, (gi, k, eps) in
+                   enumerate(mix)]
+        out[mode + "_seeds"] = serve_traffic(graphs, queries, rng, slots, workers, load, budget)
     for _, g in graphs:
         g.close()
     return out
@@ -348,6 +366,9 @@ def main():
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-budget-gib", type=float, default=64.0,
                     help="HBM budget of the resident pools")
+    ap.add_argument("--serve-seeds", default="arrival,shared",
+                    help="seed modes of the serving traffic: arrival (query i has seed i), "
+                         "shared (every query seed 0)")
     ap.add_argument("--serve-load", default="0.8,2.0",
                     help="Poisson rates as multiples of the plain back-to-back capacity")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -403,7 +424,8 @@ def main():
         out["serving"] = serve_bench([int(x) for x in a.serve_scales.split(",")],
                                      a.serve_queries, a.seed, a.serve_slots, a.serve_workers,
                                      [float(x) for x in a.serve_load.split(",")],
-                                     int(a.serve_budget_gib * (1 << 30)))
+                                     int(a.serve_budget_gib * (1 << 30)),
+                                     a.serve_seeds.split(","))
     print(json.dumps(out), flush=True)
     return 0
 

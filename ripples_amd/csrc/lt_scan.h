@@ -74,16 +74,20 @@ __host__ __device__ inline uint32_t lt_bp_next(uint32_t lo, uint32_t hi, uint32_
 
 // B[j] for the row whose weights start at w.  The bracket (lo, hi] holds
 // B[j]: the chain fails from lo (k = 0 stands for "no start") and survives
-// from hi (kLtNever stands for "never").  Candidates are evaluated in pairs
-// (k, k - 1), so an exact estimate closes the bracket in one round; every
-// round shrinks it, and after four rounds of estimates it is bisected.
+// from hi (kLtNever stands for "never").  The first two rounds evaluate one
+// candidate each (the top k = 2^24, then the estimate from its chain value,
+// which is typically tens of units off on long rows); later candidates are
+// evaluated in pairs (k, k - 1), so an exact estimate closes the bracket in
+// one round.  Every round shrinks the bracket, and after four rounds of
+// estimates it is bisected.
 __host__ __device__ inline uint32_t lt_breakpoint(const uint32_t *w, uint32_t j) {
+  constexpr int kPairFrom = 2;
   uint32_t lo = 0, hi = kLtNever;
   uint32_t k = 1u << 24;
   for (int round = 0;; ++round) {
     double s1 = 0.0, s2 = 0.0;
     const bool a = lt_chain_eval(w, j, k, s1);
-    const bool b = a && k - 1 > lo && lt_chain_eval(w, j, k - 1, s2);
+    const bool b = a && round >= kPairFrom && k - 1 > lo && lt_chain_eval(w, j, k - 1, s2);
     uint32_t at;
     bool ok;
     double slack;
@@ -91,7 +95,7 @@ __host__ __device__ inline uint32_t lt_breakpoint(const uint32_t *w, uint32_t j)
       hi = k - 1;
       at = k - 1, ok = true, slack = s2;
     } else if (a) {
-      lo = k - 1;
+      if (round >= kPairFrom) lo = k - 1;  // k - 1 was evaluated and failed
       hi = k;
       at = k, ok = true, slack = s1;
     } else {
