@@ -110,7 +110,7 @@ def measured_traffic(kernel, workload):
     return None if entry is None else entry.get("hbm_bytes_per_launch")
 
 
-def measure(name, seed, steps, warmup, batch, tune, skip_imm):
+def measure(name, seed, steps, warmup, batch, tune, skip_imm, fused_steps=0):
     wl = WORKLOADS[name]
     index, edges, w = build_graph(wl, seed)
     g = R.Graph(index, edges, w, model=wl["model"])
@@ -189,6 +189,23 @@ def measure(name, seed, steps, warmup, batch, tune, skip_imm):
         "roofline": roof,
         "graph": (wl, index, edges, w, batch),
     }
+    if wl["model"] == R.IC and fused_steps:
+        # the 64-colour fused sampler on the same batch, for the fused-versus-
+        # one-sample-per-wave comparison (the sets are the same)
+        fz = {"sampler": 2}
+        pool.reset()
+        pool.generate_rrr_sets(batch, tune=fz)
+        t0 = time.time()
+        for _ in range(fused_steps):
+            pool.reset()
+            pool.generate_rrr_sets(batch, tune=fz)
+        fwall = time.time() - t0
+        res["fused_colour_sampler"] = {
+            "value": batch * fused_steps / fwall, "ms_per_step": fwall / fused_steps * 1e3,
+            "kernel_ms": pool.last_kernel_ms()[0],
+            "wave_over_fused": (batch * steps / wall) / (batch * fused_steps / fwall)}
+        log("%s fused colour sampler: %.3f s per step (%.2fx slower than one sample per wave)"
+            % (name, fwall / fused_steps, res["fused_colour_sampler"]["wave_over_fused"]))
     pool.close()
     g.close()
     return res
@@ -416,10 +433,12 @@ def main():
                                                           out["cpu_baseline"]["cores"]))
     del index, edges, w
     if a.secondary and a.secondary != a.workload:
-        sec = measure(a.secondary, a.seed, max(1, a.steps // 3), 1, 0, None, False)
+        sec = measure(a.secondary, a.seed, max(1, a.steps // 3), 1, 0, None, False,
+                      fused_steps=2)
         sec.pop("graph")
         out["secondary"] = {k: sec[k] for k in ("value", "ms_per_step", "config", "imm_wall_s",
-                                                "imm", "pool", "kernel_counters", "roofline")}
+                                                "imm", "pool", "kernel_counters", "roofline",
+                                                "fused_colour_sampler") if k in sec}
     if a.serve_scales:
         out["serving"] = serve_bench([int(x) for x in a.serve_scales.split(",")],
                                      a.serve_queries, a.seed, a.serve_slots, a.serve_workers,

@@ -64,6 +64,20 @@ struct LtScratch {
 };
 
 
+// Device-resident arguments of the selector's seed rounds (select.hip).
+struct Args {
+  uint32_t *counts;
+  const unsigned long long *inv_start;
+  const uint32_t *inv;
+  const unsigned long long *set_off;
+  const uint32_t *set_len;
+  const uint32_t *data;
+  uint8_t *covered;
+  unsigned long long *keys;
+  unsigned long long *ctl;
+  uint32_t n;
+};
+
 struct SelectScratch {
   uint32_t n = 0;
   uint64_t cap_sets = 0, cap_elems = 0;
@@ -74,6 +88,12 @@ struct SelectScratch {
   uint8_t *d_covered = nullptr;        // cap_sets
   unsigned long long *d_keys = nullptr;  // per-seed argmax keys (k)
   uint32_t cap_k = 0;
+  // The k rounds of argmax + kill as one HIP graph, captured for graph_k
+  // seeds; its kernels read the buffer pointers from d_args, so the graph
+  // stays valid when the buffers above are reallocated.
+  struct Args *d_args = nullptr;
+  hipGraphExec_t graph = nullptr;
+  uint32_t graph_k = 0;
 };
 
 }  // namespace rip

@@ -173,6 +173,8 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->sel.d_inv);
   dev_free(p->sel.d_covered);
   dev_free(p->sel.d_keys);
+  dev_free(p->sel.d_args);
+  if (p->sel.graph) (void)hipGraphExecDestroy(p->sel.graph);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->stream) (void)hipStreamDestroy(p->stream);

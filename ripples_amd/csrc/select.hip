@@ -11,8 +11,9 @@
 //      covered and decrements the counts of their members.
 // Every set is killed at most once, so the kill passes cost O(T) over all k
 // seeds, where the reference recounts every uncovered set per seed
-// (gpu/find_most_influential.h:98-120).  The seed loop issues 2k kernels
-// back to back with no host synchronisation; the host reads the k keys once.
+// (gpu/find_most_influential.h:98-120).  The 2k kernels of the seed loop run
+// as one HIP graph with no host synchronisation; the host reads the k keys
+// once.
 #include <algorithm>
 #include <vector>
 
@@ -109,10 +110,9 @@ __global__ void copy_u64_kernel(unsigned long long *dst,
 
 __global__ void zero_word_kernel(unsigned long long *w) { *w = 0; }
 
-__global__ void clear_keys_kernel(unsigned long long *keys, uint32_t k,
-                                  unsigned long long *ctl) {
-  for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) keys[i] = 0;
-  if (threadIdx.x == 0) ctl[kCtlCovered] = 0;
+__global__ void clear_keys_kernel(const Args *a, uint32_t k) {
+  for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) a->keys[i] = 0;
+  if (threadIdx.x == 0) a->ctl[kCtlCovered] = 0;
 }
 
 __device__ inline unsigned long long wave_max_u64(unsigned long long x) {
@@ -123,9 +123,11 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long x) {
   return x;
 }
 
-__global__ void __launch_bounds__(kBT)
-argmax_kernel(const uint32_t *counts, uint32_t n, unsigned long long *key_out) {
+// Seed round i, first half: keys[i] = max over v of counts[v] << 32 | ~v.
+__global__ void __launch_bounds__(kBT) argmax_kernel(const Args *a, uint32_t i) {
   __shared__ unsigned long long s_w[kBT / 64];
+  const uint32_t *counts = a->counts;
+  const uint32_t n = a->n;
   unsigned long long best = 0;
   for (uint32_t v = blockIdx.x * kBT + threadIdx.x; v < n; v += gridDim.x * kBT) {
     const unsigned long long key =
@@ -137,22 +139,27 @@ argmax_kernel(const uint32_t *counts, uint32_t n, unsigned long long *key_out) {
   __syncthreads();
   if (threadIdx.x == 0) {
     for (uint32_t w = 1; w < kBT / 64; ++w) best = s_w[w] > best ? s_w[w] : best;
-    if (best >> 32) atomicMax(key_out, best);
+    if (best >> 32) atomicMax(a->keys + i, best);
   }
 }
 
-__global__ void __launch_bounds__(kBT)
-kill_kernel(const unsigned long long *key_in, const unsigned long long *inv_start,
-            const uint32_t *inv, const unsigned long long *set_off,
-            const uint32_t *set_len, const uint32_t *data, uint8_t *covered,
-            uint32_t *counts, unsigned long long *ctl) {
-  const unsigned long long key = *key_in;
+// Seed round i, second half: every uncovered set containing the seed is
+// marked covered once and its members' counts drop by one.
+__global__ void __launch_bounds__(kBT) kill_kernel(const Args *a, uint32_t i) {
+  const unsigned long long key = a->keys[i];
   const uint32_t gain = static_cast<uint32_t>(key >> 32);
   if (gain == 0) return;
   const uint32_t seed = 0xFFFFFFFFu - static_cast<uint32_t>(key);
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl[kCtlCovered], gain);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a->ctl[kCtlCovered], gain);
   const uint32_t lane = lane_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBT / 64);
+  const unsigned long long *inv_start = a->inv_start;
+  const uint32_t *inv = a->inv;
+  const unsigned long long *set_off = a->set_off;
+  const uint32_t *set_len = a->set_len;
+  const uint32_t *data = a->data;
+  uint8_t *covered = a->covered;
+  uint32_t *counts = a->counts;
   const unsigned long long b = inv_start[seed], e = inv_start[seed + 1];
   for (unsigned long long p = b + blockIdx.x * (kBT / 64) + (threadIdx.x >> 6); p < e;
        p += nwaves) {
@@ -162,7 +169,7 @@ kill_kernel(const unsigned long long *key_in, const unsigned long long *inv_star
     if (lane == 0) covered[s] = 1;
     const unsigned long long off = set_off[s];
     const uint32_t len = set_len[s];
-    for (uint32_t i = lane; i < len; i += 64) atomicSub(&counts[data[off + i]], 1u);
+    for (uint32_t q = lane; q < len; q += 64) atomicSub(&counts[data[off + q]], 1u);
   }
 }
 
@@ -240,6 +247,7 @@ void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t total,
     s.cap_k = k;
     s.d_keys = dev_alloc<unsigned long long>(k);
   }
+  if (!s.d_args) s.d_args = dev_alloc<Args>(1);
 }
 
 uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap) {
@@ -273,6 +281,37 @@ uint64_t build_counts(rip_pool *p, uint64_t num_sets, uint32_t k) {
   return total;
 }
 
+// Captures clear + k x (argmax, kill) on the pool's stream as an executable
+// graph.  Only kernels are recorded (no memset nodes); the capture mode is
+// thread-local so other host threads' streams are unaffected.
+void capture_rounds(rip_pool *p, uint32_t k) {
+  SelectScratch &s = p->sel;
+  if (s.graph) {
+    (void)hipGraphExecDestroy(s.graph);
+    s.graph = nullptr;
+    s.graph_k = 0;
+  }
+  hipStream_t st = p->stream;
+  const uint32_t argmax_grid = grid_for(p->g->n, kBT * 8, 1024);
+  RIP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(clear_keys_kernel, dim3(1), dim3(256), 0, st, s.d_args, k);
+  for (uint32_t i = 0; i < k; ++i) {
+    hipLaunchKernelGGL(argmax_kernel, dim3(argmax_grid), dim3(kBT), 0, st, s.d_args, i);
+    hipLaunchKernelGGL(kill_kernel, dim3(256), dim3(kBT), 0, st, s.d_args, i);
+  }
+  const hipError_t launch = hipGetLastError();
+  hipGraph_t graph = nullptr;
+  const hipError_t end = hipStreamEndCapture(st, &graph);
+  RIP_HIP(launch);
+  RIP_HIP(end);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  RIP_HIP(inst);
+  s.graph = exec;
+  s.graph_k = k;
+}
+
 }  // namespace
 
 uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
@@ -290,14 +329,16 @@ uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
                        p->d_off, p->d_len, p->d_data, num_sets, s.d_cursor, s.d_inv);
   hipLaunchKernelGGL(fill_u8_kernel, dim3(grid_for(num_sets, kBT, 2048)), dim3(kBT), 0, st,
                      s.d_covered, num_sets);
-  hipLaunchKernelGGL(clear_keys_kernel, dim3(1), dim3(256), 0, st, s.d_keys, k, p->d_ctl);
-  const uint32_t argmax_grid = grid_for(n, kBT * 8, 1024);
-  for (uint32_t i = 0; i < k; ++i) {
-    hipLaunchKernelGGL(argmax_kernel, dim3(argmax_grid), dim3(kBT), 0, st, s.d_counts, n,
-                       s.d_keys + i);
-    hipLaunchKernelGGL(kill_kernel, dim3(256), dim3(kBT), 0, st, s.d_keys + i,
-                       s.d_inv_start, s.d_inv, p->d_off, p->d_len, p->d_data,
-                       s.d_covered, s.d_counts, p->d_ctl);
+  // the seed rounds run as one graph; this call's buffers travel in d_args
+  // (`args` stays alive until the stream is synchronised below)
+  const Args args{s.d_counts, s.d_inv_start, s.d_inv, p->d_off, p->d_len, p->d_data,
+                  s.d_covered, s.d_keys, p->d_ctl, n};
+  RIP_HIP(hipMemcpyAsync(s.d_args, &args, sizeof(Args), hipMemcpyHostToDevice, st));
+  if (k) {
+    if (!s.graph || s.graph_k != k) capture_rounds(p, k);
+    RIP_HIP(hipGraphLaunch(s.graph, st));
+  } else {
+    hipLaunchKernelGGL(clear_keys_kernel, dim3(1), dim3(256), 0, st, s.d_args, 0u);
   }
   RIP_HIP(hipGetLastError());
   std::vector<unsigned long long> keys(std::max<uint32_t>(k, 1));

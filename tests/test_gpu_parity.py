@@ -70,6 +70,25 @@ def test_selection_matches_oracle(case, orc):
     p.close()
 
 
+def test_selector_graph_replays_across_pool_growth(orc):
+    """The seed rounds are one captured graph per (pool, k): replayed after the
+    pool's buffers grew and moved, and on different prefixes, each replay
+    equals the oracle."""
+    index, edges, w = rmat_csr(10, R.W_UNIFORM, 0.1, 2)
+    n = len(index) - 1
+    g = R.Graph(index, edges, w, model=R.IC)
+    p = g.pool(2)
+    for total in (500, 3000, 20000, 60000):
+        p.generate_rrr_sets(total - p.size()[0])
+        off, ver = p.export()
+        for num in (total, total // 3, 1):
+            seeds, cov = p.find_most_influential_set(8, num)
+            es, ec = orc.select(n, off[:num + 1], ver[:int(off[num])], 8)
+            assert seeds.tolist() == es.tolist() and cov == ec, (total, num)
+    p.close()
+    g.close()
+
+
 @pytest.mark.parametrize("tune", [dict(colors=1), dict(colors=17, block=256),
                                   dict(colors=64, block=512, edge_run=4),
                                   dict(colors=32, block=1024, edge_run=8, slots=3),
