@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
         uint32_t bits = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
+          if (j0 + 64 * u >= deg) break;  // past the row end (last chunk only)
           bits |= static_cast<uint32_t>(x >> 40) < thr[u] ? (1u << u) : 0u;
           x = j64.a * x + j64.b;
         }
