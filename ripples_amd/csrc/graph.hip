@@ -217,6 +217,9 @@ void graph_destroy(rip_graph *g) {
   dev_free(g->wave.d_list);
   dev_free(g->wave.d_epoch);
   dev_free(g->wave.d_big);
+  dev_free(g->wave.d_bbits);
+  dev_free(g->wave.d_bsum);
+  dev_free(g->wave.d_blist);
   dev_free(g->lt.d_long_ids);
   dev_free(g->lt.d_spill);
   dev_free(g->d_bp);

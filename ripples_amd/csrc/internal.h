@@ -43,6 +43,13 @@ struct WaveScratch {
   uint32_t *d_epoch = nullptr;            // waves
   uint64_t cap_big = 0;
   uint32_t *d_big = nullptr;              // sample indices left over
+  // large-set kernel: per resident wave a bitmap of n bits, its summary and
+  // a list of up to n vertices
+  uint64_t big_waves = 0;
+  uint32_t big_n = 0;
+  uint32_t *d_bbits = nullptr;
+  uint32_t *d_bsum = nullptr;
+  uint32_t *d_blist = nullptr;
 };
 
 // Forward CSR and per-workgroup scratch of the spread simulator, built on
@@ -155,6 +162,7 @@ enum : int {
   kCtlStatExpansions = 8,
   kCtlStatLevels = 9,
   kCtlSortCount = 10,    // LT: finished lists longer than the LDS part, to sort
+  kCtlBigCounter = 11,   // IC: work counter of the large-set kernel
   kCtlWords = 16,
 };
 
@@ -192,6 +200,7 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
 // sample_ic_wave.hip: returns how many samples were left for the colour
 // sampler; their indices (relative to first) are in g->wave.d_big
 uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count);
+void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count);
 
 // sample_lt.hip
 uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,

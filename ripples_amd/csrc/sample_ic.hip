@@ -560,8 +560,8 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
   if (cfg.sampler == RIP_SAMPLER_COLOUR) return ic_launch(p, first, count, cfg);
   const uint64_t big = ic_wave_launch(p, first, count);
   if (big == 0) return 1;
-  // sets that outgrew a wave: colour groups over the listed sample indices
-  ic_launch(p, first, big, cfg, p->g->wave.d_big, false);
+  // sets that outgrew a wave's list: the large-set kernel
+  ic_big_launch(p, first, big);
   return 3;
 }
 

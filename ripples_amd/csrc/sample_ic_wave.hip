@@ -108,6 +108,76 @@ __device__ inline void wave_bitonic(Ptr a, uint32_t len, uint32_t cap2) {
   }
 }
 
+// Expands row v of the sample whose stream starts at z: lane l evaluates
+// the coins of in-edges begin + l + 64 i (the row's first coin is P^(1+begin)
+// applied to z; jl = P^l, j64 = P^64), and the in-neighbours of live edges for
+// which visit(src) reports a first visit are appended to list[tail ...].
+// Returns false, leaving the list unfinished, when it would pass cap.
+template <int U, typename Visit>
+__device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
+                                  const uint32_t *val, uint32_t v, uint64_t z,
+                                  const Affine &jl, const Affine &j64, uint32_t *list,
+                                  uint32_t &tail, uint32_t cap,
+                                  unsigned long long &st_edges, Visit &&visit) {
+  const uint32_t lane = lane_id();
+  // the row is the same for every lane: keep it in scalar registers
+  const VertexRec rv = vrec[v];
+  const uint32_t rbegin = uniform_u32(rv.begin);
+  const uint32_t deg = uniform_u32(rv.end) - rbegin;
+  const uint32_t uthr = uniform_u32(rv.uthr);
+  const bool uniform = (uniform_u32(rv.flags) & kRowUniform) != 0;
+  st_edges += deg;
+  if (uniform && uthr == 0) return true;           // weight 0: no edge of the row is live
+  const uint64_t x0 = uniform_u64(rv.ja) * z + uniform_u64(rv.jb);  // row's first coin
+  uint64_t x = jl.a * x0 + jl.b;                    // coin of edge begin + lane
+  for (uint32_t j0 = 0; j0 < deg; j0 += 64 * U) {
+    // thresholds: from the vertex record for uniform rows, else one
+    // coalesced 4-byte load per edge; thr 0 past the row end (only the last
+    // chunk of a row needs the bounds test)
+    const uint32_t e0 = rbegin + j0 + lane;
+    uint32_t thr[U];
+    if (j0 + 64 * U <= deg) {
+      if (uniform) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) thr[u] = uthr;
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) thr[u] = val[e0 + 64 * u];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t j = j0 + lane + 64 * u;
+        thr[u] = j < deg ? (uniform ? uthr : val[e0 + 64 * u]) : 0u;
+      }
+    }
+    // all U coins first; one ballot tells whether any edge of the chunk is
+    // live (rare: about one edge per row under weighted cascade)
+    uint32_t bits = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (j0 + 64 * u >= deg) break;  // past the row end (last chunk only)
+      bits |= static_cast<uint32_t>(x >> 40) < thr[u] ? (1u << u) : 0u;
+      x = j64.a * x + j64.b;
+    }
+    if (__ballot(bits != 0) == 0) continue;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool live = ((bits >> u) & 1u) != 0;
+      const uint64_t lm = __ballot(live);
+      if (lm == 0) continue;
+      if (tail + __popcll(lm) > cap) return false;
+      // the in-neighbour id is read only for live edges
+      const uint32_t src = live ? ids[e0 + 64 * u] : 0u;
+      const bool fresh = live && visit(src);
+      const uint64_t fm = __ballot(fresh);
+      if (fresh) list[tail + __popcll(fm & lanes_below())] = src;
+      tail += __popcll(fm);
+    }
+  }
+  return true;
+}
+
 template <int BT, int U>
 __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
   constexpr int NW = BT / 64;
@@ -141,67 +211,11 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
     bool big = false;
     while (head < tail && !big) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      // the row is the same for every lane: keep it in scalar registers
       const uint32_t v = uniform_u32(list[head++]);
-      const VertexRec rv = P.vrec[v];
-      const uint32_t rbegin = uniform_u32(rv.begin);
-      const uint32_t deg = uniform_u32(rv.end) - rbegin;
-      const uint32_t uthr = uniform_u32(rv.uthr);
-      const bool uniform = (uniform_u32(rv.flags) & kRowUniform) != 0;
       ++st_exp;
-      st_edges += deg;
-      if (uniform && uthr == 0) continue;            // weight 0: no edge of the row is live
-      const uint64_t x0 = uniform_u64(rv.ja) * z + uniform_u64(rv.jb);  // row's first coin
-      uint64_t x = jl.a * x0 + jl.b;                  // coin of edge begin + lane
-      for (uint32_t j0 = 0; j0 < deg; j0 += 64 * U) {
-        // thresholds: from the vertex record for uniform rows, else one
-        // coalesced 4-byte load per edge; thr 0 past the row end (only the
-        // last chunk of a row needs the bounds test)
-        const uint32_t e0 = rbegin + j0 + lane;
-        uint32_t thr[U];
-        if (j0 + 64 * U <= deg) {
-          if (uniform) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) thr[u] = uthr;
-          } else {
-#pragma unroll
-            for (int u = 0; u < U; ++u) thr[u] = P.val[e0 + 64 * u];
-          }
-        } else {
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const uint32_t j = j0 + lane + 64 * u;
-            thr[u] = j < deg ? (uniform ? uthr : P.val[e0 + 64 * u]) : 0u;
-          }
-        }
-        // all U coins first; one ballot tells whether any edge of the chunk
-        // is live (rare: about one edge per row under weighted cascade)
-        uint32_t bits = 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (j0 + 64 * u >= deg) break;  // past the row end (last chunk only)
-          bits |= static_cast<uint32_t>(x >> 40) < thr[u] ? (1u << u) : 0u;
-          x = j64.a * x + j64.b;
-        }
-        if (__ballot(bits != 0) == 0) continue;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool live = ((bits >> u) & 1u) != 0;
-          const uint64_t lm = __ballot(live);
-          if (lm == 0) continue;
-          if (tail + __popcll(lm) > P.cap) {
-            big = true;
-            break;
-          }
-          // the in-neighbour id is read only for live edges
-          const uint32_t src = live ? P.ids[e0 + 64 * u] : 0u;
-          const bool fresh = live && table_insert(table, tmask, tag, src);
-          const uint64_t fm = __ballot(fresh);
-          if (fresh) list[tail + __popcll(fm & lanes_below())] = src;
-          tail += __popcll(fm);
-        }
-        if (big) break;
-      }
+      if (!expand_row<U>(P.vrec, P.ids, P.val, v, z, jl, j64, list, tail, P.cap, st_edges,
+                         [&](uint32_t src) { return table_insert(table, tmask, tag, src); }))
+        big = true;
     }
     if (big) {
       if (lane == 0) {
@@ -254,6 +268,139 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
     atomicAdd(&P.ctl[kCtlStatExpansions], st_exp / 64);
   }
 }
+
+// ------------------------------------------------------------ large sets
+// Samples whose sets outgrow the wave list are finished here, again one
+// reverse BFS per wave, with a visited bitmap of n bits instead of the hash
+// table and a list of up to n entries.  A summary bitmap (one bit per bitmap
+// word) is set with the first bit of each word; the finished set is written
+// in vertex order by walking the summary, then the marked words, and both
+// bitmaps are cleared on the way, so they are all zero between samples.
+struct BigParams {
+  const VertexRec *vrec;
+  const uint32_t *ids;
+  const uint32_t *val;
+  uint32_t n;
+  Affine step;
+  uint64_t seed;
+  uint64_t first;
+  const uint32_t *big_ids;  // local sample indices
+  uint64_t count;
+  uint32_t *bits;           // waves x words
+  uint32_t *sum;            // waves x swords
+  uint32_t *list;           // waves x n
+  uint32_t words, swords;
+  unsigned long long *set_off;
+  uint32_t *set_len;
+  uint32_t *data;
+  uint64_t cap_data;
+  unsigned long long *ctl;
+};
+
+template <int BT, int U>
+__global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
+  constexpr int NW = BT / 64;
+  const uint32_t lane = lane_id();
+  const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * NW + (threadIdx.x >> 6);
+  uint32_t *bits = P.bits + wave * P.words;
+  uint32_t *sum = P.sum + wave * P.swords;
+  uint32_t *list = P.list + wave * P.n;
+  const Affine jl = aff_pow(P.step, lane);
+  const Affine j64 = aff_pow(P.step, 64);
+  unsigned long long st_edges = 0, st_exp = 0;
+  auto visit = [&](uint32_t u) {
+    const uint32_t w = u >> 5, bit = 1u << (u & 31);
+    const uint32_t old = atomicOr(&bits[w], bit);
+    if (old == 0) atomicOr(&sum[w >> 5], 1u << (w & 31));
+    return (old & bit) == 0;
+  };
+  for (;;) {
+    unsigned long long idx = 0;
+    if (lane == 0) idx = atomicAdd(&P.ctl[kCtlBigCounter], 1ull);
+    idx = uniform_u64(__shfl(idx, 0, 64));
+    if (idx >= P.count) break;
+    const uint64_t s = P.first + P.big_ids[idx];
+    const uint64_t z = sample_z(P.seed, s);
+    const uint32_t root = uniform_index(z, P.n);
+    if (lane == 0) {
+      visit(root);
+      list[0] = root;
+    }
+    uint32_t head = 0, tail = 1;
+    while (head < tail) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      const uint32_t v = uniform_u32(list[head++]);
+      ++st_exp;
+      // every vertex enters the list once, so it never passes n entries
+      expand_row<U>(P.vrec, P.ids, P.val, v, z, jl, j64, list, tail, P.n, st_edges, visit);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    const uint32_t len = tail;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&P.ctl[kCtlDataBump], static_cast<unsigned long long>(len));
+    base = uniform_u64(__shfl(base, 0, 64));
+    const bool fits = base + len <= P.cap_data;
+    // 1. indices of the marked words, in order, into the (finished) list
+    uint32_t nw = 0;
+    for (uint32_t s0 = 0; s0 < P.swords; s0 += 64) {
+      const uint32_t sw = s0 + lane;
+      uint32_t m = sw < P.swords ? sum[sw] : 0u;
+      const uint32_t c = __popc(m);
+      const uint32_t incl = wave_inclusive_scan(c);
+      const uint32_t total = uniform_u32(__shfl(incl, 63, 64));
+      if (total == 0) continue;
+      uint32_t at = nw + incl - c;
+      if (m) sum[sw] = 0;
+      while (m) {
+        const uint32_t b = __ffs(m) - 1;
+        list[at++] = sw * 32 + b;
+        m &= m - 1;
+      }
+      nw += total;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    // 2. the marked words in order: their set bits are the sorted set
+    uint32_t out = 0;
+    for (uint32_t i0 = 0; i0 < nw; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      const uint32_t w = i < nw ? list[i] : 0u;
+      uint32_t m = i < nw ? bits[w] : 0u;
+      const uint32_t c = __popc(m);
+      const uint32_t incl = wave_inclusive_scan(c);
+      const uint32_t total = uniform_u32(__shfl(incl, 63, 64));
+      uint32_t at = out + incl - c;
+      if (i < nw) bits[w] = 0;
+      if (fits) {
+        while (m) {
+          const uint32_t b = __ffs(m) - 1;
+          P.data[base + at++] = w * 32 + b;
+          m &= m - 1;
+        }
+      }
+      out += total;
+    }
+    if (lane == 0) {
+      if (fits) {
+        P.set_off[s] = base;
+        P.set_len[s] = len;
+      } else {
+        atomicOr(&P.ctl[kCtlOverflow], 1ull);
+        P.set_len[s] = kLenFail;
+      }
+    }
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    st_edges += __shfl_down(st_edges, d, 64);
+    st_exp += __shfl_down(st_exp, d, 64);
+  }
+  if (lane == 0) {
+    atomicAdd(&P.ctl[kCtlStatEdges], st_edges / 64);
+    atomicAdd(&P.ctl[kCtlStatCoins], st_edges / 64);
+    atomicAdd(&P.ctl[kCtlStatExpansions], st_exp / 64);
+  }
+}
+
+__global__ void reset_big_kernel(unsigned long long *ctl) { ctl[kCtlBigCounter] = 0; }
 
 template <int BT, int U>
 uint32_t wave_grid() {
@@ -327,6 +474,73 @@ uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count) {
                          hipMemcpyDeviceToHost, p->stream));
   RIP_HIP(hipStreamSynchronize(p->stream));
   return nbig;
+}
+
+}  // namespace rip
+
+namespace rip {
+
+// Finishes the `count` samples listed in g->wave.d_big (local indices from
+// `first`) with ic_big_kernel.  Per resident wave it needs n + n/32 + n/1024
+// words; the number of waves is bounded by occupancy and by 1/16 of the
+// device memory.
+void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count) {
+  constexpr int BT = 256, U = 4;
+  rip_graph *g = p->g;
+  WaveScratch &w = g->wave;
+  const uint32_t words = (g->n + 31) / 32, swords = (words + 31) / 32;
+  const uint64_t per_wave = 4ull * (static_cast<uint64_t>(g->n) + words + swords);
+  size_t free_b = 0, total_b = 0;
+  RIP_HIP(hipMemGetInfo(&free_b, &total_b));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void *>(&ic_big_kernel<BT, U>), BT, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  int dev = 0, cus = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  uint64_t waves = std::min<uint64_t>(static_cast<uint64_t>(per_cu) * cus * (BT / 64),
+                                      std::max<uint64_t>(1, (total_b / 16) / per_wave));
+  waves = std::min<uint64_t>(waves, (count + BT / 64 - 1) / (BT / 64) * (BT / 64));
+  waves = std::max<uint64_t>(BT / 64, waves / (BT / 64) * (BT / 64));
+  if (w.big_waves < waves || w.big_n != g->n) {
+    dev_free(w.d_bbits);
+    dev_free(w.d_bsum);
+    dev_free(w.d_blist);
+    w.big_waves = 0;
+    w.d_bbits = dev_alloc<uint32_t>(waves * words);
+    w.d_bsum = dev_alloc<uint32_t>(waves * swords);
+    w.d_blist = dev_alloc<uint32_t>(waves * g->n);
+    RIP_HIP(hipMemsetAsync(w.d_bbits, 0, sizeof(uint32_t) * waves * words, p->stream));
+    RIP_HIP(hipMemsetAsync(w.d_bsum, 0, sizeof(uint32_t) * waves * swords, p->stream));
+    w.big_waves = waves;
+    w.big_n = g->n;
+  }
+  BigParams P;
+  P.vrec = g->d_vrec;
+  P.ids = g->d_ids;
+  P.val = g->d_val;
+  P.n = g->n;
+  P.step = g->step;
+  P.seed = p->seed;
+  P.first = first;
+  P.big_ids = w.d_big;
+  P.count = count;
+  P.bits = w.d_bbits;
+  P.sum = w.d_bsum;
+  P.list = w.d_blist;
+  P.words = words;
+  P.swords = swords;
+  P.set_off = p->d_off;
+  P.set_len = p->d_len;
+  P.data = p->d_data;
+  P.cap_data = p->cap_data;
+  P.ctl = p->d_ctl;
+  hipLaunchKernelGGL(reset_big_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
+  hipLaunchKernelGGL((ic_big_kernel<BT, U>), dim3(static_cast<uint32_t>(waves / (BT / 64))),
+                     dim3(BT), 0, p->stream, P);
+  RIP_HIP(hipGetLastError());
 }
 
 }  // namespace rip
