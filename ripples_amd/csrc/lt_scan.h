@@ -26,18 +26,21 @@ constexpr uint32_t kLtNever = (1u << 24) + 1;
 // Runs the float chain of the row's weights `w` from u = k * 2^-24 through
 // edge j.  Returns whether it survives; `slack` is the value after edge j
 // when it does, and w_i - t_i at the crossing edge i when it does not.
-// Weights are read eight at a time so the loads of a group are in flight
-// together.
+// Weights are read kChainGroup at a time so the loads of a group are in
+// flight together (the chain itself is a dependent sequence of subtractions,
+// so the group size sets how many memory round trips a long row costs).
+constexpr uint32_t kChainGroup = 32;
+
 __host__ __device__ inline bool lt_chain_eval(const uint32_t *w, uint32_t j, uint32_t k,
                                               double &slack) {
   float t = static_cast<float>(k) * 0x1p-24f;
-  for (uint32_t i0 = 0; i0 <= j; i0 += 8) {
-    const uint32_t cnt = j - i0 + 1 < 8 ? j - i0 + 1 : 8;
-    uint32_t wb[8];
+  for (uint32_t i0 = 0; i0 <= j; i0 += kChainGroup) {
+    const uint32_t cnt = j - i0 + 1 < kChainGroup ? j - i0 + 1 : kChainGroup;
+    uint32_t wb[kChainGroup];
 #pragma unroll
-    for (uint32_t u = 0; u < 8; ++u) wb[u] = u < cnt ? w[i0 + u] : 0u;
+    for (uint32_t u = 0; u < kChainGroup; ++u) wb[u] = u < cnt ? w[i0 + u] : 0u;
 #pragma unroll
-    for (uint32_t u = 0; u < 8; ++u) {
+    for (uint32_t u = 0; u < kChainGroup; ++u) {
       if (u >= cnt) break;
       float wi;
       __builtin_memcpy(&wi, &wb[u], 4);
