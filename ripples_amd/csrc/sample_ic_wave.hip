@@ -344,13 +344,17 @@ __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
     uint32_t nw = 0;
     for (uint32_t s0 = 0; s0 < P.swords; s0 += 64) {
       const uint32_t sw = s0 + lane;
-      uint32_t m = sw < P.swords ? sum[sw] : 0u;
+      // the bitmaps are written by device-scope atomics (in L2): read them
+      // past the vector L1, which may hold lines from an earlier sample
+      uint32_t m = sw < P.swords ? __hip_atomic_load(&sum[sw], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                 : 0u;
       const uint32_t c = __popc(m);
       const uint32_t incl = wave_inclusive_scan(c);
       const uint32_t total = uniform_u32(__shfl(incl, 63, 64));
       if (total == 0) continue;
       uint32_t at = nw + incl - c;
-      if (m) sum[sw] = 0;
+      if (m) __hip_atomic_store(&sum[sw], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       while (m) {
         const uint32_t b = __ffs(m) - 1;
         list[at++] = sw * 32 + b;
@@ -364,12 +368,13 @@ __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
     for (uint32_t i0 = 0; i0 < nw; i0 += 64) {
       const uint32_t i = i0 + lane;
       const uint32_t w = i < nw ? list[i] : 0u;
-      uint32_t m = i < nw ? bits[w] : 0u;
+      uint32_t m = i < nw ? __hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : 0u;
       const uint32_t c = __popc(m);
       const uint32_t incl = wave_inclusive_scan(c);
       const uint32_t total = uniform_u32(__shfl(incl, 63, 64));
       uint32_t at = out + incl - c;
-      if (i < nw) bits[w] = 0;
+      if (i < nw) __hip_atomic_store(&bits[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (fits) {
         while (m) {
           const uint32_t b = __ffs(m) - 1;
