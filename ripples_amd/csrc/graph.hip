@@ -6,7 +6,9 @@
 //               threshold (weighted cascade), that threshold
 //   d_val[e]  : IC integer threshold | LT float weight bits
 //   d_ids[e]  : in-neighbour (read only for live / chosen edges)
-//   d_bp[e]   : LT row breakpoints of the float chain (lt_scan.h)
+//   d_bp[2e], d_bp[2e + 1] : LT row breakpoint of the float chain (lt_scan.h)
+//               and the in-neighbour of edge e, side by side so the block a
+//               walk step searches also holds the chosen in-neighbour
 //   d_bidx    : LT fanout-16 index over the breakpoints of rows longer than 16
 // The id and value arrays carry kEdgePad zero elements for 16-byte loads.
 #include <cstring>
@@ -39,7 +41,8 @@ __global__ void build_vrec_kernel(const unsigned long long *index, const uint32_
 // breakpoint B[e - begin] of lt_scan.h.  Threads of a wave take consecutive
 // edges, so they read the same weights in step.
 __global__ void build_bp_kernel(const unsigned long long *index, const uint32_t *val,
-                                const VertexRec *vrec, uint32_t n, uint64_t m, uint32_t *bp) {
+                                const uint32_t *ids, const VertexRec *vrec, uint32_t n,
+                                uint64_t m, uint32_t *bp) {
   const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= m) return;
   uint32_t lo = 0, hi = n;  // row of e: the last v with index[v] <= e
@@ -51,11 +54,12 @@ __global__ void build_bp_kernel(const unsigned long long *index, const uint32_t 
       hi = mid;
   }
   const VertexRec r = vrec[lo];
+  bp[2 * e + 1] = ids[e];
   if (r.flags & kRowScalar) {
-    bp[e] = kLtNever;
+    bp[2 * e] = kLtNever;
     return;
   }
-  bp[e] = lt_breakpoint(val + r.begin, static_cast<uint32_t>(e) - r.begin);
+  bp[2 * e] = lt_breakpoint(val + r.begin, static_cast<uint32_t>(e) - r.begin);
 }
 
 // One thread per indexed LT row: the fanout-16 index over its breakpoints.
@@ -64,7 +68,7 @@ __global__ void build_bidx_kernel(const uint32_t *rows, uint32_t nrows, const Ve
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
   const VertexRec r = vrec[rows[i]];
-  lt_index_build(bp + r.begin, r.end - r.begin, bidx + r.uthr);
+  lt_index_build<2>(bp + 2ull * r.begin, r.end - r.begin, bidx + r.uthr);
 }
 
 }  // namespace
@@ -171,10 +175,11 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
     RIP_HIP(hipGetLastError());
     if (model == RIP_LT && m > 0) {
       // padded so a search may read 16-byte groups past the last row
-      g->d_bp = dev_alloc<uint32_t>(m + kBpPad);
-      RIP_HIP(hipMemset(g->d_bp + m, 0xFF, sizeof(uint32_t) * kBpPad));
+      g->d_bp = dev_alloc<uint32_t>(2 * (m + kBpPad));
+      RIP_HIP(hipMemset(g->d_bp + 2 * m, 0xFF, sizeof(uint32_t) * 2 * kBpPad));
       hipLaunchKernelGGL(build_bp_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)),
-                         dim3(256), 0, 0, d_index, g->d_val, g->d_vrec, n, m, g->d_bp);
+                         dim3(256), 0, 0, d_index, g->d_val, g->d_ids, g->d_vrec, n, m,
+                         g->d_bp);
       RIP_HIP(hipGetLastError());
       if (!index_rows.empty()) {
         const uint32_t nrows = static_cast<uint32_t>(index_rows.size());

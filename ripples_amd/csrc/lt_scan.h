@@ -132,8 +132,9 @@ __host__ __device__ inline uint32_t lt_bp_cross(const uint32_t *B, uint32_t d, u
 // 0xFFFFFFFF (above every threshold), so a search reads one aligned 64-byte
 // node per level and a last block of at most 16 breakpoints.
 constexpr uint32_t kLtFan = 16;
-// Entries after the last breakpoint of the graph, so a block search may read
-// up to 19 entries from the 4-aligned position before a block's start.
+// Pairs after the last (breakpoint, in-neighbour) pair of the graph, so a
+// block search may read 36 words from the 16-byte aligned position at or
+// before a block's start.
 constexpr uint64_t kBpPad = 32;
 
 __host__ __device__ inline uint32_t lt_pad16(uint32_t x) { return (x + 15u) & ~15u; }
@@ -160,8 +161,10 @@ __host__ __device__ inline uint32_t lt_index_size(uint32_t d) {
   return total;
 }
 
-// Writes the index of row breakpoints B[0 .. d) at idx (lt_index_size(d)
-// entries, top level first).
+// Writes the index of row breakpoints B[0], B[S], ... B[(d - 1) S] at idx
+// (lt_index_size(d) entries, top level first); S = 2 reads the breakpoints of
+// a (breakpoint, in-neighbour) pair array.
+template <uint32_t S = 1>
 __host__ __device__ inline void lt_index_build(const uint32_t *B, uint32_t d, uint32_t *idx) {
   uint32_t sz[8];
   const int nl = lt_index_levels(d, sz);
@@ -178,7 +181,7 @@ __host__ __device__ inline void lt_index_build(const uint32_t *B, uint32_t d, ui
       uint32_t v = 0xFFFFFFFFu;
       if (b < sz[l]) {
         const uint32_t last = (16u * b + 15u < below) ? 16u * b + 15u : below - 1u;
-        v = l == 0 ? B[last] : idx[start[l - 1] + last];
+        v = l == 0 ? B[static_cast<uint64_t>(last) * S] : idx[start[l - 1] + last];
       }
       idx[start[l] + b] = v;
     }

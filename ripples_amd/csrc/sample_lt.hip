@@ -69,10 +69,12 @@ __device__ inline uint32_t count_le(uint4 q, uint32_t k) {
 // Crossing edge of threshold k in row [begin, begin + d) (first j with
 // B[j] > k, or d) through the row's fanout-16 breakpoint index at bidx + off
 // (lt_index_cross of lt_scan.h with 16-byte loads): one aligned 64-byte node
-// per level, then one block of at most 16 breakpoints.  *probes counts the
-// nodes and blocks read.
+// per level, then one block of at most 16 (breakpoint, in-neighbour) pairs,
+// which also gives *id, the in-neighbour of the crossing edge when there is
+// one.  *probes counts the nodes and blocks read.
 __device__ inline uint32_t lt_cross(const uint32_t *bp, const uint32_t *bidx, uint32_t begin,
-                                    uint32_t d, uint32_t off, uint32_t k, uint32_t *probes) {
+                                    uint32_t d, uint32_t off, uint32_t k, uint32_t *probes,
+                                    uint32_t *id) {
   // level l holds ceil(d / 16^(l+1)) entries; the top is the first level of
   // at most 16
   int nl = 0;
@@ -94,23 +96,34 @@ __device__ inline uint32_t lt_cross(const uint32_t *bp, const uint32_t *bidx, ui
     node = child;
     start += lt_pad16(sz);
   }
-  // the block [first, first + cnt) of at most 16 breakpoints, read as five
-  // aligned 16-byte groups from first & ~3 (the array is padded past its end)
+  // the block of at most 16 pairs from edge first = begin + lo: words
+  // [2 first, 2 first + 2 cnt), read as nine aligned 16-byte groups from
+  // 2 first & ~3 (the pair array is padded past its end)
   const uint32_t lo = 16u * node;
   const uint32_t cnt = d - lo < 16u ? d - lo : 16u;
-  const uint32_t first = begin + lo, a = first & ~3u, skip = first - a;
+  const uint64_t w0 = 2ull * (begin + lo), a = w0 & ~3ull;
+  const bool shifted = w0 != a;  // the block starts at the second pair of a group
   const uint4 *q = reinterpret_cast<const uint4 *>(bp + a);
-  uint32_t c = 0;
+  uint32_t w[36];
 #pragma unroll
-  for (uint32_t t = 0; t < 5; ++t) {
+  for (int t = 0; t < 9; ++t) {
     const uint4 v = q[t];
-    const uint32_t e0 = 4 * t - skip;  // position of v.x within the block (wraps below 0)
-    c += (e0 < cnt && v.x <= k) ? 1u : 0u;
-    c += (e0 + 1 < cnt && v.y <= k) ? 1u : 0u;
-    c += (e0 + 2 < cnt && v.z <= k) ? 1u : 0u;
-    c += (e0 + 3 < cnt && v.w <= k) ? 1u : 0u;
+    w[4 * t] = v.x;
+    w[4 * t + 1] = v.y;
+    w[4 * t + 2] = v.z;
+    w[4 * t + 3] = v.w;
+  }
+  uint32_t c = 0, chosen = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 16; ++j) {
+    const uint32_t b = shifted ? w[2 * j + 2] : w[2 * j];
+    const uint32_t u = shifted ? w[2 * j + 3] : w[2 * j + 1];
+    // breakpoints rise along the block: the crossing pair is the first > k
+    if (j < cnt && j == c && b <= k) ++c;
+    if (j == c) chosen = u;
   }
   ++*probes;
+  if (c < cnt) *id = chosen;
   return lo + c;
 }
 
@@ -246,11 +259,12 @@ lt_walk_kernel(LtParams P) {
     bool done = false, is_long = false;
     if (busy) {
       bool found = false;
+      uint32_t u = 0;  // the chosen in-neighbour
       if (e >= end) {
         done = true;  // no in-edge crosses the threshold: the walk stops
       } else if (!(rflags & kRowScalar)) {
         const uint32_t d = end - e;  // e is the row start
-        const uint32_t c = lt_cross(P.bp, P.bidx, e, d, ridx, kthr, &probes);
+        const uint32_t c = lt_cross(P.bp, P.bidx, e, d, ridx, kthr, &probes, &u);
         edges += c < d ? c + 1 : d;
         if (c < d) {
           found = true;
@@ -284,9 +298,9 @@ lt_walk_kernel(LtParams P) {
         edges += stop - lo;
         e = base + stop;
         if (!found && e >= end) done = true;
+        if (found) u = P.ids[e - 1];
       }
       if (found) {
-        const uint32_t u = P.ids[e - 1];  // the chosen in-neighbour
         bool seen = false;
         if (filter_has(u))
           for (uint32_t i = 0; i < len; ++i)
@@ -448,21 +462,21 @@ __global__ void lt_long_kernel(LtParams P, uint32_t num_long, uint32_t lo,
   for (;;) {
     const VertexRec r = P.vrec[v];
     const uint32_t k = lt_threshold(r, z);
-    uint32_t e = r.end;
+    uint32_t e = r.end, u = 0;
     if (!(r.flags & kRowScalar)) {
-      e = r.begin + lt_cross(P.bp, P.bidx, r.begin, r.end - r.begin, r.uthr, k, &probes);
+      e = r.begin + lt_cross(P.bp, P.bidx, r.begin, r.end - r.begin, r.uthr, k, &probes, &u);
     } else {
       float t = static_cast<float>(k) * 0x1p-24f;
       for (uint32_t i = r.begin; i < r.end; ++i) {
         t -= __uint_as_float(P.val[i]);
         if (!(t > 0)) {
           e = i;
+          u = P.ids[i];
           break;
         }
       }
     }
     if (e >= r.end) break;
-    const uint32_t u = P.ids[e];
     if ((bm[u >> 5] >> (u & 31)) & 1u) break;
     list[len++] = u;
     bm[u >> 5] |= 1u << (u & 31);
