@@ -112,7 +112,8 @@ __device__ inline void wave_bitonic(Ptr a, uint32_t len, uint32_t cap2) {
 // the coins of in-edges begin + l + 64 i (the row's first coin is P^(1+begin)
 // applied to z; jl = P^l, j64 = P^64), and the in-neighbours of live edges for
 // which visit(src) reports a first visit are appended to list[tail ...].
-// Returns false, leaving the list unfinished, when it would pass cap.
+// Returns false, leaving the row unfinished, when the live edges of a round
+// (an upper bound on the new vertices) would take the list past cap.
 template <int U, typename Visit>
 __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
                                   const uint32_t *val, uint32_t v, uint64_t z,
@@ -331,8 +332,11 @@ __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       const uint32_t v = uniform_u32(list[head++]);
       ++st_exp;
-      // every vertex enters the list once, so it never passes n entries
-      expand_row<U>(P.vrec, P.ids, P.val, v, z, jl, j64, list, tail, P.n, st_edges, visit);
+      // every vertex enters the list once, so it never passes n entries; the
+      // cap test of expand_row counts live edges, not new vertices, so it
+      // gets no bound here (with cap n it could stop a row near the end)
+      expand_row<U>(P.vrec, P.ids, P.val, v, z, jl, j64, list, tail, 0xFFFFFFFFu, st_edges,
+                    visit);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const uint32_t len = tail;
