@@ -81,8 +81,9 @@ def ic_plain_bytes(index, ver):
 
 def cpu_baseline(wl, index, edges, w, seed, batch, target_s):
     from oracle import oracle as O
-    threads = os.cpu_count() or 1
-    threads = min(threads, 64)
+    # the host's CPU share: OMP_NUM_THREADS where the environment sets it,
+    # else at most 16 threads
+    threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
     probe = max(1000, min(batch, 20000))
     t = time.time()
     O.sample(wl["model"], index, edges, w, seed, 0, probe, threads=threads)

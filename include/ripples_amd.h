@@ -56,8 +56,9 @@ typedef struct {
 } rip_tune;
 
 /* IC sampler kinds: one reverse BFS per wave, with sets larger than 4096
- * vertices finished by a bitmap-visited BFS per wave (AUTO, WAVE), or
- * 64-sample colour groups for every sample (COLOUR).  All give the same sets. */
+ * vertices finished by 64-sample colour groups (AUTO) or by a bitmap-visited
+ * BFS per wave (WAVE), or colour groups for every sample (COLOUR).  All give
+ * the same sets. */
 #define RIP_SAMPLER_AUTO 0
 #define RIP_SAMPLER_WAVE 1
 #define RIP_SAMPLER_COLOUR 2

@@ -560,8 +560,15 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
   if (cfg.sampler == RIP_SAMPLER_COLOUR) return ic_launch(p, first, count, cfg);
   const uint64_t big = ic_wave_launch(p, first, count);
   if (big == 0) return 1;
-  // sets that outgrew a wave's list: the large-set kernel
-  ic_big_launch(p, first, big);
+  // sets that outgrew a wave's list: by default colour groups over the listed
+  // sample indices (samples that reach a giant component share most of their
+  // rows, which the groups read once for 64 of them); WAVE finishes them with
+  // the bitmap-visited BFS per wave instead
+  if (cfg.sampler == RIP_SAMPLER_WAVE) {
+    ic_big_launch(p, first, big);
+  } else {
+    ic_launch(p, first, big, cfg, p->g->wave.d_big, false);
+  }
   return 3;
 }
 

@@ -166,11 +166,11 @@ __device__ inline void wave_sort128(uint32_t &x0, uint32_t &x1) {
 // longer walks are handed to lt_long_kernel.  Finished lists up to CAP are
 // sorted four at a time in 16-lane register groups; longer ones are copied
 // by the whole wave and sorted by lt_sort_kernel.
-// Eight waves per SIMD: the walk is latency-bound, and at 64 registers the
-// kernel needs no scratch.
+// The compiler's own register budget (87 registers, five waves per SIMD, no
+// scratch) measured as fast as six or seven waves and 4% faster than eight
+// waves with spills, on R-MAT scale 20.
 template <int BT, int CAP, int SPILL, int UNROLL>
-__global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(8, 8)))
-lt_walk_kernel(LtParams P) {
+__global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   static_assert(UNROLL % 4 == 0, "groups of four weights per load");
   static_assert(CAP <= 16 && CAP + SPILL <= 128, "16-lane group sort and 128-key wave sort");
   __shared__ uint32_t s_list[CAP * BT];

@@ -70,15 +70,17 @@ def test_selection_matches_oracle(case, orc):
     p.close()
 
 
-def test_ic_bimodal_sets_match_oracle(orc):
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_ic_bimodal_sets_match_oracle(sampler, orc):
     """R-MAT scale 16 with U[0,0.1] weights: most sets are small, a fifth reach
     the giant component of ~12K vertices (the wave kernel hands those to the
-    large-set kernel); appended in three calls of uneven size."""
+    colour groups or, with WAVE, to the large-set kernel); appended in three
+    calls of uneven size."""
     index, edges, w = rmat_csr(16, R.W_UNIFORM, 0.1, 3)
     g = R.Graph(index, edges, w, model=R.IC)
     p = g.pool(6)
     for c in (700, 1, 1299):
-        p.generate_rrr_sets(c)
+        p.generate_rrr_sets(c, tune={"sampler": sampler})
     got = p.export()
     assert_same_sets(got, orc.sample(R.IC, index, edges, w, 6, 0, 2000))
     assert (np.diff(got[0]) > 4096).sum() > 0
@@ -155,11 +157,13 @@ def test_pool_growth_retry(orc):
     g.close()
 
 
+@pytest.mark.parametrize("sampler", [0, 1])
 @pytest.mark.parametrize("n", [1500, 3000, 6000, 20000, 70000])
-def test_ic_large_sets(n, orc):
-    """Sets of ~n vertices: LDS sort (<= 1024), HBM sort (<= 4096) and the
-    large-set kernel (bitmap visited, summary walk over several passes of 64
-    summary words at n = 70000) for sets a wave's list cannot hold."""
+def test_ic_large_sets(n, sampler, orc):
+    """Sets of ~n vertices: LDS sort (<= 1024), HBM sort (<= 4096) and, for
+    sets a wave's list cannot hold, the colour groups (AUTO) or the large-set
+    kernel (WAVE: bitmap visited, summary walk over several passes of 64
+    summary words at n = 70000)."""
     rng = np.random.default_rng(n)
     s = rng.integers(0, n, 8 * n)
     d = rng.integers(0, n, 8 * n)
@@ -171,7 +175,7 @@ def test_ic_large_sets(n, orc):
     g = R.Graph(index, edges, w, model=R.IC)
     p = g.pool(4)
     cnt = 300 if n <= 20000 else 80
-    p.generate_rrr_sets(cnt)
+    p.generate_rrr_sets(cnt, tune={"sampler": sampler})
     got = p.export()
     assert_same_sets(got, orc.sample(R.IC, index, edges, w, 4, 0, cnt))
     assert np.diff(got[0]).max() > min(n - 100, 4096) or n < 4096
