@@ -374,7 +374,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0, help="sets per step (0 = theta)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="CPU-baseline budget as estimated from a small probe (the full "
+                         "run usually takes a third of it)")
     ap.add_argument("--tune", default="", help='sampler tuning, e.g. "sampler=2,block=256"')
     ap.add_argument("--skip-imm", action="store_true")
     ap.add_argument("--serve-scales", default="18,20,22",
