@@ -87,6 +87,24 @@ struct alignas(32) VertexRec {
   uint32_t flags;        // kRowUniform, kRowScalar
 };
 
+// LT walk record of a vertex: one aligned 128-byte line holds what a walk
+// step reads before its breakpoint block.  The row's (breakpoint,
+// in-neighbour) pairs start at pair boff, a multiple of 16, so each block of
+// 16 pairs is one aligned 128-byte line; `top` is the top level of the row's
+// breakpoint index (rows longer than 16 edges), the levels below it sit in
+// the graph's index array at ioff, after their own copy of the top level.
+struct alignas(128) LtRec {
+  uint64_t ja, jb;        // P^(1 + begin): jump to the vertex's threshold element
+  uint32_t boff;          // first pair of the row
+  uint32_t d;             // in-degree
+  uint32_t ioff;          // the row's index in the index array
+  uint32_t flags;         // kRowScalar
+  uint32_t top[16];       // top index level, padded with 0xFFFFFFFF
+  uint32_t begin;         // transposed-CSR row start (rows scanned by weight)
+  uint32_t pad[7];
+};
+static_assert(sizeof(LtRec) == 128, "one line per LT record");
+
 constexpr uint32_t kRowUniform = 1u;  // IC: uthr holds the row's threshold
 constexpr uint32_t kRowScalar = 2u;   // LT: no breakpoints, scan the float chain
 // Extra zero elements after the per-edge arrays, so 16-byte loads of an

@@ -6,9 +6,12 @@
 //               threshold (weighted cascade), that threshold
 //   d_val[e]  : IC integer threshold | LT float weight bits
 //   d_ids[e]  : in-neighbour (read only for live / chosen edges)
-//   d_bp[2e], d_bp[2e + 1] : LT row breakpoint of the float chain (lt_scan.h)
-//               and the in-neighbour of edge e, side by side so the block a
-//               walk step searches also holds the chosen in-neighbour
+//   d_lrec[v] : LT walk record (common.h), one 128-byte line per vertex
+//   d_bp[2p], d_bp[2p + 1] : for pair p = boff(v) + j, the LT row breakpoint
+//               of the float chain (lt_scan.h) and the in-neighbour of the
+//               row's edge j, side by side so the block a walk step searches
+//               also holds the chosen in-neighbour; rows are padded to whole
+//               blocks of 16 pairs (breakpoint 0xFFFFFFFF)
 //   d_bidx    : LT fanout-16 index over the breakpoints of rows longer than 16
 // The id and value arrays carry kEdgePad zero elements for 16-byte loads.
 #include <cstring>
@@ -37,12 +40,31 @@ __global__ void build_vrec_kernel(const unsigned long long *index, const uint32_
   vrec[v] = r;
 }
 
-// One thread per edge e of an LT row without irregular weights: the row
-// breakpoint B[e - begin] of lt_scan.h.  Threads of a wave take consecutive
-// edges, so they read the same weights in step.
+__global__ void build_lrec_kernel(const VertexRec *vrec, const uint32_t *boff, uint32_t n,
+                                  LtRec *lrec) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const VertexRec r = vrec[v];
+  LtRec o;
+  o.ja = r.ja;
+  o.jb = r.jb;
+  o.boff = boff[v];
+  o.d = r.end - r.begin;
+  o.ioff = r.uthr;
+  o.flags = r.flags;
+  for (int i = 0; i < 16; ++i) o.top[i] = 0xFFFFFFFFu;
+  o.begin = r.begin;
+  for (int i = 0; i < 7; ++i) o.pad[i] = 0;
+  lrec[v] = o;
+}
+
+// One thread per edge e of an LT row: the row breakpoint B[e - begin] of
+// lt_scan.h (kLtNever in rows with irregular weights) and the in-neighbour,
+// at pair boff + e - begin.  Threads of a wave take consecutive edges, so
+// they read the same weights in step.
 __global__ void build_bp_kernel(const unsigned long long *index, const uint32_t *val,
-                                const uint32_t *ids, const VertexRec *vrec, uint32_t n,
-                                uint64_t m, uint32_t *bp) {
+                                const uint32_t *ids, const VertexRec *vrec, const LtRec *lrec,
+                                uint32_t n, uint64_t m, uint32_t *bp) {
   const uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= m) return;
   uint32_t lo = 0, hi = n;  // row of e: the last v with index[v] <= e
@@ -54,21 +76,24 @@ __global__ void build_bp_kernel(const unsigned long long *index, const uint32_t 
       hi = mid;
   }
   const VertexRec r = vrec[lo];
-  bp[2 * e + 1] = ids[e];
+  const uint64_t p = lrec[lo].boff + (e - r.begin);
+  bp[2 * p + 1] = ids[e];
   if (r.flags & kRowScalar) {
-    bp[2 * e] = kLtNever;
+    bp[2 * p] = kLtNever;
     return;
   }
-  bp[2 * e] = lt_breakpoint(val + r.begin, static_cast<uint32_t>(e) - r.begin);
+  bp[2 * p] = lt_breakpoint(val + r.begin, static_cast<uint32_t>(e) - r.begin);
 }
 
-// One thread per indexed LT row: the fanout-16 index over its breakpoints.
-__global__ void build_bidx_kernel(const uint32_t *rows, uint32_t nrows, const VertexRec *vrec,
+// One thread per indexed LT row: the fanout-16 index over its breakpoints,
+// top level first, and a copy of the top level in the walk record.
+__global__ void build_bidx_kernel(const uint32_t *rows, uint32_t nrows, LtRec *lrec,
                                   const uint32_t *bp, uint32_t *bidx) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
-  const VertexRec r = vrec[rows[i]];
-  lt_index_build<2>(bp + 2ull * r.begin, r.end - r.begin, bidx + r.uthr);
+  LtRec &r = lrec[rows[i]];
+  lt_index_build<2>(bp + 2ull * r.boff, r.d, bidx + r.ioff);
+  for (int t = 0; t < 16; ++t) r.top[t] = bidx[r.ioff + t];
 }
 
 }  // namespace
@@ -153,7 +178,7 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
   for (int b = 0; b < 33; ++b) g->deg_ge_pow2[b] = deg_ge[b];
 
   unsigned long long *d_index = nullptr;
-  uint32_t *d_uthr = nullptr, *d_flags = nullptr, *d_rows = nullptr;
+  uint32_t *d_uthr = nullptr, *d_flags = nullptr, *d_rows = nullptr, *d_boff = nullptr;
   try {
     g->d_vrec = dev_alloc<VertexRec>(n);
     g->d_ids = dev_alloc<uint32_t>(m + kEdgePad);
@@ -175,11 +200,24 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
     RIP_HIP(hipGetLastError());
     if (model == RIP_LT && m > 0) {
       // padded so a search may read 16-byte groups past the last row
-      g->d_bp = dev_alloc<uint32_t>(2 * (m + kBpPad));
-      RIP_HIP(hipMemset(g->d_bp + 2 * m, 0xFF, sizeof(uint32_t) * 2 * kBpPad));
+      // pair offsets: every row padded to whole blocks of 16 pairs
+      std::vector<uint32_t> boff(n);
+      uint64_t pairs = 0;
+      for (uint32_t v = 0; v < n; ++v) {
+        boff[v] = static_cast<uint32_t>(pairs);
+        pairs += lt_pad16(static_cast<uint32_t>(index[v + 1] - index[v]));
+        RIP_REQUIRE(pairs + kBpPad < (1ull << 32), -1, "graph: more than 2^32 LT pairs");
+      }
+      d_boff = dev_alloc<uint32_t>(n);
+      RIP_HIP(hipMemcpy(d_boff, boff.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+      g->d_lrec = dev_alloc<LtRec>(n);
+      hipLaunchKernelGGL(build_lrec_kernel, grid, dim3(256), 0, 0, g->d_vrec, d_boff, n,
+                         g->d_lrec);
+      g->d_bp = dev_alloc<uint32_t>(2 * (pairs + kBpPad));
+      RIP_HIP(hipMemset(g->d_bp, 0xFF, sizeof(uint32_t) * 2 * (pairs + kBpPad)));
       hipLaunchKernelGGL(build_bp_kernel, dim3(static_cast<uint32_t>((m + 255) / 256)),
-                         dim3(256), 0, 0, d_index, g->d_val, g->d_ids, g->d_vrec, n, m,
-                         g->d_bp);
+                         dim3(256), 0, 0, d_index, g->d_val, g->d_ids, g->d_vrec, g->d_lrec, n,
+                         m, g->d_bp);
       RIP_HIP(hipGetLastError());
       if (!index_rows.empty()) {
         const uint32_t nrows = static_cast<uint32_t>(index_rows.size());
@@ -188,7 +226,7 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
         RIP_HIP(hipMemcpy(d_rows, index_rows.data(), sizeof(uint32_t) * nrows,
                           hipMemcpyHostToDevice));
         hipLaunchKernelGGL(build_bidx_kernel, dim3((nrows + 255) / 256), dim3(256), 0, 0, d_rows,
-                           nrows, g->d_vrec, g->d_bp, g->d_bidx);
+                           nrows, g->d_lrec, g->d_bp, g->d_bidx);
         RIP_HIP(hipGetLastError());
       }
     }
@@ -198,6 +236,7 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
     dev_free(d_uthr);
     dev_free(d_flags);
     dev_free(d_rows);
+    dev_free(d_boff);
     graph_destroy(g);
     throw;
   }
@@ -205,6 +244,7 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
   dev_free(d_uthr);
   dev_free(d_flags);
   dev_free(d_rows);
+  dev_free(d_boff);
 }
 
 void graph_destroy(rip_graph *g) {
@@ -228,6 +268,7 @@ void graph_destroy(rip_graph *g) {
   dev_free(g->lt.d_long_ids);
   dev_free(g->lt.d_spill);
   dev_free(g->d_bp);
+  dev_free(g->d_lrec);
   dev_free(g->d_bidx);
   dev_free(g->fwd.d_rec);
   dev_free(g->fwd.d_dst);

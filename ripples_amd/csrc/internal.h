@@ -115,7 +115,9 @@ struct rip_graph {
   rip::VertexRec *d_vrec = nullptr;  // n records
   uint32_t *d_ids = nullptr;         // m + kEdgePad: in-neighbour of each edge
   uint32_t *d_val = nullptr;         // m + kEdgePad: IC threshold | LT weight bits
-  uint32_t *d_bp = nullptr;          // LT, 2m: (row breakpoint, in-neighbour) per edge
+  rip::LtRec *d_lrec = nullptr;      // LT: n walk records
+  uint32_t *d_bp = nullptr;          // LT: (row breakpoint, in-neighbour) pairs, rows
+                                     // padded to multiples of 16 pairs
   uint32_t *d_bidx = nullptr;        // LT: fanout-16 index of rows longer than 16
   uint64_t deg_ge_pow2[33] = {};     // vertices with in-degree >= 2^b
   // Sampler scratch shared by every pool of this graph; pool_sample holds

@@ -36,7 +36,7 @@ constexpr uint32_t kLenFail = 0xFFFFFFFFu;
 constexpr uint32_t kLenLong = 0xFFFFFFFEu;
 
 struct LtParams {
-  const VertexRec *vrec;
+  const LtRec *lrec;
   const uint32_t *ids;
   const uint32_t *val;
   const uint32_t *bp;
@@ -57,24 +57,46 @@ struct LtParams {
 // Sample ids a wave claims from the work counter at a time.
 constexpr unsigned long long kClaim = 256;
 
-// LT threshold of the walk at a vertex: U = k * 2^-24.
-__device__ inline uint32_t lt_threshold(const VertexRec &r, uint64_t z) {
-  return static_cast<uint32_t>((r.ja * z + r.jb) >> 40) + 1u;
+// LT threshold of the walk at a vertex (ja, jb from its record): U = k * 2^-24.
+__device__ inline uint32_t lt_threshold(uint64_t ja, uint64_t jb, uint64_t z) {
+  return static_cast<uint32_t>((ja * z + jb) >> 40) + 1u;
+}
+
+// The first 32 bytes of an LT record: jump, pair offset, degree, index
+// offset, flags.
+struct LtHead {
+  uint64_t ja, jb;
+  uint32_t boff, d, ioff, flags;
+};
+
+__device__ inline LtHead lt_head(const LtRec *lrec, uint32_t v) {
+  const uint4 *q = reinterpret_cast<const uint4 *>(lrec + v);
+  const uint4 a = q[0], b = q[1];
+  LtHead h;
+  h.ja = (static_cast<uint64_t>(a.y) << 32) | a.x;
+  h.jb = (static_cast<uint64_t>(a.w) << 32) | a.z;
+  h.boff = b.x;
+  h.d = b.y;
+  h.ioff = b.z;
+  h.flags = b.w;
+  return h;
 }
 
 __device__ inline uint32_t count_le(uint4 q, uint32_t k) {
   return (q.x <= k ? 1u : 0u) + (q.y <= k ? 1u : 0u) + (q.z <= k ? 1u : 0u) + (q.w <= k ? 1u : 0u);
 }
 
-// Crossing edge of threshold k in row [begin, begin + d) (first j with
-// B[j] > k, or d) through the row's fanout-16 breakpoint index at bidx + off
-// (lt_index_cross of lt_scan.h with 16-byte loads): one aligned 64-byte node
-// per level, then one block of at most 16 (breakpoint, in-neighbour) pairs,
+// Crossing edge of threshold k in the row of vertex v (d edges, first pair
+// boff): the first j with B[j] > k, or d.  The top level of the row's
+// fanout-16 index comes from v's record (the same line as its head); each
+// level below it is one aligned 64-byte node of the index array; the last
+// read is one aligned 128-byte block of 16 (breakpoint, in-neighbour) pairs,
 // which also gives *id, the in-neighbour of the crossing edge when there is
-// one.  *probes counts the nodes and blocks read.
-__device__ inline uint32_t lt_cross(const uint32_t *bp, const uint32_t *bidx, uint32_t begin,
-                                    uint32_t d, uint32_t off, uint32_t k, uint32_t *probes,
-                                    uint32_t *id) {
+// one.  *probes counts the nodes and blocks read (lt_index_cross of lt_scan.h
+// is the same search over plain arrays).
+__device__ inline uint32_t lt_cross(const LtRec *lrec, uint32_t v, const uint32_t *bp,
+                                    const uint32_t *bidx, uint32_t boff, uint32_t d,
+                                    uint32_t ioff, uint32_t k, uint32_t *probes, uint32_t *id) {
   // level l holds ceil(d / 16^(l+1)) entries; the top is the first level of
   // at most 16
   int nl = 0;
@@ -83,44 +105,44 @@ __device__ inline uint32_t lt_cross(const uint32_t *bp, const uint32_t *bidx, ui
     ++nl;
   if (d > kLtFan) ++nl;
   uint32_t node = 0;
-  uint32_t start = off;  // the top level comes first
-  for (int l = nl - 1; l >= 0; --l) {
-    const uint32_t sz =
-        static_cast<uint32_t>((static_cast<uint64_t>(d) + (1ull << (4 * l + 4)) - 1) >> (4 * l + 4));
-    const uint4 *q = reinterpret_cast<const uint4 *>(bidx + start + 16u * node);
+  if (nl > 0) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(lrec[v].top);
     const uint4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
     const uint32_t p = count_le(q0, k) + count_le(q1, k) + count_le(q2, k) + count_le(q3, k);
+    const uint32_t szt = static_cast<uint32_t>(
+        (static_cast<uint64_t>(d) + (1ull << (4 * (nl - 1) + 4)) - 1) >> (4 * (nl - 1) + 4));
     ++*probes;
-    const uint32_t child = 16u * node + p;
-    if (child >= sz) return d;  // every breakpoint is <= k
-    node = child;
-    start += lt_pad16(sz);
+    if (p >= szt) return d;  // every breakpoint is <= k
+    node = p;
+    uint32_t start = ioff + lt_pad16(szt);  // levels below the top
+    for (int l = nl - 2; l >= 0; --l) {
+      const uint32_t sz = static_cast<uint32_t>(
+          (static_cast<uint64_t>(d) + (1ull << (4 * l + 4)) - 1) >> (4 * l + 4));
+      const uint4 *r = reinterpret_cast<const uint4 *>(bidx + start + 16u * node);
+      const uint4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+      const uint32_t pl = count_le(r0, k) + count_le(r1, k) + count_le(r2, k) + count_le(r3, k);
+      ++*probes;
+      const uint32_t child = 16u * node + pl;
+      if (child >= sz) return d;
+      node = child;
+      start += lt_pad16(sz);
+    }
   }
-  // the block of at most 16 pairs from edge first = begin + lo: words
-  // [2 first, 2 first + 2 cnt), read as nine aligned 16-byte groups from
-  // 2 first & ~3 (the pair array is padded past its end)
+  // block `node`: pairs boff + 16 node .. + 15, one aligned line
   const uint32_t lo = 16u * node;
   const uint32_t cnt = d - lo < 16u ? d - lo : 16u;
-  const uint64_t w0 = 2ull * (begin + lo), a = w0 & ~3ull;
-  const bool shifted = w0 != a;  // the block starts at the second pair of a group
-  const uint4 *q = reinterpret_cast<const uint4 *>(bp + a);
-  uint32_t w[36];
-#pragma unroll
-  for (int t = 0; t < 9; ++t) {
-    const uint4 v = q[t];
-    w[4 * t] = v.x;
-    w[4 * t + 1] = v.y;
-    w[4 * t + 2] = v.z;
-    w[4 * t + 3] = v.w;
-  }
+  const uint4 *q = reinterpret_cast<const uint4 *>(bp + 2ull * (boff + lo));
   uint32_t c = 0, chosen = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < 16; ++j) {
-    const uint32_t b = shifted ? w[2 * j + 2] : w[2 * j];
-    const uint32_t u = shifted ? w[2 * j + 3] : w[2 * j + 1];
-    // breakpoints rise along the block: the crossing pair is the first > k
-    if (j < cnt && j == c && b <= k) ++c;
-    if (j == c) chosen = u;
+  for (int t = 0; t < 8; ++t) {
+    const uint4 g = q[t];
+    // pairs 2t and 2t + 1: breakpoints rise along the block, so the crossing
+    // pair is the first whose breakpoint passes k
+    const uint32_t j0 = 2 * t, j1 = 2 * t + 1;
+    if (j0 < cnt && j0 == c && g.x <= k) ++c;
+    if (j0 == c) chosen = g.y;
+    if (j1 < cnt && j1 == c && g.z <= k) ++c;
+    if (j1 == c) chosen = g.w;
   }
   ++*probes;
   if (c < cnt) *id = chosen;
@@ -187,7 +209,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   // ids are handed out in increasing order, so a wave only stops once every
   // id it still holds is past `count`
   uint64_t claim_lo = 0, claim_hi = 0;
-  uint32_t len = 0, e = 0, end = 0, rflags = 0, kthr = 0, ridx = 0;
+  uint32_t len = 0, e = 0, end = 0, rflags = 0, kthr = 0, ridx = 0, vcur = 0;
   uint64_t filt0 = 0, filt1 = 0;  // visited filter, bits 0..63 and 64..127
   auto filter_add = [&](uint32_t v) {
     const uint32_t b = lt_filter_bit(v);
@@ -205,14 +227,17 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   // breakpoint blocks read, vertices written out (every visited vertex is)
   uint32_t edges = 0, probes = 0, outs = 0;
 
+  // e, end: the row's pairs for a breakpoint search, its transposed-CSR
+  // range for rows scanned by weight
   auto enter = [&](uint32_t v) {
-    const VertexRec r = P.vrec[v];
-    kthr = lt_threshold(r, z);
+    const LtHead h = lt_head(P.lrec, v);
+    kthr = lt_threshold(h.ja, h.jb, z);
     t = static_cast<float>(kthr) * 0x1p-24f;
-    e = r.begin;
-    end = r.end;
-    rflags = r.flags;
-    ridx = r.uthr;
+    e = (h.flags & kRowScalar) ? P.lrec[v].begin : h.boff;
+    end = e + h.d;
+    rflags = h.flags;
+    ridx = h.ioff;
+    vcur = v;
   };
 
   for (;;) {
@@ -263,8 +288,8 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
       if (e >= end) {
         done = true;  // no in-edge crosses the threshold: the walk stops
       } else if (!(rflags & kRowScalar)) {
-        const uint32_t d = end - e;  // e is the row start
-        const uint32_t c = lt_cross(P.bp, P.bidx, e, d, ridx, kthr, &probes, &u);
+        const uint32_t d = end - e;  // e is the row's first pair
+        const uint32_t c = lt_cross(P.lrec, vcur, P.bp, P.bidx, e, d, ridx, kthr, &probes, &u);
         edges += c < d ? c + 1 : d;
         if (c < d) {
           found = true;
@@ -460,23 +485,24 @@ __global__ void lt_long_kernel(LtParams P, uint32_t num_long, uint32_t lo,
   list[len++] = v;
   bm[v >> 5] |= 1u << (v & 31);
   for (;;) {
-    const VertexRec r = P.vrec[v];
-    const uint32_t k = lt_threshold(r, z);
-    uint32_t e = r.end, u = 0;
-    if (!(r.flags & kRowScalar)) {
-      e = r.begin + lt_cross(P.bp, P.bidx, r.begin, r.end - r.begin, r.uthr, k, &probes, &u);
+    const LtHead h = lt_head(P.lrec, v);
+    const uint32_t k = lt_threshold(h.ja, h.jb, z);
+    uint32_t c = h.d, u = 0;  // crossing edge of the row, d when none
+    if (!(h.flags & kRowScalar)) {
+      c = lt_cross(P.lrec, v, P.bp, P.bidx, h.boff, h.d, h.ioff, k, &probes, &u);
     } else {
+      const uint32_t begin = P.lrec[v].begin;
       float t = static_cast<float>(k) * 0x1p-24f;
-      for (uint32_t i = r.begin; i < r.end; ++i) {
-        t -= __uint_as_float(P.val[i]);
+      for (uint32_t i = 0; i < h.d; ++i) {
+        t -= __uint_as_float(P.val[begin + i]);
         if (!(t > 0)) {
-          e = i;
-          u = P.ids[i];
+          c = i;
+          u = P.ids[begin + i];
           break;
         }
       }
     }
-    if (e >= r.end) break;
+    if (c >= h.d) break;
     if ((bm[u >> 5] >> (u & 31)) & 1u) break;
     list[len++] = u;
     bm[u >> 5] |= 1u << (u & 31);
@@ -536,7 +562,7 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   constexpr int BT = 256, CAP = 16, SPILL = 112, UNROLL = 16;
   const rip_graph *g = p->g;
   LtParams P;
-  P.vrec = g->d_vrec;
+  P.lrec = g->d_lrec;
   P.ids = g->d_ids;
   P.val = g->d_val;
   P.bp = g->d_bp;
