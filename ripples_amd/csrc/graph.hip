@@ -7,11 +7,12 @@
 //   d_val[e]  : IC integer threshold | LT float weight bits
 //   d_ids[e]  : in-neighbour (read only for live / chosen edges)
 //   d_lrec[v] : LT walk record (common.h), one 128-byte line per vertex
-//   d_bp[2p], d_bp[2p + 1] : for pair p = boff(v) + j, the LT row breakpoint
-//               of the float chain (lt_scan.h) and the in-neighbour of the
-//               row's edge j, side by side so the block a walk step searches
-//               also holds the chosen in-neighbour; rows are padded to whole
-//               blocks of 16 pairs (breakpoint 0xFFFFFFFF)
+//   d_bp      : LT rows in blocks of 16 edges, one aligned 128-byte line
+//               each: the 16 row breakpoints of the float chain (lt_scan.h)
+//               then the 16 in-neighbours, so the block a walk step searches
+//               also holds the chosen in-neighbour.  Edge j of the row of v is
+//               slot p % 16 of block p / 16 for p = boff(v) + j; rows are
+//               padded to whole blocks (breakpoint 0xFFFFFFFF).
 //   d_bidx    : LT fanout-16 index over the breakpoints of rows longer than 16
 // The id and value arrays carry kEdgePad zero elements for 16-byte loads.
 #include <cstring>
@@ -60,8 +61,8 @@ __global__ void build_lrec_kernel(const VertexRec *vrec, const uint32_t *boff, u
 
 // One thread per edge e of an LT row: the row breakpoint B[e - begin] of
 // lt_scan.h (kLtNever in rows with irregular weights) and the in-neighbour,
-// at pair boff + e - begin.  Threads of a wave take consecutive edges, so
-// they read the same weights in step.
+// in slot p % 16 of block p / 16 for p = boff + e - begin.  Threads of a
+// wave take consecutive edges, so they read the same weights in step.
 __global__ void build_bp_kernel(const unsigned long long *index, const uint32_t *val,
                                 const uint32_t *ids, const VertexRec *vrec, const LtRec *lrec,
                                 uint32_t n, uint64_t m, uint32_t *bp) {
@@ -77,12 +78,13 @@ __global__ void build_bp_kernel(const unsigned long long *index, const uint32_t 
   }
   const VertexRec r = vrec[lo];
   const uint64_t p = lrec[lo].boff + (e - r.begin);
-  bp[2 * p + 1] = ids[e];
+  uint32_t *blk = bp + 32 * (p >> 4);
+  blk[16 + (p & 15)] = ids[e];
   if (r.flags & kRowScalar) {
-    bp[2 * p] = kLtNever;
+    blk[p & 15] = kLtNever;
     return;
   }
-  bp[2 * p] = lt_breakpoint(val + r.begin, static_cast<uint32_t>(e) - r.begin);
+  blk[p & 15] = lt_breakpoint(val + r.begin, static_cast<uint32_t>(e) - r.begin);
 }
 
 // One thread per indexed LT row: the fanout-16 index over its breakpoints,
@@ -92,7 +94,9 @@ __global__ void build_bidx_kernel(const uint32_t *rows, uint32_t nrows, LtRec *l
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
   LtRec &r = lrec[rows[i]];
-  lt_index_build<2>(bp + 2ull * r.boff, r.d, bidx + r.ioff);
+  const uint32_t *rows_bp = bp + 2ull * r.boff;  // the row's blocks (boff is a multiple of 16)
+  lt_index_build([&](uint32_t j) { return rows_bp[32 * (j >> 4) + (j & 15)]; }, r.d,
+                 bidx + r.ioff);
   for (int t = 0; t < 16; ++t) r.top[t] = bidx[r.ioff + t];
 }
 

@@ -132,9 +132,8 @@ __host__ __device__ inline uint32_t lt_bp_cross(const uint32_t *B, uint32_t d, u
 // 0xFFFFFFFF (above every threshold), so a search reads one aligned 64-byte
 // node per level and a last block of at most 16 breakpoints.
 constexpr uint32_t kLtFan = 16;
-// Pairs after the last (breakpoint, in-neighbour) pair of the graph, so a
-// block search may read 36 words from the 16-byte aligned position at or
-// before a block's start.
+// Blocks of 16 breakpoints and their 16 in-neighbours after the last row
+// of the graph (a search of an empty row reads one).
 constexpr uint64_t kBpPad = 32;
 
 __host__ __device__ inline uint32_t lt_pad16(uint32_t x) { return (x + 15u) & ~15u; }
@@ -152,6 +151,10 @@ __host__ __device__ inline int lt_index_levels(uint32_t d, uint32_t *sz) {
   return h + 1;
 }
 
+__host__ __device__ inline uint32_t lt_bp_at(const uint32_t *B, uint32_t j) { return B[j]; }
+template <typename F>
+__host__ __device__ inline uint32_t lt_bp_at(const F &B, uint32_t j) { return B(j); }
+
 // Index entries (with padding) of a row of d edges.
 __host__ __device__ inline uint32_t lt_index_size(uint32_t d) {
   uint32_t sz[8];
@@ -161,11 +164,11 @@ __host__ __device__ inline uint32_t lt_index_size(uint32_t d) {
   return total;
 }
 
-// Writes the index of row breakpoints B[0], B[S], ... B[(d - 1) S] at idx
-// (lt_index_size(d) entries, top level first); S = 2 reads the breakpoints of
-// a (breakpoint, in-neighbour) pair array.
-template <uint32_t S = 1>
-__host__ __device__ inline void lt_index_build(const uint32_t *B, uint32_t d, uint32_t *idx) {
+// Writes the index of row breakpoints B(0) .. B(d - 1) at idx
+// (lt_index_size(d) entries, top level first); B is an array or a callable
+// that returns breakpoint j.
+template <typename Bp>
+__host__ __device__ inline void lt_index_build(const Bp &B, uint32_t d, uint32_t *idx) {
   uint32_t sz[8];
   const int nl = lt_index_levels(d, sz);
   if (nl == 0) return;
@@ -181,7 +184,7 @@ __host__ __device__ inline void lt_index_build(const uint32_t *B, uint32_t d, ui
       uint32_t v = 0xFFFFFFFFu;
       if (b < sz[l]) {
         const uint32_t last = (16u * b + 15u < below) ? 16u * b + 15u : below - 1u;
-        v = l == 0 ? B[static_cast<uint64_t>(last) * S] : idx[start[l - 1] + last];
+        v = l == 0 ? lt_bp_at(B, last) : idx[start[l - 1] + last];
       }
       idx[start[l] + b] = v;
     }

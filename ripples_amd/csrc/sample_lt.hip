@@ -90,10 +90,11 @@ __device__ inline uint32_t count_le(uint4 q, uint32_t k) {
 // boff): the first j with B[j] > k, or d.  The top level of the row's
 // fanout-16 index comes from v's record (the same line as its head); each
 // level below it is one aligned 64-byte node of the index array; the last
-// read is one aligned 128-byte block of 16 (breakpoint, in-neighbour) pairs,
-// which also gives *id, the in-neighbour of the crossing edge when there is
-// one.  *probes counts the nodes and blocks read (lt_index_cross of lt_scan.h
-// is the same search over plain arrays).
+// reads are the 16 breakpoints of one aligned 128-byte block and, in the same
+// line, *id, the in-neighbour of the crossing edge when there is
+// one (read from the block's line after the search).  *probes counts the
+// nodes and blocks read (lt_index_cross of lt_scan.h is the same search over
+// plain arrays).
 __device__ inline uint32_t lt_cross(const LtRec *lrec, uint32_t v, const uint32_t *bp,
                                     const uint32_t *bidx, uint32_t boff, uint32_t d,
                                     uint32_t ioff, uint32_t k, uint32_t *probes, uint32_t *id) {
@@ -128,24 +129,18 @@ __device__ inline uint32_t lt_cross(const LtRec *lrec, uint32_t v, const uint32_
       start += lt_pad16(sz);
     }
   }
-  // block `node`: pairs boff + 16 node .. + 15, one aligned line
+  // block `node` of the row: 16 breakpoints, then their 16 in-neighbours,
+  // in one aligned line; breakpoints rise along the row, so the crossing is
+  // the number of them <= k
   const uint32_t lo = 16u * node;
   const uint32_t cnt = d - lo < 16u ? d - lo : 16u;
-  const uint4 *q = reinterpret_cast<const uint4 *>(bp + 2ull * (boff + lo));
-  uint32_t c = 0, chosen = 0;
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const uint4 g = q[t];
-    // pairs 2t and 2t + 1: breakpoints rise along the block, so the crossing
-    // pair is the first whose breakpoint passes k
-    const uint32_t j0 = 2 * t, j1 = 2 * t + 1;
-    if (j0 < cnt && j0 == c && g.x <= k) ++c;
-    if (j0 == c) chosen = g.y;
-    if (j1 < cnt && j1 == c && g.z <= k) ++c;
-    if (j1 == c) chosen = g.w;
-  }
+  const uint32_t *blk = bp + 2ull * (boff + lo);
+  const uint4 *q = reinterpret_cast<const uint4 *>(blk);
+  const uint4 b0 = q[0], b1 = q[1], b2 = q[2], b3 = q[3];
+  const uint32_t c = count_le(b0, k) + count_le(b1, k) + count_le(b2, k) + count_le(b3, k);
   ++*probes;
-  if (c < cnt) *id = chosen;
+  // padding breakpoints (0xFFFFFFFF) are above every threshold, so c <= cnt
+  if (c < cnt) *id = blk[16 + c];
   return lo + c;
 }
 
