@@ -152,6 +152,7 @@ __host__ __device__ inline int lt_index_levels(uint32_t d, uint32_t *sz) {
 }
 
 __host__ __device__ inline uint32_t lt_bp_at(const uint32_t *B, uint32_t j) { return B[j]; }
+__host__ __device__ inline uint32_t lt_bp_at(uint32_t *B, uint32_t j) { return B[j]; }
 template <typename F>
 __host__ __device__ inline uint32_t lt_bp_at(const F &B, uint32_t j) { return B(j); }
 
