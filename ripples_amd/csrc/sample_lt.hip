@@ -191,6 +191,11 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   static_assert(UNROLL % 4 == 0, "groups of four weights per load");
   static_assert(CAP <= 16 && CAP + SPILL <= 128, "16-lane group sort and 128-key wave sort");
   __shared__ uint32_t s_list[CAP * BT];
+  // s_pow[r] = M^(2r): from the stream start of a batch's first sample id to
+  // that of the id r places later (element 0 of sample s is M^(2s+2)(seed))
+  __shared__ Affine s_pow[kClaim];
+  for (uint32_t r = threadIdx.x; r < kClaim; r += BT) s_pow[r] = aff_pow(master_step(), 2ull * r);
+  __syncthreads();
   uint32_t *list = s_list + threadIdx.x;  // element i < CAP at list[i * BT]
   uint32_t *spill = P.spill + (static_cast<uint64_t>(blockIdx.x) * BT + threadIdx.x) * SPILL;
   auto at = [&](uint32_t i) -> uint32_t & {
@@ -204,6 +209,8 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   // ids are handed out in increasing order, so a wave only stops once every
   // id it still holds is past `count`
   uint64_t claim_lo = 0, claim_hi = 0;
+  // the batch [claim_base, claim_hi) and the stream start of its first id
+  uint64_t claim_base = 0, z_base = 0;
   uint32_t len = 0, e = 0, end = 0, rflags = 0, kthr = 0, ridx = 0, vcur = 0;
   uint64_t filt0 = 0, filt1 = 0;  // visited filter, bits 0..63 and 64..127
   auto filter_add = [&](uint32_t v) {
@@ -243,7 +250,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
       const uint32_t need = __popcll(wm);
       const uint32_t rank = __popcll(wm & lanes_below());
       const uint64_t have = claim_hi - claim_lo;
-      uint64_t id;
+      uint64_t id, base = claim_base, zb = z_base;
       if (have >= need) {
         id = claim_lo + rank;
         claim_lo += need;
@@ -253,14 +260,21 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         unsigned long long nb = 0;
         if (lane == leader) nb = atomicAdd(&P.ctl[kCtlGroupCounter], kClaim);
         nb = __shfl(nb, static_cast<int>(leader), 64);
+        const uint64_t znb = sample_z(P.seed, P.first + nb);  // once per batch
         id = rank < have ? claim_lo + rank : nb + (rank - have);
+        if (rank >= have) {
+          base = nb;
+          zb = znb;
+        }
         claim_lo = nb + (need - have);
         claim_hi = nb + kClaim;
+        claim_base = nb;
+        z_base = znb;
       }
       if (want) {
         gid = id;
         if (gid < P.count) {
-          z = sample_z(P.seed, P.first + gid);
+          z = aff_apply(s_pow[gid - base], zb);  // = sample_z(P.seed, P.first + gid)
           const uint32_t root = uniform_index(z, P.n);
           list[0] = root;
           len = 1;
