@@ -188,6 +188,7 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count,
 // sample_ic.hip
 struct IcLaunch {
   uint32_t colors, block, slots, edge_run, heavy_degree, sampler;
+  bool colors_auto;  // colours not set by the caller
 };
 IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune);
 void ic_ensure_slots(rip_pool *p, uint32_t slots, uint32_t heavy_deg);
