@@ -369,8 +369,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat20_lt", choices=sorted(WORKLOADS))
-    ap.add_argument("--secondary", default="rmat22_wc",
-                    help="second workload reported beside the first ('' for none)")
+    ap.add_argument("--secondary", default="rmat22_wc,rmat16_ic",
+                    help="workloads reported beside the first, comma-separated ('' for none)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0, help="sets per step (0 = theta)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -435,13 +435,15 @@ def main():
         log("cpu baseline: %.0f sets/s on %d threads" % (out["cpu_baseline"]["value"],
                                                           out["cpu_baseline"]["cores"]))
     del index, edges, w
-    if a.secondary and a.secondary != a.workload:
-        sec = measure(a.secondary, a.seed, max(1, a.steps // 3), 1, 0, None, False,
-                      fused_steps=2)
+    others = [w for w in a.secondary.split(",") if w and w != a.workload]
+    for i, name in enumerate(others):
+        sec = measure(name, a.seed, max(1, a.steps // 3), 1, 0, None, False,
+                      fused_steps=2 if i == 0 else 0)
         sec.pop("graph")
-        out["secondary"] = {k: sec[k] for k in ("value", "ms_per_step", "config", "imm_wall_s",
-                                                "imm", "pool", "kernel_counters", "roofline",
-                                                "fused_colour_sampler") if k in sec}
+        out["secondary" if i == 0 else "secondary_" + name] = {
+            k: sec[k] for k in ("value", "ms_per_step", "config", "imm_wall_s", "imm", "pool",
+                                "kernel_counters", "roofline", "fused_colour_sampler")
+            if k in sec}
     if a.serve_scales:
         out["serving"] = serve_bench([int(x) for x in a.serve_scales.split(",")],
                                      a.serve_queries, a.seed, a.serve_slots, a.serve_workers,
