@@ -82,6 +82,8 @@ struct Args {
   uint8_t *covered;
   unsigned long long *keys;
   unsigned long long *ctl;
+  uint32_t *kill;      // dense pools: sets killed this round
+  uint64_t num_sets;
   uint32_t n;
 };
 
@@ -92,6 +94,8 @@ struct SelectScratch {
   unsigned long long *d_inv_start = nullptr;  // n + 1
   unsigned long long *d_cursor = nullptr;     // n
   uint32_t *d_inv = nullptr;           // total elements: set ids by vertex
+  uint32_t *d_kill = nullptr;          // cap_kill: dense pools' kill list
+  uint64_t cap_kill = 0;
   uint8_t *d_covered = nullptr;        // cap_sets
   unsigned long long *d_keys = nullptr;  // per-seed argmax keys (k)
   uint32_t cap_k = 0;
@@ -101,6 +105,7 @@ struct SelectScratch {
   struct Args *d_args = nullptr;
   hipGraphExec_t graph = nullptr;
   uint32_t graph_k = 0;
+  bool graph_dense = false;
 };
 
 }  // namespace rip
@@ -165,6 +170,8 @@ enum : int {
   kCtlStatLevels = 9,
   kCtlSortCount = 10,    // LT: finished lists longer than the LDS part, to sort
   kCtlBigCounter = 11,   // IC: work counter of the large-set kernel
+  kCtlKillSets = 12,     // selector, dense pools: sets killed this round
+  kCtlKillElems = 13,    // selector, dense pools: their elements
   kCtlWords = 16,
 };
 

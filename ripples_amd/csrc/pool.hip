@@ -171,6 +171,7 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->sel.d_inv_start);
   dev_free(p->sel.d_cursor);
   dev_free(p->sel.d_inv);
+  dev_free(p->sel.d_kill);
   dev_free(p->sel.d_covered);
   dev_free(p->sel.d_keys);
   dev_free(p->sel.d_args);

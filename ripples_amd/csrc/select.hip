@@ -25,6 +25,32 @@ namespace {
 
 constexpr uint32_t kBT = 256;
 
+// Each lane of a wave holds one run (off, len) of `data`; the wave visits
+// every element of the 64 runs once, lanes taking elements of any run:
+// f(value, run index).
+template <typename F>
+__device__ inline void for_each_run_element(unsigned long long off, uint32_t len,
+                                            const uint32_t *data, uint32_t *s_pre,
+                                            unsigned long long *s_off, F &&f) {
+  const uint32_t lane = lane_id();
+  s_off[lane] = off;
+  const uint32_t incl = wave_inclusive_scan(len);
+  s_pre[lane + 1] = incl;
+  if (lane == 0) s_pre[0] = 0;
+  const uint32_t total = __shfl(incl, 63, 64);
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t i = lane; i < total; i += 64) {
+    uint32_t lo = 0, hi = 64;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_pre[mid + 1] <= i) lo = mid + 1;
+      else hi = mid;
+    }
+    f(data[s_off[lo] + (i - s_pre[lo])], lo);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Each wave walks 64 consecutive sets; lanes take elements of any of them.
 template <typename F>
 __device__ inline void for_each_element(const unsigned long long *set_off,
@@ -38,22 +64,9 @@ __device__ inline void for_each_element(const unsigned long long *set_off,
        w * 64 < num_sets; w += nwaves) {
     const uint64_t s = w * 64 + lane;
     const uint32_t len = s < num_sets ? set_len[s] : 0u;
-    s_off[wid][lane] = s < num_sets ? set_off[s] : 0ull;
-    const uint32_t incl = wave_inclusive_scan(len);
-    s_pre[wid][lane + 1] = incl;
-    if (lane == 0) s_pre[wid][0] = 0;
-    const uint32_t total = __shfl(incl, 63, 64);
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t i = lane; i < total; i += 64) {
-      uint32_t lo = 0, hi = 64;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_pre[wid][mid + 1] <= i) lo = mid + 1;
-        else hi = mid;
-      }
-      f(data[s_off[wid][lo] + (i - s_pre[wid][lo])], w * 64 + lo);
-    }
-    __builtin_amdgcn_wave_barrier();
+    const unsigned long long off = s < num_sets ? set_off[s] : 0ull;
+    for_each_run_element(off, len, data, s_pre[wid], s_off[wid],
+                         [&](uint32_t v, uint32_t j) { f(v, w * 64 + j); });
   }
 }
 
@@ -77,6 +90,139 @@ fill_inverted_kernel(const unsigned long long *set_off, const uint32_t *set_len,
                      const unsigned long long p = atomicAdd(&cursor[v], 1ull);
                      inv[p] = static_cast<uint32_t>(s);
                    });
+}
+
+// ------------------------------------------------------------ dense pools
+// Pools whose sets are long compared with the number of vertex ranges below
+// build no inverted index.  Counts come from LDS histograms: block (set chunk
+// c, vertex range r) visits the sets of chunk c and, in each, only the run of
+// vertices inside range r, located by binary search (sets are sorted and
+// duplicate-free), then adds its histogram to the global counts with one
+// atomic per vertex.  A seed round finds the uncovered sets holding the seed
+// by binary search in each of them, and subtracts the killed sets' members the
+// same way (or with per-element atomics when the killed sets are small).
+constexpr uint32_t kDenseBT = 512;
+constexpr uint32_t kDenseRange = 16384;        // u32 counters: 64 KiB of LDS
+constexpr uint32_t kDenseLong = 256;           // runs this long are read by the whole wave
+constexpr uint32_t kDenseBlocks = 2048;        // blocks per histogram pass
+constexpr unsigned long long kDenseDecMin = 1ull << 21;  // killed elements for LDS decrements
+constexpr uint32_t kScanGrid = 1024;
+
+// First i in [0, len) with a[i] >= x (len if none).
+__device__ inline uint32_t lower_bound_u32(const uint32_t *a, uint32_t len, uint32_t x) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The block's (chunk, range).  Blocks are dealt to the 8 XCDs round-robin;
+// the remap gives each XCD a run of consecutive logical blocks, so the
+// ranges of one chunk share an L2.
+__device__ inline void dense_block(uint32_t ranges, uint32_t *c, uint32_t *r) {
+  uint32_t b = blockIdx.x;
+  const uint32_t g = gridDim.x;
+  if ((g & 7u) == 0) b = (b & 7u) * (g >> 3) + (b >> 3);
+  *c = b / ranges;
+  *r = b % ranges;
+}
+
+// Lane j of the wave holds set j of a group (its data at o, length L; live
+// false for no set).  f(v) for every member v of the 64 sets with
+// v_lo <= v < v_lo + width.  Runs of kDenseLong or more are read by the whole
+// wave with eight loads in flight per lane, the rest lane-mapped.
+template <typename F>
+__device__ inline void dense_group(unsigned long long o, uint32_t L, bool live,
+                                   const uint32_t *data, uint32_t v_lo, uint32_t width,
+                                   uint32_t *s_pre, unsigned long long *s_off, F &&f) {
+  unsigned long long off = 0;
+  uint32_t len = 0;
+  if (live) {
+    const uint32_t b = lower_bound_u32(data + o, L, v_lo);
+    // at most `width` distinct vertices fall in the range
+    const uint32_t span = L - b < width ? L - b : width;
+    off = o + b;
+    len = lower_bound_u32(data + off, span, v_lo + width);
+  }
+  const uint32_t lane = lane_id();
+  uint64_t longs = __ballot(len >= kDenseLong);
+  while (longs) {
+    const int j = __builtin_ctzll(longs);
+    longs &= longs - 1;
+    const uint32_t lo32 = __builtin_amdgcn_readlane(static_cast<uint32_t>(off), j);
+    const uint32_t hi32 = __builtin_amdgcn_readlane(static_cast<uint32_t>(off >> 32), j);
+    const uint32_t rl = __builtin_amdgcn_readlane(len, j);
+    const uint32_t *run = data + ((static_cast<unsigned long long>(hi32) << 32) | lo32);
+    for (uint32_t q = lane; q < rl; q += 64 * 8) {
+      uint32_t v[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) v[u] = q + 64 * u < rl ? run[q + 64 * u] : 0u;
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u)
+        if (q + 64 * u < rl) f(v[u]);
+    }
+  }
+  for_each_run_element(off, len >= kDenseLong ? 0u : len, data, s_pre, s_off,
+                       [&](uint32_t v, uint32_t) { f(v); });
+}
+
+// counts[v] += (kSub: -=) the number of sets holding v among sets [0, total)
+// (list == nullptr) or list[0, total).  Grid: chunks x ranges blocks.
+template <bool kSub>
+__device__ inline void dense_hist(const unsigned long long *set_off, const uint32_t *set_len,
+                                  const uint32_t *data, const uint32_t *list, uint64_t total,
+                                  uint32_t n, uint32_t ranges, uint32_t *counts) {
+  __shared__ uint32_t hist[kDenseRange];
+  __shared__ uint32_t s_pre[kDenseBT / 64][65];
+  __shared__ unsigned long long s_off[kDenseBT / 64][64];
+  const uint32_t chunks = gridDim.x / ranges;
+  uint32_t c, r;
+  dense_block(ranges, &c, &r);
+  const uint64_t per = (total + chunks - 1) / chunks;
+  const uint64_t sb = c * per;
+  const uint64_t se = sb + per < total ? sb + per : total;
+  if (sb >= se) return;  // block-uniform
+  const uint32_t v_lo = r * kDenseRange;
+  const uint32_t width = n - v_lo < kDenseRange ? n - v_lo : kDenseRange;
+  for (uint32_t i = threadIdx.x; i < width; i += kDenseBT) hist[i] = 0;
+  __syncthreads();
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  for (uint64_t base = sb + static_cast<uint64_t>(wid) * 64; base < se; base += kDenseBT) {
+    const uint64_t t = base + lane;
+    const bool live = t < se;
+    const uint64_t s = live ? (list ? list[t] : t) : 0;
+    const unsigned long long o = live ? set_off[s] : 0ull;
+    const uint32_t L = live ? set_len[s] : 0u;
+    // the range test also keeps an unsorted set from indexing past LDS
+    dense_group(o, L, live, data, v_lo, width, s_pre[wid], s_off[wid], [&](uint32_t v) {
+      if (v - v_lo < width) atomicAdd(&hist[v - v_lo], 1u);
+    });
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < width; i += kDenseBT) {
+    const uint32_t h = hist[i];
+    if (h == 0) continue;
+    if (kSub) atomicSub(&counts[v_lo + i], h);
+    else atomicAdd(&counts[v_lo + i], h);
+  }
+}
+
+__global__ void __launch_bounds__(kDenseBT)
+dense_count_kernel(const unsigned long long *set_off, const uint32_t *set_len,
+                   const uint32_t *data, uint64_t num_sets, uint32_t n, uint32_t ranges,
+                   uint32_t *counts) {
+  dense_hist<false>(set_off, set_len, data, nullptr, num_sets, n, ranges, counts);
+}
+
+// Seed round, dense pools, last step for large kills: subtract the members of
+// the killed sets through LDS histograms.
+__global__ void __launch_bounds__(kDenseBT) dense_dec_kernel(const Args *a, uint32_t ranges) {
+  if (a->ctl[kCtlKillElems] <= kDenseDecMin) return;
+  dense_hist<true>(a->set_off, a->set_len, a->data, a->kill, a->ctl[kCtlKillSets], a->n,
+                   ranges, a->counts);
 }
 
 __global__ void sum_len_kernel(const uint32_t *set_len, uint64_t num_sets,
@@ -140,6 +286,10 @@ __global__ void __launch_bounds__(kBT) argmax_kernel(const Args *a, uint32_t i) 
   if (threadIdx.x == 0) {
     for (uint32_t w = 1; w < kBT / 64; ++w) best = s_w[w] > best ? s_w[w] : best;
     if (best >> 32) atomicMax(a->keys + i, best);
+    if (blockIdx.x == 0) {  // the dense pools' kill list of this round
+      a->ctl[kCtlKillSets] = 0;
+      a->ctl[kCtlKillElems] = 0;
+    }
   }
 }
 
@@ -170,6 +320,72 @@ __global__ void __launch_bounds__(kBT) kill_kernel(const Args *a, uint32_t i) {
     const unsigned long long off = set_off[s];
     const uint32_t len = set_len[s];
     for (uint32_t q = lane; q < len; q += 64) atomicSub(&counts[data[off + q]], 1u);
+  }
+}
+
+// Seed round i, dense pools, first step: every uncovered set holding the
+// seed (binary search; sets are sorted) is marked covered and appended to the
+// kill list; the killed elements are summed.
+__global__ void __launch_bounds__(kBT) scan_kill_kernel(const Args *a, uint32_t i) {
+  const unsigned long long key = a->keys[i];
+  const uint32_t gain = static_cast<uint32_t>(key >> 32);
+  if (gain == 0) return;
+  const uint32_t seed = 0xFFFFFFFFu - static_cast<uint32_t>(key);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a->ctl[kCtlCovered], gain);
+  const uint64_t num_sets = a->num_sets;
+  const uint8_t *covered_in = a->covered;
+  uint8_t *covered = a->covered;
+  const unsigned long long *set_off = a->set_off;
+  const uint32_t *set_len = a->set_len;
+  const uint32_t *data = a->data;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBT;
+  const uint64_t lane = lane_id();
+  for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBT + threadIdx.x - lane; w < num_sets;
+       w += stride) {
+    const uint64_t s = w + lane;
+    bool hit = false;
+    uint32_t len = 0;
+    if (s < num_sets && !covered_in[s]) {
+      const unsigned long long off = set_off[s];
+      len = set_len[s];
+      const uint32_t pos = lower_bound_u32(data + off, len, seed);
+      hit = pos < len && data[off + pos] == seed;
+      if (hit) covered[s] = 1;
+    }
+    const uint64_t m = __ballot(hit);
+    if (m == 0) continue;  // wave-uniform
+    unsigned long long elems = hit ? len : 0u;
+    for (int d = 32; d > 0; d >>= 1) elems += __shfl_xor(elems, d, 64);
+    const uint32_t leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) {
+      base = atomicAdd(&a->ctl[kCtlKillSets], static_cast<unsigned long long>(__popcll(m)));
+      atomicAdd(&a->ctl[kCtlKillElems], elems);
+    }
+    base = __shfl(base, static_cast<int>(leader), 64);
+    if (hit) a->kill[base + __popcll(m & lanes_below())] = static_cast<uint32_t>(s);
+  }
+}
+
+// Seed round, dense pools, last step for small kills: one atomic per element.
+__global__ void __launch_bounds__(kBT) dec_small_kernel(const Args *a) {
+  __shared__ uint32_t s_pre[kBT / 64][65];
+  __shared__ unsigned long long s_off[kBT / 64][64];
+  if (a->ctl[kCtlKillElems] > kDenseDecMin) return;
+  const uint64_t cnt = a->ctl[kCtlKillSets];
+  const uint32_t *kill = a->kill;
+  const unsigned long long *set_off = a->set_off;
+  const uint32_t *set_len = a->set_len;
+  uint32_t *counts = a->counts;
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBT / 64);
+  for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * (kBT / 64) + wid; w * 64 < cnt;
+       w += nwaves) {
+    const uint64_t t = w * 64 + lane;
+    const uint32_t s = t < cnt ? kill[t] : 0u;
+    for_each_run_element(t < cnt ? set_off[s] : 0ull, t < cnt ? set_len[s] : 0u, a->data,
+                         s_pre[wid], s_off[wid],
+                         [&](uint32_t v, uint32_t) { atomicSub(&counts[v], 1u); });
   }
 }
 
@@ -219,8 +435,10 @@ __global__ void flag_seeds_kernel(const uint32_t *seeds, uint32_t k, uint32_t *f
     flags[seeds[i]] = 1;
 }
 
-void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t total,
-                           uint32_t k) {
+// inv_elems: elements of the inverted index (0: none); kill_sets: length of
+// the dense pools' kill list (0: none).
+void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t inv_elems,
+                           uint64_t kill_sets, uint32_t k) {
   SelectScratch &s = p->sel;
   const uint32_t n = p->g->n;
   if (s.n != n) {
@@ -237,10 +455,15 @@ void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t total,
     s.cap_sets = std::max<uint64_t>(num_sets, s.cap_sets + s.cap_sets / 2);
     s.d_covered = dev_alloc<uint8_t>(s.cap_sets);
   }
-  if (s.cap_elems < total) {
+  if (s.cap_elems < inv_elems) {
     dev_free(s.d_inv);
-    s.cap_elems = std::max<uint64_t>(total, s.cap_elems + s.cap_elems / 2);
+    s.cap_elems = std::max<uint64_t>(inv_elems, s.cap_elems + s.cap_elems / 2);
     s.d_inv = dev_alloc<uint32_t>(s.cap_elems);
+  }
+  if (s.cap_kill < kill_sets) {
+    dev_free(s.d_kill);
+    s.cap_kill = std::max<uint64_t>(kill_sets, s.cap_kill + s.cap_kill / 2);
+    s.d_kill = dev_alloc<uint32_t>(s.cap_kill);
   }
   if (s.cap_k < k) {
     dev_free(s.d_keys);
@@ -255,8 +478,29 @@ uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap) {
   return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
 }
 
-// counts over sets [0, num_sets); returns the element total
-uint64_t build_counts(rip_pool *p, uint64_t num_sets, uint32_t k) {
+constexpr uint64_t kDenseMinElements = 1ull << 22;
+
+uint32_t ranges_of(uint32_t n) { return (n + kDenseRange - 1) / kDenseRange; }
+
+// Dense pools: at least 4M elements and a mean set length of at least 64
+// per vertex range, since the range searches cost O(num_sets x ranges)
+// loads where the per-element kernels make one global atomic per element.
+bool is_dense(uint32_t n, uint64_t num_sets, uint64_t total) {
+  if (num_sets == 0 || n == 0 || total < kDenseMinElements) return false;
+  return total / num_sets >= 64ull * ranges_of(n);
+}
+
+// Blocks per vertex range of a histogram pass over `sets` sets.
+uint32_t dense_chunks(uint32_t ranges, uint64_t sets) {
+  const uint64_t want = (kDenseBlocks + ranges - 1) / ranges;
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, sets / kDenseBT)));
+}
+
+enum class Need { Counts, Select };
+
+// counts over sets [0, num_sets) and the scratch a selection needs; returns
+// whether the pool takes the dense path
+bool build_counts(rip_pool *p, uint64_t num_sets, uint32_t k, Need need) {
   RIP_REQUIRE(num_sets <= p->num_sets, -1, "select: num_sets beyond the pool");
   const uint32_t n = p->g->n;
   hipStream_t st = p->stream;
@@ -269,22 +513,33 @@ uint64_t build_counts(rip_pool *p, uint64_t num_sets, uint32_t k) {
   RIP_HIP(hipMemcpyAsync(&total, p->d_ctl + kCtlWords - 1, sizeof(total),
                          hipMemcpyDeviceToHost, st));
   RIP_HIP(hipStreamSynchronize(st));
-  ensure_select_scratch(p, num_sets, total, k);
+  const bool dense = is_dense(n, num_sets, total);
+  const bool sel = need == Need::Select;
+  ensure_select_scratch(p, sel ? num_sets : 0, sel && !dense ? total : 0,
+                        sel && dense ? num_sets : 0, k);
   SelectScratch &s = p->sel;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, kBT, 2048)), dim3(kBT), 0, st,
                      s.d_counts, static_cast<uint64_t>(n), 0u);
-  if (num_sets)
+  if (dense) {
+    const uint32_t ranges = ranges_of(n);
+    hipLaunchKernelGGL(dense_count_kernel, dim3(dense_chunks(ranges, num_sets) * ranges),
+                       dim3(kDenseBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets, n,
+                       ranges, s.d_counts);
+  } else if (num_sets) {
     hipLaunchKernelGGL(count_kernel, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
                        dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets,
                        s.d_counts);
+  }
   RIP_HIP(hipGetLastError());
-  return total;
+  return dense;
 }
 
-// Captures clear + k x (argmax, kill) on the pool's stream as an executable
-// graph.  Only kernels are recorded (no memset nodes); the capture mode is
-// thread-local so other host threads' streams are unaffected.
-void capture_rounds(rip_pool *p, uint32_t k) {
+// Captures clear + k seed rounds on the pool's stream as an executable graph:
+// (argmax, kill) per round, or for dense pools (argmax, scan, two decrement
+// kernels of which one does the work).  Only kernels are recorded (no memset
+// nodes); the capture mode is thread-local so other host threads' streams
+// are unaffected.
+void capture_rounds(rip_pool *p, uint32_t k, bool dense) {
   SelectScratch &s = p->sel;
   if (s.graph) {
     (void)hipGraphExecDestroy(s.graph);
@@ -292,12 +547,22 @@ void capture_rounds(rip_pool *p, uint32_t k) {
     s.graph_k = 0;
   }
   hipStream_t st = p->stream;
-  const uint32_t argmax_grid = grid_for(p->g->n, kBT * 8, 1024);
+  const uint32_t n = p->g->n;
+  const uint32_t argmax_grid = grid_for(n, kBT * 8, 1024);
+  const uint32_t ranges = ranges_of(n);
+  const uint32_t dec_grid = ranges * dense_chunks(ranges, ~0ull);
   RIP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(clear_keys_kernel, dim3(1), dim3(256), 0, st, s.d_args, k);
   for (uint32_t i = 0; i < k; ++i) {
     hipLaunchKernelGGL(argmax_kernel, dim3(argmax_grid), dim3(kBT), 0, st, s.d_args, i);
-    hipLaunchKernelGGL(kill_kernel, dim3(256), dim3(kBT), 0, st, s.d_args, i);
+    if (dense) {
+      hipLaunchKernelGGL(scan_kill_kernel, dim3(kScanGrid), dim3(kBT), 0, st, s.d_args, i);
+      hipLaunchKernelGGL(dec_small_kernel, dim3(256), dim3(kBT), 0, st, s.d_args);
+      hipLaunchKernelGGL(dense_dec_kernel, dim3(dec_grid), dim3(kDenseBT), 0, st, s.d_args,
+                         ranges);
+    } else {
+      hipLaunchKernelGGL(kill_kernel, dim3(256), dim3(kBT), 0, st, s.d_args, i);
+    }
   }
   const hipError_t launch = hipGetLastError();
   hipGraph_t graph = nullptr;
@@ -310,6 +575,7 @@ void capture_rounds(rip_pool *p, uint32_t k) {
   RIP_HIP(inst);
   s.graph = exec;
   s.graph_k = k;
+  s.graph_dense = dense;
 }
 
 }  // namespace
@@ -318,24 +584,26 @@ uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
                     uint64_t *covered) {
   const uint32_t n = p->g->n;
   hipStream_t st = p->stream;
-  build_counts(p, num_sets, std::max<uint32_t>(k, 1));
+  const bool dense = build_counts(p, num_sets, std::max<uint32_t>(k, 1), Need::Select);
   SelectScratch &s = p->sel;
-  exclusive_scan_u32(s.d_counts, s.d_inv_start, n, st);
-  hipLaunchKernelGGL(copy_u64_kernel, dim3(grid_for(n, kBT, 2048)), dim3(kBT), 0, st,
-                     s.d_cursor, s.d_inv_start, static_cast<uint64_t>(n));
-  if (num_sets)
-    hipLaunchKernelGGL(fill_inverted_kernel,
-                       dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)), dim3(kBT), 0, st,
-                       p->d_off, p->d_len, p->d_data, num_sets, s.d_cursor, s.d_inv);
+  if (!dense) {
+    exclusive_scan_u32(s.d_counts, s.d_inv_start, n, st);
+    hipLaunchKernelGGL(copy_u64_kernel, dim3(grid_for(n, kBT, 2048)), dim3(kBT), 0, st,
+                       s.d_cursor, s.d_inv_start, static_cast<uint64_t>(n));
+    if (num_sets)
+      hipLaunchKernelGGL(fill_inverted_kernel,
+                         dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)), dim3(kBT), 0, st,
+                         p->d_off, p->d_len, p->d_data, num_sets, s.d_cursor, s.d_inv);
+  }
   hipLaunchKernelGGL(fill_u8_kernel, dim3(grid_for(num_sets, kBT, 2048)), dim3(kBT), 0, st,
                      s.d_covered, num_sets);
   // the seed rounds run as one graph; this call's buffers travel in d_args
   // (`args` stays alive until the stream is synchronised below)
   const Args args{s.d_counts, s.d_inv_start, s.d_inv, p->d_off, p->d_len, p->d_data,
-                  s.d_covered, s.d_keys, p->d_ctl, n};
+                  s.d_covered, s.d_keys, p->d_ctl, s.d_kill, num_sets, n};
   RIP_HIP(hipMemcpyAsync(s.d_args, &args, sizeof(Args), hipMemcpyHostToDevice, st));
   if (k) {
-    if (!s.graph || s.graph_k != k) capture_rounds(p, k);
+    if (!s.graph || s.graph_k != k || s.graph_dense != dense) capture_rounds(p, k, dense);
     RIP_HIP(hipGraphLaunch(s.graph, st));
   } else {
     hipLaunchKernelGGL(clear_keys_kernel, dim3(1), dim3(256), 0, st, s.d_args, 0u);
@@ -366,7 +634,7 @@ uint64_t coverage_run(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
     RIP_REQUIRE(seeds[i] < n, -1, "coverage: seed vertex out of range");
   hipStream_t st = p->stream;
   RIP_HIP(hipSetDevice(p->g->device));
-  ensure_select_scratch(p, 0, 0, std::max<uint32_t>(num_seeds, 1));
+  ensure_select_scratch(p, 0, 0, 0, std::max<uint32_t>(num_seeds, 1));
   SelectScratch &s = p->sel;
   // the seeds travel through the key buffer (k >= num_seeds words of 8 bytes)
   if (num_seeds)
@@ -391,7 +659,7 @@ uint64_t coverage_run(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
 }
 
 void count_run(rip_pool *p, uint64_t num_sets, uint32_t *counts_host) {
-  build_counts(p, num_sets, 1);
+  build_counts(p, num_sets, 1, Need::Counts);
   RIP_HIP(hipMemcpyAsync(counts_host, p->sel.d_counts, sizeof(uint32_t) * p->g->n,
                          hipMemcpyDeviceToHost, p->stream));
   RIP_HIP(hipStreamSynchronize(p->stream));

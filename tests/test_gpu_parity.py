@@ -88,6 +88,71 @@ def test_ic_bimodal_sets_match_oracle(sampler, orc):
     g.close()
 
 
+def dense_pool_sets(n, count, seed):
+    """Sorted, duplicate-free sets over [0, n): mostly a few hundred to a few
+    thousand vertices, some empty or single, 1% the whole vertex set, and a
+    few pinned to both sides of the 8192/16384 vertex range boundaries."""
+    rng = np.random.default_rng(seed)
+    pinned = [[8191, 8192], [16383, 16384],
+              [b + o for b in range(8192, n, 8192) for o in (-1, 0)]]
+    sets = [[], [0], [n - 1], list(range(n))] + [[v for v in x if v < n] for x in pinned]
+    while len(sets) < count:
+        u = rng.random()
+        if u < 0.2:
+            size = int(rng.integers(0, 4))
+        elif u < 0.99:
+            size = int(rng.integers(200, 3000))
+        else:
+            size = n
+        sets.append(np.unique(rng.integers(0, n, size)).tolist() if size < n
+                    else list(range(n)))
+    off = np.zeros(len(sets) + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in sets])
+    ver = np.fromiter((v for x in sets for v in x), np.uint32, int(off[-1]))
+    return off, ver
+
+
+@pytest.mark.parametrize("n", [5000, 40000, 70001])
+def test_dense_selection_matches_oracle(n, orc):
+    """Pools of long sets (4M+ elements, mean length at least 64 per range of
+    16K vertices) count and index per vertex range in LDS; shorter prefixes
+    take the per-element kernels.  Counts, seeds and coverage equal the
+    oracle's on every prefix."""
+    off, ver = dense_pool_sets(n, 5000, n)
+    assert off[-1] >= 4 << 20
+    g = R.Graph(np.zeros(n + 1, np.uint64), np.zeros(0, np.uint32), np.zeros(0, np.float32))
+    p = g.pool(0)
+    p.append(off[:2001], ver[:int(off[2000])])
+    p.append(off[2000:] - off[2000], ver[int(off[2000]):])
+    assert np.array_equal(p.count_occurrences(), orc.count(n, off, ver))
+    for num in (5000, 3001, 700, 9):
+        for k in (1, 40):
+            seeds, cov = p.find_most_influential_set(k, num)
+            es, ec = orc.select(n, off[:num + 1], ver[:int(off[num])], k)
+            assert seeds.tolist() == es.tolist() and cov == ec, (num, k)
+    p.close()
+    g.close()
+
+
+def test_dense_selection_on_ic_pool(orc):
+    """R-MAT scale 16, U[0,0.1]: a fifth of the sets reach the ~12K-vertex
+    giant component, so the 6000-set pool takes the dense kernels."""
+    index, edges, w = rmat_csr(16, R.W_UNIFORM, 0.1, 3)
+    n = len(index) - 1
+    g = R.Graph(index, edges, w, model=R.IC)
+    p = g.pool(9)
+    p.generate_rrr_sets(6000)
+    off, ver = p.export()
+    assert off[-1] >= 4 << 20
+    assert np.array_equal(off, orc.sample(R.IC, index, edges, w, 9, 0, 6000)[0])
+    for num, k in ((6000, 100), (4500, 7)):
+        seeds, cov = p.find_most_influential_set(k, num)
+        es, ec = orc.select(n, off[:num + 1], ver[:int(off[num])], k)
+        assert seeds.tolist() == es.tolist() and cov == ec, (num, k)
+    p.close()
+    g.close()
+
+
 def test_selector_graph_replays_across_pool_growth(orc):
     """The seed rounds are one captured graph per (pool, k): replayed after the
     pool's buffers grew and moved, and on different prefixes, each replay
