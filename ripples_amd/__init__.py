@@ -41,7 +41,7 @@ class Tune(C.Structure):
     _fields_ = [("colors", C.c_uint32), ("block", C.c_uint32),
                 ("slots", C.c_uint32), ("edge_run", C.c_uint32),
                 ("heavy_degree", C.c_uint32), ("sampler", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("out_chunk", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class LoadOptions(C.Structure):

@@ -138,6 +138,7 @@ int rip_pool_reset(rip_pool *p) {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
     p->num_sets = 0;
     p->used_data = 0;
+    p->num_elements = 0;
   });
 }
 
@@ -145,7 +146,7 @@ int rip_pool_size(const rip_pool *p, uint64_t *num_sets, uint64_t *num_elements)
   return guarded([&] {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
     if (num_sets) *num_sets = p->num_sets;
-    if (num_elements) *num_elements = p->used_data;
+    if (num_elements) *num_elements = p->num_elements;
   });
 }
 

@@ -106,7 +106,7 @@ void imm_run(rip_graph *g, uint32_t k, double eps, uint64_t seed,
   r.select_ms = ms_since(ts);
   r.theta = theta;
   r.num_sets = pool.num_sets;
-  r.rrr_elements = pool.used_data;
+  r.rrr_elements = pool.num_elements;
   r.num_seeds = got;
   r.lb = lb;
   r.final_f = theta ? static_cast<double>(cov) / static_cast<double>(theta) : 0.0;

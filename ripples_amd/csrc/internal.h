@@ -144,7 +144,9 @@ struct rip_pool {
   uint32_t *d_len = nullptr;            // per-sample size
   uint32_t *d_data = nullptr;
   uint64_t cap_data = 0;
-  uint64_t used_data = 0;
+  uint64_t used_data = 0;     // next free element of d_data
+  uint64_t num_elements = 0;  // sum of set sizes: used_data less the gaps the
+                              // LT sampler leaves after its waves' output chunks
   unsigned long long *d_ctl = nullptr;  // control words, see pool.hip
   rip::SelectScratch sel;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -172,6 +174,7 @@ enum : int {
   kCtlBigCounter = 11,   // IC: work counter of the large-set kernel
   kCtlKillSets = 12,     // selector, dense pools: sets killed this round
   kCtlKillElems = 13,    // selector, dense pools: their elements
+  kCtlElems = 14,        // elements of the sets of the last sampling launch
   kCtlWords = 16,
 };
 
@@ -215,6 +218,9 @@ void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count);
 // sample_lt.hip
 uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
                    const rip_tune *tune);
+// Most pool elements an LT launch of `count` samples can leave unused (the
+// rest of each wave's last output chunk).
+uint64_t lt_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune);
 
 // select.hip
 uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k,

@@ -147,7 +147,7 @@ void opimc_run(rip_graph *g, uint32_t k, double epsilon, double delta_cfg, uint6
     }
     theta *= 2;
   }
-  r.rrr_elements = P.r1.used_data + P.r2.used_data;
+  r.rrr_elements = P.r1.num_elements + P.r2.num_elements;
   r.total_ms = ms_since(t_total);
   if (rec) *rec = r;
 }

@@ -89,7 +89,18 @@ __global__ void set_ctl_kernel(unsigned long long *ctl, unsigned long long bump)
   ctl[kCtlOverflow] = 0;
   ctl[kCtlGroupCounter] = 0;
   ctl[kCtlLongCount] = 0;
+  ctl[kCtlElems] = 0;
   for (int i = kCtlStatBytes; i <= kCtlStatLevels; ++i) ctl[i] = 0;
+}
+
+// *out += sum of len[0, count)
+__global__ void sum_sizes_kernel(const uint32_t *len, uint64_t count, unsigned long long *out) {
+  unsigned long long acc = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    acc += len[i];
+  for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d, 64);
+  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
 }
 
 // Copy sets [first, first+count) into CSR order at out (prefix from scan).
@@ -225,7 +236,8 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
   RIP_REQUIRE(p->num_sets + count < (1ull << 32), -1,
               "pool: at most 2^32-1 samples per stream");
   pool_reserve_sets(p, p->num_sets + count);
-  const double est = static_cast<double>(count) * p->avg_set_size * 1.25 + 65536.0;
+  double est = static_cast<double>(count) * p->avg_set_size * 1.25 + 65536.0;
+  if (p->g->model == RIP_LT) est += static_cast<double>(lt_output_slack(p, count, tune));
   if (p->used_data + static_cast<uint64_t>(est) > p->cap_data)
     pool_reserve_data(p, std::max<uint64_t>(p->used_data + static_cast<uint64_t>(est),
                                             p->cap_data + p->cap_data / 2));
@@ -243,6 +255,12 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
       launches = ic_sample(p, first, count, ic);
     else
       launches = lt_launch(p, first, count, tune);
+    {
+      const uint64_t blocks = std::min<uint64_t>((count + 255) / 256, 1024);
+      hipLaunchKernelGGL(sum_sizes_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+                         p->stream, p->d_len + first, count, p->d_ctl + kCtlElems);
+      RIP_HIP(hipGetLastError());
+    }
     unsigned long long ctl[kCtlWords];
     RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, p->stream));
     RIP_HIP(hipStreamSynchronize(p->stream));
@@ -254,6 +272,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
       for (int i = 0; i < RIP_STAT_COUNT; ++i) p->last_stats[i] = ctl[kCtlStatBytes + i];
       const uint64_t added = ctl[kCtlDataBump] - p->used_data;
       p->used_data = ctl[kCtlDataBump];
+      p->num_elements += ctl[kCtlElems];
       p->avg_set_size = std::max(1.0, static_cast<double>(added) / static_cast<double>(count));
       p->num_sets += count;
       return;
@@ -299,6 +318,7 @@ void pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
                     hipMemcpyHostToDevice));
   p->num_sets += count;
   p->used_data += total;
+  p->num_elements += total;
 }
 
 void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,

@@ -52,10 +52,13 @@ struct LtParams {
   unsigned long long *ctl;
   uint32_t *long_ids;
   uint32_t *spill;
+  uint32_t chunk;  // pool elements a wave reserves at a time
 };
 
-// Sample ids a wave claims from the work counter at a time.
+// Sample ids a wave claims from the work counter at a time (at least a
+// wave's worth: one refill hands out up to 64 ids).
 constexpr unsigned long long kClaim = 256;
+static_assert(kClaim >= 64, "a refill takes at most one new batch");
 
 // LT threshold of the walk at a vertex (ja, jb from its record): U = k * 2^-24.
 __device__ inline uint32_t lt_threshold(uint64_t ja, uint64_t jb, uint64_t z) {
@@ -213,6 +216,10 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   uint64_t claim_base = 0, z_base = 0;
   uint32_t len = 0, e = 0, end = 0, rflags = 0, kthr = 0, ridx = 0, vcur = 0;
   uint64_t filt0 = 0, filt1 = 0;  // visited filter, bits 0..63 and 64..127
+  // the wave's reserved output [out_cur, out_end) (wave-uniform): finished
+  // walks are written there, and a new chunk is reserved only when they do
+  // not fit, so waves rarely meet on the pool's one allocation counter
+  unsigned long long out_cur = 0, out_end = 0;
   auto filter_add = [&](uint32_t v) {
     const uint32_t b = lt_filter_bit(v);
     if (b < 64)
@@ -363,9 +370,19 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
       const uint32_t incl = wave_inclusive_scan(size);
       const uint32_t total = __shfl(incl, 63, 64);
       unsigned long long base = 0;
-      if (lane == 0 && total)
-        base = atomicAdd(&P.ctl[kCtlDataBump], static_cast<unsigned long long>(total));
-      base = __shfl(base, 0, 64);
+      if (total) {
+        if (out_end - out_cur < total) {
+          // the rest of the old chunk stays a gap in the pool's data
+          const unsigned long long want = total > P.chunk ? total : P.chunk;
+          unsigned long long nb = 0;
+          if (lane == 0) nb = atomicAdd(&P.ctl[kCtlDataBump], want);
+          nb = __shfl(nb, 0, 64);
+          out_cur = nb;
+          out_end = nb + want;
+        }
+        base = out_cur;
+        out_cur += total;
+      }
       const bool fail = total && base + total > P.cap_data;
       if (fail && lane == 0) atomicOr(&P.ctl[kCtlOverflow], 1ull);
       const unsigned long long off = base + incl - size;
@@ -565,10 +582,30 @@ __global__ void reset_long_kernel(unsigned long long *ctl) {
 
 }  // namespace
 
+constexpr int kLtBT = 256, kLtCap = 16, kLtSpill = 112, kLtUnroll = 16;
+
+// Output elements a wave reserves at a time: about a quarter of what one
+// busy wave writes (so a wave reserves a few times and leaves at most one
+// partial chunk unused), a power of two in [64, 2048].
+uint32_t lt_chunk(const rip_pool *p, uint64_t count, const rip_tune *tune, uint32_t blocks) {
+  if (tune && tune->out_chunk) return tune->out_chunk;
+  const uint64_t waves = std::max<uint64_t>(
+      1, std::min<uint64_t>(static_cast<uint64_t>(blocks) * (kLtBT / 64),
+                            (count + kClaim - 1) / kClaim));
+  const double quarter = static_cast<double>(count) * p->avg_set_size / static_cast<double>(waves) / 4.0;
+  uint32_t c = 64;
+  while (c < 2048 && 2.0 * c <= quarter) c *= 2;
+  return c;
+}
+
+uint64_t lt_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune) {
+  const uint32_t blocks = lt_grid<kLtBT, kLtCap, kLtSpill, kLtUnroll>(count);
+  return static_cast<uint64_t>(blocks) * (kLtBT / 64) * lt_chunk(p, count, tune, blocks);
+}
+
 uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
                    const rip_tune *tune) {
-  (void)tune;
-  constexpr int BT = 256, CAP = 16, SPILL = 112, UNROLL = 16;
+  constexpr int BT = kLtBT, CAP = kLtCap, SPILL = kLtSpill, UNROLL = kLtUnroll;
   const rip_graph *g = p->g;
   LtParams P;
   P.lrec = g->d_lrec;
@@ -594,6 +631,7 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   P.long_ids = p->g->lt.d_long_ids;
   hipLaunchKernelGGL(reset_long_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
   const uint32_t blocks = lt_grid<BT, CAP, SPILL, UNROLL>(count);
+  P.chunk = lt_chunk(p, count, tune, blocks);
   const uint64_t spill_words = static_cast<uint64_t>(blocks) * BT * SPILL;
   if (p->g->lt.cap_spill < spill_words) {
     dev_free(p->g->lt.d_spill);

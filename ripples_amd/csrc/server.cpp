@@ -256,7 +256,7 @@ void step_query(rip_pool *pool, uint64_t n, Query *q) {
   rec.select_ms = ms_between(t0, clk::now());
   rec.theta = q->need;
   rec.num_sets = q->drawn;
-  rec.rrr_elements = pool->used_data;  // elements of the shared pool
+  rec.rrr_elements = pool->num_elements;  // elements of the shared pool
   rec.num_seeds = got;
   rec.lb = q->lb;
   rec.final_f = q->need ? static_cast<double>(cov) / static_cast<double>(q->need) : 0.0;
@@ -308,6 +308,7 @@ void worker_loop(rip_server *s) {
         e->pool.seed = e->seed;
         e->pool.num_sets = 0;
         e->pool.used_data = 0;
+        e->pool.num_elements = 0;
       }
       e->reseed = false;
       // one shared launch extends the pool to the largest prefix waited for

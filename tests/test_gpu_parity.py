@@ -335,6 +335,34 @@ def test_imm_matches_oracle(model, k, eps, orc):
     assert rec["ThetaPrime"] == list(erec.theta_prime[:erec.rounds])
     assert rec["F"] == list(erec.f[:erec.rounds])
     assert rec["FinalF"] == erec.final_f
+    # elements of the pool's sets (the LT pool's data also holds unused gaps)
+    pool_off = orc.sample(model, index, edges, w, 0, 0, rec["NumSets"])[0]
+    assert rec["RRRElements"] == int(pool_off[-1])
+    g.close()
+
+
+@pytest.mark.parametrize("chunk", [1, 7, 100, 4096])
+def test_lt_output_chunks_do_not_change_results(chunk, orc):
+    """LT waves reserve pool space a chunk at a time and leave the rest of
+    their last chunk unused: sets, element count and seeds are the same for
+    every chunk size, across appended calls and a host append."""
+    index, edges, w = rmat_csr(12, R.W_LT_NORM, 0.9)
+    n = len(index) - 1
+    g = R.Graph(index, edges, w, model=R.LT)
+    p = g.pool(5)
+    p.generate_rrr_sets(3000, tune={"out_chunk": chunk})
+    p.generate_rrr_sets(1500, tune={"out_chunk": chunk})
+    exp = orc.sample(R.LT, index, edges, w, 5, 0, 4500)
+    assert_same_sets(p.export(), exp)
+    assert p.size() == (4500, int(exp[0][-1]))
+    p.append(np.array([0, 2, 3], np.uint64), np.array([1, 7, 4], np.uint32))
+    off, ver = p.export()
+    assert p.size() == (4502, int(exp[0][-1]) + 3)
+    assert ver[-3:].tolist() == [1, 7, 4]
+    seeds, cov = p.find_most_influential_set(20)
+    es, ec = orc.select(n, off, ver, 20)
+    assert seeds.tolist() == es.tolist() and cov == ec
+    p.close()
     g.close()
 
 
