@@ -52,8 +52,9 @@ typedef struct {
   uint32_t edge_run;     /* IC: consecutive in-edges per lane: 4, 8 or 16 */
   uint32_t heavy_degree; /* IC: rows this long wait for the hub phase */
   uint32_t sampler;      /* IC: RIP_SAMPLER_AUTO, _WAVE or _COLOUR */
-  uint32_t out_chunk;    /* LT: pool elements a wave reserves at a time (the
-                            pool's data then holds gaps between chunks) */
+  uint32_t out_chunk;    /* LT and IC wave sampler: pool elements a wave
+                            reserves at a time (the pool's data then holds
+                            gaps between chunks) */
   uint32_t reserved;     /* must be zero */
 } rip_tune;
 

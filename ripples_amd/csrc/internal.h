@@ -198,8 +198,20 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count,
 // sample_ic.hip
 struct IcLaunch {
   uint32_t colors, block, slots, edge_run, heavy_degree, sampler;
-  bool colors_auto;  // colours not set by the caller
+  bool colors_auto;    // colours not set by the caller
+  uint32_t out_chunk;  // wave sampler's output chunk, 0 = automatic
 };
+
+// Pool elements a sampler wave reserves at a time: a power of two in
+// [64, max_chunk], at most half of what one busy wave writes, so a wave
+// reserves a few times and leaves at most one partial chunk unused.  Waves
+// that took one global atomic per finished set or walk spent a large part of
+// their time waiting on that one address.
+inline uint32_t out_chunk_rule(double elems_per_wave, uint32_t max_chunk) {
+  uint32_t c = 64;
+  while (c < max_chunk && 4.0 * c <= elems_per_wave) c *= 2;
+  return c;
+}
 IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune);
 void ic_ensure_slots(rip_pool *p, uint32_t slots, uint32_t heavy_deg);
 // Colour sampler over samples first + i, i in [0, count), or first + ids[i].
@@ -212,7 +224,10 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
 
 // sample_ic_wave.hip: returns how many samples were left for the colour
 // sampler; their indices (relative to first) are in g->wave.d_big
-uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count);
+uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, uint32_t out_chunk);
+// Most pool elements a wave-sampler launch of `count` samples can leave
+// unused (the rest of each wave's last output chunk).
+uint64_t ic_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune);
 void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count);
 
 // sample_lt.hip

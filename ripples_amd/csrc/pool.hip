@@ -237,7 +237,8 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
               "pool: at most 2^32-1 samples per stream");
   pool_reserve_sets(p, p->num_sets + count);
   double est = static_cast<double>(count) * p->avg_set_size * 1.25 + 65536.0;
-  if (p->g->model == RIP_LT) est += static_cast<double>(lt_output_slack(p, count, tune));
+  est += static_cast<double>(p->g->model == RIP_LT ? lt_output_slack(p, count, tune)
+                                                  : ic_output_slack(p, count, tune));
   if (p->used_data + static_cast<uint64_t>(est) > p->cap_data)
     pool_reserve_data(p, std::max<uint64_t>(p->used_data + static_cast<uint64_t>(est),
                                             p->cap_data + p->cap_data / 2));

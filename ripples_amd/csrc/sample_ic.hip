@@ -441,7 +441,7 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
   // defaults from a sweep on R-MAT scale 22 weighted cascade: 8-edge runs and
   // 256-thread groups (more independent groups per CU); hub deferral off,
   // because colours of one group rarely meet at the same hub there
-  IcLaunch c{64, 256, 0, 8, 0xFFFFFFFFu, RIP_SAMPLER_AUTO, true};
+  IcLaunch c{64, 256, 0, 8, 0xFFFFFFFFu, RIP_SAMPLER_AUTO, true, 0};
   if (tune) {
     if (tune->colors) {
       c.colors = tune->colors;
@@ -452,6 +452,7 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
     if (tune->edge_run) c.edge_run = tune->edge_run;
     if (tune->heavy_degree) c.heavy_degree = tune->heavy_degree;
     c.sampler = tune->sampler;
+    c.out_chunk = tune->out_chunk;
   }
   RIP_REQUIRE(c.colors >= 1 && c.colors <= 64, -1, "tune: colors must be in [1, 64]");
   RIP_REQUIRE(c.block == 256 || c.block == 512 || c.block == 1024, -1,
@@ -561,7 +562,7 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
 
 uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &cfg) {
   if (cfg.sampler == RIP_SAMPLER_COLOUR) return ic_launch(p, first, count, cfg);
-  const uint64_t big = ic_wave_launch(p, first, count);
+  const uint64_t big = ic_wave_launch(p, first, count, cfg.out_chunk);
   if (big == 0) return 1;
   // sets that outgrew a wave's list: by default colour groups over the listed
   // sample indices (samples that reach a giant component share most of their

@@ -48,6 +48,7 @@ struct WaveParams {
   uint32_t *data;
   uint64_t cap_data;
   unsigned long long *ctl;
+  uint32_t chunk;              // pool elements a wave reserves at a time
 };
 
 __device__ inline uint32_t hash_vertex(uint32_t v) {
@@ -190,6 +191,9 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
   uint32_t *list = P.list + wave * P.cap;
   const uint32_t tmask = P.tcap - 1;
   uint32_t epoch = P.epoch[wave];
+  // the wave's reserved output [out_cur, out_end) (wave-uniform); the rest
+  // of a chunk that cannot take the next set stays a gap in the pool's data
+  unsigned long long out_cur = 0, out_end = 0;
   const Affine jl = aff_pow(P.step, lane);    // coin offset of this lane
   const Affine j64 = aff_pow(P.step, 64);     // one round of 64 edges
   unsigned long long st_edges = 0, st_exp = 0;
@@ -241,9 +245,16 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
       wave_bitonic(list, len, cap2);
       sorted = list;
     }
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(&P.ctl[kCtlDataBump], static_cast<unsigned long long>(len));
-    base = __shfl(base, 0, 64);
+    if (out_end - out_cur < len) {
+      const unsigned long long want = len > P.chunk ? len : P.chunk;
+      unsigned long long nb = 0;
+      if (lane == 0) nb = atomicAdd(&P.ctl[kCtlDataBump], want);
+      nb = uniform_u64(__shfl(nb, 0, 64));
+      out_cur = nb;
+      out_end = nb + want;
+    }
+    const unsigned long long base = out_cur;
+    out_cur += len;
     if (base + len > P.cap_data) {
       if (lane == 0) {
         atomicOr(&P.ctl[kCtlOverflow], 1ull);
@@ -429,13 +440,34 @@ uint32_t wave_grid() {
 
 // Returns the number of samples left for the colour sampler (their local
 // indices are in g->wave.d_big).
-uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count) {
-  constexpr int BT = 256, U = 4;
+constexpr int kWaveBT = 256, kWaveU = 4;
+
+uint32_t ic_wave_blocks(uint64_t count) {
+  return std::max<uint32_t>(
+      1, std::min<uint64_t>(wave_grid<kWaveBT, kWaveU>(),
+                            (count + kWaveBT / 64 - 1) / (kWaveBT / 64)));
+}
+
+uint32_t ic_chunk(const rip_pool *p, uint64_t count, uint32_t out_chunk, uint64_t waves) {
+  if (out_chunk) return out_chunk;
+  // rows past the wave's list go to the fallback, so sets written here are
+  // at most 4096 long
+  return out_chunk_rule(static_cast<double>(count) * std::min(p->avg_set_size, 4096.0) /
+                            static_cast<double>(waves),
+                        4096);
+}
+
+uint64_t ic_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune) {
+  const uint64_t waves = static_cast<uint64_t>(ic_wave_blocks(count)) * (kWaveBT / 64);
+  return waves * ic_chunk(p, count, tune ? tune->out_chunk : 0, waves);
+}
+
+uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, uint32_t out_chunk) {
+  constexpr int BT = kWaveBT, U = kWaveU;
   constexpr uint32_t CAP = 4096, TCAP = 8192;
   rip_graph *g = p->g;
   WaveScratch &w = g->wave;
-  const uint32_t blocks = std::max<uint32_t>(
-      1, std::min<uint64_t>(wave_grid<BT, U>(), (count + BT / 64 - 1) / (BT / 64)));
+  const uint32_t blocks = ic_wave_blocks(count);
   const uint64_t waves = static_cast<uint64_t>(blocks) * (BT / 64);
   if (w.waves < waves) {
     dev_free(w.d_table);
@@ -474,6 +506,7 @@ uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count) {
   P.data = p->d_data;
   P.cap_data = p->cap_data;
   P.ctl = p->d_ctl;
+  P.chunk = ic_chunk(p, count, out_chunk, waves);
   RIP_HIP(hipEventRecord(p->ev0, p->stream));
   hipLaunchKernelGGL((ic_wave_kernel<BT, U>), dim3(blocks), dim3(BT), 0, p->stream, P);
   RIP_HIP(hipGetLastError());

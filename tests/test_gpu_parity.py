@@ -180,13 +180,17 @@ def test_selector_graph_replays_across_pool_growth(orc):
                                   dict(colors=64, heavy_degree=1 << 30),
                                   dict(colors=7, heavy_degree=3, edge_run=4),
                                   dict(sampler=1), dict(sampler=2),
-                                  dict(sampler=2, colors=64, heavy_degree=64)])
+                                  dict(sampler=2, colors=64, heavy_degree=64),
+                                  dict(out_chunk=1), dict(out_chunk=5, sampler=1),
+                                  dict(out_chunk=1 << 20)])
 def test_ic_tuning_does_not_change_results(tune, orc):
     index, edges, w = rmat_csr(12, R.W_UNIFORM, 0.1, 1)
     g = R.Graph(index, edges, w, model=R.IC)
     p = g.pool(5)
     p.generate_rrr_sets(2500, tune=tune)
-    assert_same_sets(p.export(), orc.sample(R.IC, index, edges, w, 5, 0, 2500))
+    exp = orc.sample(R.IC, index, edges, w, 5, 0, 2500)
+    assert_same_sets(p.export(), exp)
+    assert p.size() == (2500, int(exp[0][-1]))
     p.close()
     g.close()
 
