@@ -83,6 +83,7 @@ struct Args {
   unsigned long long *keys;
   unsigned long long *ctl;
   uint32_t *kill;      // dense pools: sets killed this round
+  unsigned long long *partial;  // per-block maxima of the argmax pass
   uint64_t num_sets;
   uint32_t n;
 };
@@ -98,6 +99,7 @@ struct SelectScratch {
   uint64_t cap_kill = 0;
   uint8_t *d_covered = nullptr;        // cap_sets
   unsigned long long *d_keys = nullptr;  // per-seed argmax keys (k)
+  unsigned long long *d_partial = nullptr;  // kArgmaxBlocks per-block maxima
   uint32_t cap_k = 0;
   // The k rounds of argmax + kill as one HIP graph, captured for graph_k
   // seeds; its kernels read the buffer pointers from d_args, so the graph

@@ -185,6 +185,7 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->sel.d_kill);
   dev_free(p->sel.d_covered);
   dev_free(p->sel.d_keys);
+  dev_free(p->sel.d_partial);
   dev_free(p->sel.d_args);
   if (p->sel.graph) (void)hipGraphExecDestroy(p->sel.graph);
   if (p->ev0) (void)hipEventDestroy(p->ev0);

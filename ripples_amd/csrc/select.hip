@@ -24,6 +24,7 @@ namespace rip {
 namespace {
 
 constexpr uint32_t kBT = 256;
+constexpr uint32_t kArgmaxBlocks = 1024;  // most blocks of an argmax pass
 
 // Each lane of a wave holds one run (off, len) of `data`; the wave visits
 // every element of the 64 runs once, lanes taking elements of any run:
@@ -269,8 +270,20 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long x) {
   return x;
 }
 
-// Seed round i, first half: keys[i] = max over v of counts[v] << 32 | ~v.
-__global__ void __launch_bounds__(kBT) argmax_kernel(const Args *a, uint32_t i) {
+__device__ inline unsigned long long block_max_u64(unsigned long long x,
+                                                   unsigned long long *s_w) {
+  x = wave_max_u64(x);
+  if (lane_id() == 0) s_w[threadIdx.x >> 6] = x;
+  __syncthreads();
+  x = s_w[0];
+  for (uint32_t w = 1; w < kBT / 64; ++w) x = s_w[w] > x ? s_w[w] : x;
+  return x;
+}
+
+// Seed round, first half: partial[b] = max over block b's vertices v of
+// counts[v] << 32 | ~v.  The maxima are reduced by the second half, where
+// one atomicMax per block on a single key measured as a serial tail.
+__global__ void __launch_bounds__(kBT) argmax_kernel(const Args *a) {
   __shared__ unsigned long long s_w[kBT / 64];
   const uint32_t *counts = a->counts;
   const uint32_t n = a->n;
@@ -280,12 +293,9 @@ __global__ void __launch_bounds__(kBT) argmax_kernel(const Args *a, uint32_t i) 
         (static_cast<unsigned long long>(counts[v]) << 32) | (0xFFFFFFFFu - v);
     best = key > best ? key : best;
   }
-  best = wave_max_u64(best);
-  if (lane_id() == 0) s_w[threadIdx.x >> 6] = best;
-  __syncthreads();
+  best = block_max_u64(best, s_w);
   if (threadIdx.x == 0) {
-    for (uint32_t w = 1; w < kBT / 64; ++w) best = s_w[w] > best ? s_w[w] : best;
-    if (best >> 32) atomicMax(a->keys + i, best);
+    a->partial[blockIdx.x] = best;
     if (blockIdx.x == 0) {  // the dense pools' kill list of this round
       a->ctl[kCtlKillSets] = 0;
       a->ctl[kCtlKillElems] = 0;
@@ -293,10 +303,24 @@ __global__ void __launch_bounds__(kBT) argmax_kernel(const Args *a, uint32_t i) 
   }
 }
 
+// Seed round i, second half, first step in every block: the round's key from
+// the argmax blocks' maxima; block 0 records it in keys[i] (0 when no vertex
+// covers an uncovered set).
+__device__ inline unsigned long long round_key(const Args *a, uint32_t i, uint32_t parts) {
+  __shared__ unsigned long long s_w[kBT / 64];
+  unsigned long long best = 0;
+  for (uint32_t b = threadIdx.x; b < parts; b += kBT)
+    best = a->partial[b] > best ? a->partial[b] : best;
+  best = block_max_u64(best, s_w);
+  if (best >> 32 == 0) best = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a->keys[i] = best;
+  return best;
+}
+
 // Seed round i, second half: every uncovered set containing the seed is
 // marked covered once and its members' counts drop by one.
-__global__ void __launch_bounds__(kBT) kill_kernel(const Args *a, uint32_t i) {
-  const unsigned long long key = a->keys[i];
+__global__ void __launch_bounds__(kBT) kill_kernel(const Args *a, uint32_t i, uint32_t parts) {
+  const unsigned long long key = round_key(a, i, parts);
   const uint32_t gain = static_cast<uint32_t>(key >> 32);
   if (gain == 0) return;
   const uint32_t seed = 0xFFFFFFFFu - static_cast<uint32_t>(key);
@@ -326,8 +350,9 @@ __global__ void __launch_bounds__(kBT) kill_kernel(const Args *a, uint32_t i) {
 // Seed round i, dense pools, first step: every uncovered set holding the
 // seed (binary search; sets are sorted) is marked covered and appended to the
 // kill list; the killed elements are summed.
-__global__ void __launch_bounds__(kBT) scan_kill_kernel(const Args *a, uint32_t i) {
-  const unsigned long long key = a->keys[i];
+__global__ void __launch_bounds__(kBT) scan_kill_kernel(const Args *a, uint32_t i,
+                                                        uint32_t parts) {
+  const unsigned long long key = round_key(a, i, parts);
   const uint32_t gain = static_cast<uint32_t>(key >> 32);
   if (gain == 0) return;
   const uint32_t seed = 0xFFFFFFFFu - static_cast<uint32_t>(key);
@@ -471,6 +496,7 @@ void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t inv_elems,
     s.d_keys = dev_alloc<unsigned long long>(k);
   }
   if (!s.d_args) s.d_args = dev_alloc<Args>(1);
+  if (!s.d_partial) s.d_partial = dev_alloc<unsigned long long>(kArgmaxBlocks);
 }
 
 uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap) {
@@ -548,20 +574,21 @@ void capture_rounds(rip_pool *p, uint32_t k, bool dense) {
   }
   hipStream_t st = p->stream;
   const uint32_t n = p->g->n;
-  const uint32_t argmax_grid = grid_for(n, kBT * 8, 1024);
+  const uint32_t argmax_grid = grid_for(n, kBT * 8, kArgmaxBlocks);
   const uint32_t ranges = ranges_of(n);
   const uint32_t dec_grid = ranges * dense_chunks(ranges, ~0ull);
   RIP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(clear_keys_kernel, dim3(1), dim3(256), 0, st, s.d_args, k);
   for (uint32_t i = 0; i < k; ++i) {
-    hipLaunchKernelGGL(argmax_kernel, dim3(argmax_grid), dim3(kBT), 0, st, s.d_args, i);
+    hipLaunchKernelGGL(argmax_kernel, dim3(argmax_grid), dim3(kBT), 0, st, s.d_args);
     if (dense) {
-      hipLaunchKernelGGL(scan_kill_kernel, dim3(kScanGrid), dim3(kBT), 0, st, s.d_args, i);
+      hipLaunchKernelGGL(scan_kill_kernel, dim3(kScanGrid), dim3(kBT), 0, st, s.d_args, i,
+                         argmax_grid);
       hipLaunchKernelGGL(dec_small_kernel, dim3(256), dim3(kBT), 0, st, s.d_args);
       hipLaunchKernelGGL(dense_dec_kernel, dim3(dec_grid), dim3(kDenseBT), 0, st, s.d_args,
                          ranges);
     } else {
-      hipLaunchKernelGGL(kill_kernel, dim3(256), dim3(kBT), 0, st, s.d_args, i);
+      hipLaunchKernelGGL(kill_kernel, dim3(256), dim3(kBT), 0, st, s.d_args, i, argmax_grid);
     }
   }
   const hipError_t launch = hipGetLastError();
@@ -600,7 +627,7 @@ uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
   // the seed rounds run as one graph; this call's buffers travel in d_args
   // (`args` stays alive until the stream is synchronised below)
   const Args args{s.d_counts, s.d_inv_start, s.d_inv, p->d_off, p->d_len, p->d_data,
-                  s.d_covered, s.d_keys, p->d_ctl, s.d_kill, num_sets, n};
+                  s.d_covered, s.d_keys, p->d_ctl, s.d_kill, s.d_partial, num_sets, n};
   RIP_HIP(hipMemcpyAsync(s.d_args, &args, sizeof(Args), hipMemcpyHostToDevice, st));
   if (k) {
     if (!s.graph || s.graph_k != k || s.graph_dense != dense) capture_rounds(p, k, dense);
