@@ -189,8 +189,11 @@ int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
  *                        loop scans (up to and including the chosen one)
  *   RIP_STAT_COINS       IC: (edge, sample) coin evaluations;
  *                        LT: row breakpoints read
- *   RIP_STAT_EXPANSIONS  IC: frontier entries expanded (vertex, colour group)
- *   RIP_STAT_LEVELS      IC: BFS levels summed over colour groups */
+ *   RIP_STAT_EXPANSIONS  IC: frontier entries expanded (vertex, colour group);
+ *                        LT: walk steps (rounds of a busy lane)
+ *   RIP_STAT_LEVELS      IC: BFS levels summed over colour groups;
+ *                        LT: loop rounds summed over waves (64 x this over
+ *                        steps is the lane utilisation of the walk kernel) */
 #define RIP_STAT_ALGO_BYTES 0
 #define RIP_STAT_EDGES 1
 #define RIP_STAT_COINS 2

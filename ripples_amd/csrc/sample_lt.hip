@@ -97,7 +97,9 @@ __device__ inline uint32_t count_le(uint4 q, uint32_t k) {
 // line, *id, the in-neighbour of the crossing edge when there is
 // one (read from the block's line after the search).  *probes counts the
 // nodes and blocks read (lt_index_cross of lt_scan.h is the same search over
-// plain arrays).
+// plain arrays).  Loading the top level with the head, and the ids with the
+// breakpoints, measured no faster: the walk is bound by the lines in flight,
+// not by the number of dependent loads.
 __device__ inline uint32_t lt_cross(const LtRec *lrec, uint32_t v, const uint32_t *bp,
                                     const uint32_t *bidx, uint32_t boff, uint32_t d,
                                     uint32_t ioff, uint32_t k, uint32_t *probes, uint32_t *id) {
@@ -186,9 +188,9 @@ __device__ inline void wave_sort128(uint32_t &x0, uint32_t &x1) {
 // longer walks are handed to lt_long_kernel.  Finished lists up to CAP are
 // sorted four at a time in 16-lane register groups; longer ones are copied
 // by the whole wave and sorted by lt_sort_kernel.
-// The compiler's own register budget (87 registers, five waves per SIMD, no
-// scratch) measured as fast as six or seven waves and 4% faster than eight
-// waves with spills, on R-MAT scale 20.
+// The compiler's own register budget (about 100 registers, four waves per
+// SIMD, no scratch) is kept: forcing six to eight waves per SIMD measured no
+// faster, and eight waves spilled, on R-MAT scale 20.
 template <int BT, int CAP, int SPILL, int UNROLL>
 __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   static_assert(UNROLL % 4 == 0, "groups of four weights per load");
@@ -201,8 +203,27 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   __syncthreads();
   uint32_t *list = s_list + threadIdx.x;  // element i < CAP at list[i * BT]
   uint32_t *spill = P.spill + (static_cast<uint64_t>(blockIdx.x) * BT + threadIdx.x) * SPILL;
-  auto at = [&](uint32_t i) -> uint32_t & {
-    return i < static_cast<uint32_t>(CAP) ? list[i * BT] : spill[i - CAP];
+  // Whether u is among the walk's first len vertices.  The LDS part and the
+  // spill are searched with their own loads, four at a time (one access path
+  // choosing between them per element compiles to dependent flat loads).
+  auto listed = [&](uint32_t u, uint32_t len) -> bool {
+    const uint32_t nl = len < static_cast<uint32_t>(CAP) ? len : static_cast<uint32_t>(CAP);
+    bool hit = false;
+    for (uint32_t i = 0; i < nl && !hit; i += 4) {
+      uint32_t x[4];
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) x[q] = i + q < nl ? list[(i + q) * BT] : ~u;
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) hit = hit || x[q] == u;
+    }
+    for (uint32_t i = CAP; i < len && !hit; i += 4) {
+      uint32_t x[4];
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) x[q] = i + q < len ? spill[i + q - CAP] : ~u;
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) hit = hit || x[q] == u;
+    }
+    return hit;
   };
   const uint4 *val4 = reinterpret_cast<const uint4 *>(P.val);
   const uint32_t lane = lane_id();
@@ -214,7 +235,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   uint64_t claim_lo = 0, claim_hi = 0;
   // the batch [claim_base, claim_hi) and the stream start of its first id
   uint64_t claim_base = 0, z_base = 0;
-  uint32_t len = 0, e = 0, end = 0, rflags = 0, kthr = 0, ridx = 0, vcur = 0;
+  uint32_t len = 0, e = 0, end = 0, rflags = 0, kthr = 0, ridx = 0;
   uint64_t filt0 = 0, filt1 = 0;  // visited filter, bits 0..63 and 64..127
   // the wave's reserved output [out_cur, out_end) (wave-uniform): finished
   // walks are written there, and a new chunk is reserved only when they do
@@ -235,9 +256,12 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   // work counters: in-edges the subtraction loop scans, index nodes and
   // breakpoint blocks read, vertices written out (every visited vertex is)
   uint32_t edges = 0, probes = 0, outs = 0;
+  // busy-lane rounds (walk steps) and loop rounds of the wave
+  uint32_t steps = 0, rounds = 0;
 
   // e, end: the row's pairs for a breakpoint search, its transposed-CSR
   // range for rows scanned by weight
+  uint32_t vcur = 0;  // the walk's current vertex
   auto enter = [&](uint32_t v) {
     const LtHead h = lt_head(P.lrec, v);
     kthr = lt_threshold(h.ja, h.jb, z);
@@ -295,10 +319,12 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
       }
     }
     if (__ballot(busy) == 0) break;
+    ++rounds;
 
     // ---- one step (or one scan round of a row without breakpoints)
     bool done = false, is_long = false;
     if (busy) {
+      ++steps;
       bool found = false;
       uint32_t u = 0;  // the chosen in-neighbour
       if (e >= end) {
@@ -342,20 +368,17 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         if (found) u = P.ids[e - 1];
       }
       if (found) {
-        bool seen = false;
-        if (filter_has(u))
-          for (uint32_t i = 0; i < len; ++i)
-            if (at(i) == u) {
-              seen = true;
-              break;
-            }
+        const bool seen = filter_has(u) && listed(u, len);
         if (seen) {
           done = true;
         } else if (len == CAP + SPILL) {
           done = true;
           is_long = true;
         } else {
-          at(len) = u;
+          if (len < static_cast<uint32_t>(CAP))
+            list[len * BT] = u;
+          else
+            spill[len - CAP] = u;
           ++len;
           filter_add(u);
           enter(u);
@@ -451,12 +474,15 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
       }
     }
   }
-  unsigned long long e64 = edges, p64 = probes, o64 = outs;
+  unsigned long long e64 = edges, p64 = probes, o64 = outs, s64 = steps;
   for (int d = 32; d > 0; d >>= 1) {
     e64 += __shfl_down(e64, d, 64);
     p64 += __shfl_down(p64, d, 64);
     o64 += __shfl_down(o64, d, 64);
+    s64 += __shfl_down(s64, d, 64);
   }
+  if (lane == 0 && s64) atomicAdd(&P.ctl[kCtlStatExpansions], s64);
+  if (lane == 0 && rounds) atomicAdd(&P.ctl[kCtlStatLevels], static_cast<unsigned long long>(rounds));
   // algorithmic bytes of the subtraction loop: 8 per visited vertex (its CSR
   // offsets), 8 per scanned in-edge (id + weight), 4 per output id
   if (lane == 0 && (e64 || o64)) atomicAdd(&P.ctl[kCtlStatBytes], 12ull * o64 + 8ull * e64);
