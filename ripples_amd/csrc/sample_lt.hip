@@ -53,12 +53,16 @@ struct LtParams {
   uint32_t *long_ids;
   uint32_t *spill;
   uint32_t chunk;  // pool elements a wave reserves at a time
+  uint32_t claim;  // sample ids a wave claims at a time, 64..kClaim
 };
 
-// Sample ids a wave claims from the work counter at a time (at least a
-// wave's worth: one refill hands out up to 64 ids).
-constexpr unsigned long long kClaim = 256;
-static_assert(kClaim >= 64, "a refill takes at most one new batch");
+// Resident blocks of the walk kernel per CU (at most).
+constexpr int kLtBlocksPerCu = 2;
+
+// Most sample ids a wave claims from the work counter at a time; the batch
+// (LtParams::claim) is at least a wave's worth, since one refill hands out
+// up to 64 ids.
+constexpr unsigned long long kClaim = 1024;
 
 // LT threshold of the walk at a vertex (ja, jb from its record): U = k * 2^-24.
 __device__ inline uint32_t lt_threshold(uint64_t ja, uint64_t jb, uint64_t z) {
@@ -178,7 +182,7 @@ __device__ inline void wave_sort128(uint32_t &x0, uint32_t &x1) {
 }
 
 // Persistent walkers: every lane runs its own walk and, when it ends, takes
-// the next sample id from its wave's batch (claimed kClaim at a time from a
+// the next sample id from its wave's batch (claimed P.claim at a time from a
 // device counter), so short walks never wait for the long ones of the wave.  Each round
 // takes one step of every busy lane (a breakpoint search), or scans up to
 // UNROLL in-edges of a row without breakpoints with 16-byte loads issued
@@ -199,7 +203,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   // s_pow[r] = M^(2r): from the stream start of a batch's first sample id to
   // that of the id r places later (element 0 of sample s is M^(2s+2)(seed))
   __shared__ Affine s_pow[kClaim];
-  for (uint32_t r = threadIdx.x; r < kClaim; r += BT) s_pow[r] = aff_pow(master_step(), 2ull * r);
+  for (uint32_t r = threadIdx.x; r < P.claim; r += BT) s_pow[r] = aff_pow(master_step(), 2ull * r);
   __syncthreads();
   uint32_t *list = s_list + threadIdx.x;  // element i < CAP at list[i * BT]
   uint32_t *spill = P.spill + (static_cast<uint64_t>(blockIdx.x) * BT + threadIdx.x) * SPILL;
@@ -289,7 +293,8 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         // a new batch; the rest of the current one is handed out first
         const uint32_t leader = __ffsll(static_cast<unsigned long long>(wm)) - 1;
         unsigned long long nb = 0;
-        if (lane == leader) nb = atomicAdd(&P.ctl[kCtlGroupCounter], kClaim);
+        if (lane == leader)
+          nb = atomicAdd(&P.ctl[kCtlGroupCounter], static_cast<unsigned long long>(P.claim));
         nb = __shfl(nb, static_cast<int>(leader), 64);
         const uint64_t znb = sample_z(P.seed, P.first + nb);  // once per batch
         id = rank < have ? claim_lo + rank : nb + (rank - have);
@@ -298,7 +303,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
           zb = znb;
         }
         claim_lo = nb + (need - have);
-        claim_hi = nb + kClaim;
+        claim_hi = nb + P.claim;
         claim_base = nb;
         z_base = znb;
       }
@@ -498,6 +503,10 @@ uint32_t lt_grid(uint64_t count) {
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
+  // Walks are bound by the lines in flight, not by walker count: on R-MAT
+  // scale 20 two blocks per CU (eight waves) measured 0.93 ms per launch, one
+  // block 1.03 ms and the four that fit 1.05 ms.
+  per_cu = std::min(per_cu, kLtBlocksPerCu);
   int dev = 0, cus = 1;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -610,15 +619,25 @@ __global__ void reset_long_kernel(unsigned long long *ctl) {
 
 constexpr int kLtBT = 256, kLtCap = 16, kLtSpill = 112, kLtUnroll = 16;
 
+// Ids per claim: an even share of the launch per wave, so every wave gets
+// one batch, rounded up to whole waves of lanes.
+uint32_t lt_claim(uint64_t count, uint32_t blocks) {
+  const uint64_t waves = static_cast<uint64_t>(blocks) * (kLtBT / 64);
+  const uint64_t share = (count + waves - 1) / waves;
+  return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>((share + 63) / 64 * 64, 64),
+                                                  kClaim));
+}
+
 // Output elements a wave reserves at a time.  On R-MAT scale 20 (657,104
 // walks of 5.3 vertices) chunks of 1, 256 and 512 elements measured 1.69,
 // 1.18 and 1.16 ms per launch.
 uint32_t lt_chunk(const rip_pool *p, uint64_t count, const rip_tune *tune, uint32_t blocks) {
   if (tune && tune->out_chunk) return tune->out_chunk;
   // only waves that get a batch of ids write anything
+  const uint32_t claim = lt_claim(count, blocks);
   const uint64_t waves = std::max<uint64_t>(
       1, std::min<uint64_t>(static_cast<uint64_t>(blocks) * (kLtBT / 64),
-                            (count + kClaim - 1) / kClaim));
+                            (count + claim - 1) / claim));
   return out_chunk_rule(static_cast<double>(count) * p->avg_set_size / static_cast<double>(waves),
                         2048);
 }
@@ -657,6 +676,7 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   hipLaunchKernelGGL(reset_long_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
   const uint32_t blocks = lt_grid<BT, CAP, SPILL, UNROLL>(count);
   P.chunk = lt_chunk(p, count, tune, blocks);
+  P.claim = lt_claim(count, blocks);
   const uint64_t spill_words = static_cast<uint64_t>(blocks) * BT * SPILL;
   if (p->g->lt.cap_spill < spill_words) {
     dev_free(p->g->lt.d_spill);
