@@ -191,6 +191,8 @@ void pool_create(rip_pool *p, rip_graph *g, uint64_t seed);
 void pool_destroy(rip_pool *p);
 void pool_reserve_sets(rip_pool *p, uint64_t cap_sets);
 void pool_reserve_data(rip_pool *p, uint64_t cap_data);
+// *out += sum of v[0, n) on stream st (out is device memory)
+void launch_sum_u32(const uint32_t *v, uint64_t n, unsigned long long *out, hipStream_t st);
 void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune);
 void pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
                       const uint32_t *vertices);

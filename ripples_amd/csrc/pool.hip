@@ -93,14 +93,22 @@ __global__ void set_ctl_kernel(unsigned long long *ctl, unsigned long long bump)
   for (int i = kCtlStatBytes; i <= kCtlStatLevels; ++i) ctl[i] = 0;
 }
 
-// *out += sum of len[0, count)
-__global__ void sum_sizes_kernel(const uint32_t *len, uint64_t count, unsigned long long *out) {
+// *out += sum of v[0, n): one atomic per block (one per wave measured as a
+// serial tail of tens of microseconds on the single target word)
+__global__ void __launch_bounds__(256) sum_u32_kernel(const uint32_t *v, uint64_t n,
+                                                      unsigned long long *out) {
+  __shared__ unsigned long long s_w[4];
   unsigned long long acc = 0;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-    acc += len[i];
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * 256)
+    acc += v[i];
   for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d, 64);
-  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    acc = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    if (acc) atomicAdd(out, acc);
+  }
 }
 
 // Copy sets [first, first+count) into CSR order at out (prefix from scan).
@@ -219,6 +227,14 @@ void pool_reserve_sets(rip_pool *p, uint64_t cap) {
   p->cap_sets = cap;
 }
 
+void launch_sum_u32(const uint32_t *v, uint64_t n, unsigned long long *out, hipStream_t st) {
+  if (n == 0) return;
+  const uint64_t blocks = std::min<uint64_t>((n + 4095) / 4096, 512);
+  hipLaunchKernelGGL(sum_u32_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, st, v, n,
+                     out);
+  RIP_HIP(hipGetLastError());
+}
+
 void pool_reserve_data(rip_pool *p, uint64_t cap) {
   if (cap <= p->cap_data) return;
   auto *data = dev_alloc<uint32_t>(cap);
@@ -257,12 +273,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
       launches = ic_sample(p, first, count, ic);
     else
       launches = lt_launch(p, first, count, tune);
-    {
-      const uint64_t blocks = std::min<uint64_t>((count + 255) / 256, 1024);
-      hipLaunchKernelGGL(sum_sizes_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
-                         p->stream, p->d_len + first, count, p->d_ctl + kCtlElems);
-      RIP_HIP(hipGetLastError());
-    }
+    launch_sum_u32(p->d_len + first, count, p->d_ctl + kCtlElems, p->stream);
     unsigned long long ctl[kCtlWords];
     RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, p->stream));
     RIP_HIP(hipStreamSynchronize(p->stream));

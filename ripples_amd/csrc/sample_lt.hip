@@ -156,12 +156,14 @@ __device__ inline uint32_t lt_cross(const LtRec *lrec, uint32_t v, const uint32_
 // Bit of a vertex in a walk's 128-bit visited filter.
 __device__ inline uint32_t lt_filter_bit(uint32_t v) { return (v * 0x9E3779B1u) >> 25; }
 
-// Ascending bitonic sort of 128 keys held two per lane (a[lane], a[lane + 64])
-// across a wave.
-__device__ inline void wave_sort128(uint32_t &x0, uint32_t &x1) {
+// Ascending bitonic sort of the first w keys of a wave (w = 32, 64 or 128,
+// wave-uniform), held two per lane as a[lane], a[lane + 64]; for w < 128
+// only x0 takes part, and for w == 32 lanes 32..63 sort a sequence of their
+// own that the caller ignores.  Smaller w saves stages: 15, 21 or 28.
+__device__ inline void wave_sort(uint32_t &x0, uint32_t &x1, uint32_t w) {
   const uint32_t lane = lane_id();
 #pragma unroll 1
-  for (uint32_t k = 2; k <= 128; k <<= 1) {
+  for (uint32_t k = 2; k <= w; k <<= 1) {
 #pragma unroll 1
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
       if (j == 64) {
@@ -172,10 +174,13 @@ __device__ inline void wave_sort128(uint32_t &x0, uint32_t &x1) {
       } else {
         const bool lower = (lane & j) == 0;
         const uint32_t p0 = __shfl_xor(x0, static_cast<int>(j), 64);
-        const uint32_t p1 = __shfl_xor(x1, static_cast<int>(j), 64);
-        const bool asc0 = (lane & k) == 0, asc1 = ((lane + 64) & k) == 0;
+        const bool asc0 = (lane & k) == 0;
         x0 = (lower == asc0) ? min(x0, p0) : max(x0, p0);
-        x1 = (lower == asc1) ? min(x1, p1) : max(x1, p1);
+        if (w == 128) {
+          const uint32_t p1 = __shfl_xor(x1, static_cast<int>(j), 64);
+          const bool asc1 = ((lane + 64) & k) == 0;
+          x1 = (lower == asc1) ? min(x1, p1) : max(x1, p1);
+        }
       }
     }
   }
@@ -604,7 +609,7 @@ __global__ void lt_sort_kernel(LtParams P) {
     uint32_t x0 = 0xFFFFFFFFu, x1 = 0xFFFFFFFFu;
     if (lane < len) x0 = P.data[off + lane];
     if (lane + 64 < len) x1 = P.data[off + lane + 64];
-    wave_sort128(x0, x1);
+    wave_sort(x0, x1, len <= 32 ? 32u : len <= 64 ? 64u : 128u);
     if (lane < len) P.data[off + lane] = x0;
     if (lane + 64 < len) P.data[off + lane + 64] = x1;
   }
@@ -689,7 +694,7 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
                      p->stream, P);
   RIP_HIP(hipGetLastError());
   RIP_HIP(hipEventRecord(p->ev1, p->stream));  // the events time the walk kernel alone
-  hipLaunchKernelGGL(lt_sort_kernel, dim3(256), dim3(256), 0, p->stream, P);
+  hipLaunchKernelGGL(lt_sort_kernel, dim3(1024), dim3(256), 0, p->stream, P);
   RIP_HIP(hipGetLastError());
   uint32_t launches = 2;
   unsigned long long nlong = 0;

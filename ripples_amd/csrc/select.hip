@@ -226,16 +226,6 @@ __global__ void __launch_bounds__(kDenseBT) dense_dec_kernel(const Args *a, uint
                    ranges, a->counts);
 }
 
-__global__ void sum_len_kernel(const uint32_t *set_len, uint64_t num_sets,
-                               unsigned long long *out) {
-  unsigned long long acc = 0;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-       i < num_sets; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-    acc += set_len[i];
-  for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d, 64);
-  if (lane_id() == 0 && acc) atomicAdd(out, acc);
-}
-
 __global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
        i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
@@ -533,8 +523,7 @@ bool build_counts(rip_pool *p, uint64_t num_sets, uint32_t k, Need need) {
   RIP_HIP(hipSetDevice(p->g->device));
   // total element count
   hipLaunchKernelGGL(zero_word_kernel, dim3(1), dim3(1), 0, st, p->d_ctl + kCtlWords - 1);
-  hipLaunchKernelGGL(sum_len_kernel, dim3(grid_for(num_sets, kBT, 1024)), dim3(kBT), 0, st,
-                     p->d_len, num_sets, p->d_ctl + kCtlWords - 1);
+  launch_sum_u32(p->d_len, num_sets, p->d_ctl + kCtlWords - 1, st);
   unsigned long long total = 0;
   RIP_HIP(hipMemcpyAsync(&total, p->d_ctl + kCtlWords - 1, sizeof(total),
                          hipMemcpyDeviceToHost, st));
