@@ -101,17 +101,30 @@ __device__ inline void discover(const IcParams &P, unsigned long long *vp,
 // of atomic is issued for all N before any of its results is used, so a run
 // waits for three rounds of atomics rather than 3N.  A lane's pairs that
 // share a vertex are ordered as in N calls: atomics from one lane to one
-// address complete in issue order.
+// address complete in issue order.  Within a group visited masks only gain
+// bits, so a read that already shows every colour makes the atomic
+// unnecessary (a read that lags can only cost an extra atomic): in a giant
+// component most edges lead to vertices every colour has reached, and their
+// atomics, many on the same hub words, were most of the expansion time.  The
+// read is a device-scope atomic load, served by L2 like the atomics: a plain
+// load could meet a vector-L1 line from an earlier group, whose masks have
+// since been cleared.
 template <int N>
 __device__ inline void discover_n(unsigned long long *vp, uint32_t *bm0, uint32_t *bm1,
                                   const uint32_t *u, const unsigned long long *bits,
                                   uint32_t *s_union, uint32_t *s_qn, uint32_t *qnext) {
+  unsigned long long need[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    need[i] = bits[i] ? bits[i] & ~__hip_atomic_load(&vp[2ull * u[i]], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                      : 0ull;
   unsigned long long fresh[N];
   bool first[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    const unsigned long long old = bits[i] ? atomicOr(&vp[2ull * u[i]], bits[i]) : ~0ull;
-    fresh[i] = bits[i] & ~old;
+    const unsigned long long old = need[i] ? atomicOr(&vp[2ull * u[i]], need[i]) : ~0ull;
+    fresh[i] = need[i] & ~old;
     first[i] = fresh[i] != 0 && old == 0;
   }
   bool push[N];
