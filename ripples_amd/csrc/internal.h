@@ -204,6 +204,8 @@ struct IcLaunch {
   uint32_t colors, block, slots, edge_run, heavy_degree, sampler;
   bool colors_auto;    // colours not set by the caller
   uint32_t out_chunk;  // wave sampler's output chunk, 0 = automatic
+  bool block_auto;     // block size not set by the caller
+  bool slots_auto;     // slots not set by the caller
 };
 
 // Pool elements a sampler wave reserves at a time: a power of two in

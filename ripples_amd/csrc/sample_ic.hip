@@ -499,14 +499,20 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
   // defaults from a sweep on R-MAT scale 22 weighted cascade: 8-edge runs and
   // 256-thread groups (more independent groups per CU); hub deferral off,
   // because colours of one group rarely meet at the same hub there
-  IcLaunch c{64, 256, 0, 8, 0xFFFFFFFFu, RIP_SAMPLER_AUTO, true, 0};
+  IcLaunch c{64, 256, 0, 8, 0xFFFFFFFFu, RIP_SAMPLER_AUTO, true, 0, true, true};
   if (tune) {
     if (tune->colors) {
       c.colors = tune->colors;
       c.colors_auto = false;
     }
-    if (tune->block) c.block = tune->block;
-    if (tune->slots) c.slots = tune->slots;
+    if (tune->block) {
+      c.block = tune->block;
+      c.block_auto = false;
+    }
+    if (tune->slots) {
+      c.slots = tune->slots;
+      c.slots_auto = false;
+    }
     if (tune->edge_run) c.edge_run = tune->edge_run;
     if (tune->heavy_degree) c.heavy_degree = tune->heavy_degree;
     c.sampler = tune->sampler;
@@ -629,13 +635,24 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
   if (cfg.sampler == RIP_SAMPLER_WAVE) {
     ic_big_launch(p, first, big);
   } else {
-    // the groups are latency-bound, one level at a time: about one group per
-    // two resident slots, i.e. colours near 2 big / slots (rounded to a power
-    // of two), measured best on R-MAT scales 16 and 18 with U[0,0.1] weights
-    // from 1,000 to 17,000 large samples (up to 2.3x over 64 colours)
+    // the groups are latency-bound, one level at a time: 512-thread groups
+    // (on R-MAT scale 18 with U[0,0.1] weights, 100,000 samples: 0.119 s at
+    // 64 colours against 0.150 s with 256 threads and 0.123 s with 1024) and
+    // about one group per two resident slots, i.e. colours near 2 big / slots
+    // (rounded to a power of two), measured best on R-MAT scales 16 and 18
+    // with U[0,0.1] weights from 1,000 to 17,000 large samples
     IcLaunch fb = cfg;
+    if (cfg.block_auto) {
+      fb.block = 512;
+      if (cfg.slots_auto) {
+        int cus = 1;
+        RIP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->g->device));
+        fb.slots = std::min<uint32_t>(
+            cfg.slots, static_cast<uint32_t>(cus) * resident_blocks(fb.block, fb.edge_run));
+      }
+    }
     if (cfg.colors_auto) {
-      const double want = 2.0 * static_cast<double>(big) / std::max<uint32_t>(cfg.slots, 1);
+      const double want = 2.0 * static_cast<double>(big) / std::max<uint32_t>(fb.slots, 1);
       uint32_t c = 1;
       while (c < 64 && static_cast<double>(c) * 1.41421356 < want) c <<= 1;
       fb.colors = c;
