@@ -191,6 +191,11 @@ void pool_create(rip_pool *p, rip_graph *g, uint64_t seed);
 void pool_destroy(rip_pool *p);
 void pool_reserve_sets(rip_pool *p, uint64_t cap_sets);
 void pool_reserve_data(rip_pool *p, uint64_t cap_data);
+// Resident blocks per CU of a kernel at a block size (at least 1), and the
+// CU count of the current device; both cached, so launchers can ask on
+// every call.
+int blocks_per_cu(const void *kernel, int block);
+int cu_count();
 // *out += sum of v[0, n) on stream st (out is device memory)
 void launch_sum_u32(const uint32_t *v, uint64_t n, unsigned long long *out, hipStream_t st);
 void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune);
@@ -239,6 +244,9 @@ void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count);
 // sample_lt.hip
 uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
                    const rip_tune *tune);
+// Finishes the nlong walks lt_launch left longer than its lists; returns the
+// launches made (synchronises the pool's stream).
+uint32_t lt_finish_long(rip_pool *p, uint64_t first, uint64_t count, uint64_t nlong);
 // Most pool elements an LT launch of `count` samples can leave unused (the
 // rest of each wave's last output chunk).
 uint64_t lt_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune);

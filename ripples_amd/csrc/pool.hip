@@ -8,6 +8,9 @@
 // include/ripples/rrr_sets.h:81-105 and the flattened device copy built for
 // selection (streaming_find_most_influential.h:139-244).
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstring>
 #include <vector>
 
@@ -92,6 +95,8 @@ __global__ void set_ctl_kernel(unsigned long long *ctl, unsigned long long bump)
   ctl[kCtlElems] = 0;
   for (int i = kCtlStatBytes; i <= kCtlStatLevels; ++i) ctl[i] = 0;
 }
+
+__global__ void zero_elems_kernel(unsigned long long *ctl) { ctl[kCtlElems] = 0; }
 
 // *out += sum of v[0, n): one atomic per block (one per wave measured as a
 // serial tail of tens of microseconds on the single target word)
@@ -227,6 +232,40 @@ void pool_reserve_sets(rip_pool *p, uint64_t cap) {
   p->cap_sets = cap;
 }
 
+int blocks_per_cu(const void *kernel, int block) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void *, int>, int> cache;
+  int dev = 0;
+  RIP_HIP(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, kernel, block);
+  {
+    std::lock_guard<std::mutex> hold(mu);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  std::lock_guard<std::mutex> hold(mu);
+  cache[key] = per_cu;
+  return per_cu;
+}
+
+int cu_count() {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  int dev = 0;
+  RIP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> hold(mu);
+  const auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  int cus = 1;
+  RIP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  cache[dev] = std::max(cus, 1);
+  return cache[dev];
+}
+
 void launch_sum_u32(const uint32_t *v, uint64_t n, unsigned long long *out, hipStream_t st) {
   if (n == 0) return;
   const uint64_t blocks = std::min<uint64_t>((n + 4095) / 4096, 512);
@@ -277,6 +316,14 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
     unsigned long long ctl[kCtlWords];
     RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, p->stream));
     RIP_HIP(hipStreamSynchronize(p->stream));
+    if (p->g->model == RIP_LT && ctl[kCtlOverflow] == 0 && ctl[kCtlLongCount] != 0) {
+      // walks longer than the walk kernel's lists, then the sizes again
+      launches += lt_finish_long(p, first, count, ctl[kCtlLongCount]);
+      hipLaunchKernelGGL(zero_elems_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
+      launch_sum_u32(p->d_len + first, count, p->d_ctl + kCtlElems, p->stream);
+      RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, p->stream));
+      RIP_HIP(hipStreamSynchronize(p->stream));
+    }
     if (ctl[kCtlOverflow] == 0) {
       float ms = 0.0f;
       RIP_HIP(hipEventElapsedTime(&ms, p->ev0, p->ev1));

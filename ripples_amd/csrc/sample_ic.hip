@@ -452,13 +452,7 @@ __global__ void reset_counter_kernel(unsigned long long *ctl) {
 // Workgroups of the persistent kernel that fit on one CU at once.
 template <int BT, int RUN>
 int occupancy_of() {
-  int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &blocks, reinterpret_cast<const void *>(&ic_fused_kernel<BT, RUN>), BT, 0) !=
-          hipSuccess ||
-      blocks < 1)
-    blocks = 1;
-  return blocks;
+  return blocks_per_cu(reinterpret_cast<const void *>(&ic_fused_kernel<BT, RUN>), BT);
 }
 
 template <int BT>
@@ -524,8 +518,6 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
   RIP_REQUIRE(c.edge_run == 4 || c.edge_run == 8 || c.edge_run == 16, -1,
               "tune: edge_run must be 4, 8 or 16");
   if (c.slots == 0) {
-    hipDeviceProp_t prop;
-    RIP_HIP(hipGetDeviceProperties(&prop, p->g->device));
     size_t free_b = 0, total_b = 0;
     RIP_HIP(hipMemGetInfo(&free_b, &total_b));
     const size_t per = slot_bytes_for(p->g, c.heavy_degree);
@@ -535,8 +527,7 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
     if (p->g->ic.slots && p->g->ic.n == p->g->n) held = p->g->ic.slot_bytes * p->g->ic.slots;
     const size_t budget = std::min((free_b + held) / 8, static_cast<size_t>(24) << 30);
     uint64_t by_mem = std::max<uint64_t>(1, budget / per);
-    uint64_t by_cu = static_cast<uint64_t>(prop.multiProcessorCount) *
-                     resident_blocks(c.block, c.edge_run);
+    uint64_t by_cu = static_cast<uint64_t>(cu_count()) * resident_blocks(c.block, c.edge_run);
     c.slots = static_cast<uint32_t>(std::min(by_mem, by_cu));
   }
   return c;
@@ -645,10 +636,8 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
     if (cfg.block_auto) {
       fb.block = 512;
       if (cfg.slots_auto) {
-        int cus = 1;
-        RIP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->g->device));
         fb.slots = std::min<uint32_t>(
-            cfg.slots, static_cast<uint32_t>(cus) * resident_blocks(fb.block, fb.edge_run));
+            cfg.slots, static_cast<uint32_t>(cu_count()) * resident_blocks(fb.block, fb.edge_run));
       }
     }
     if (cfg.colors_auto) {

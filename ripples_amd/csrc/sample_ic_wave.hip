@@ -424,16 +424,8 @@ __global__ void reset_big_kernel(unsigned long long *ctl) { ctl[kCtlBigCounter] 
 
 template <int BT, int U>
 uint32_t wave_grid() {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void *>(&ic_wave_kernel<BT, U>), BT, 0) !=
-          hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  int dev = 0, cus = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  return static_cast<uint32_t>(per_cu * cus);
+  return static_cast<uint32_t>(
+      blocks_per_cu(reinterpret_cast<const void *>(&ic_wave_kernel<BT, U>), BT) * cu_count());
 }
 
 }  // namespace
@@ -534,14 +526,8 @@ void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count) {
   const uint64_t per_wave = 4ull * (static_cast<uint64_t>(g->n) + words + swords);
   size_t free_b = 0, total_b = 0;
   RIP_HIP(hipMemGetInfo(&free_b, &total_b));
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void *>(&ic_big_kernel<BT, U>), BT, 0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  int dev = 0, cus = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(&ic_big_kernel<BT, U>), BT);
+  const int cus = cu_count();
   uint64_t waves = std::min<uint64_t>(static_cast<uint64_t>(per_cu) * cus * (BT / 64),
                                       std::max<uint64_t>(1, (total_b / 16) / per_wave));
   waves = std::min<uint64_t>(waves, (count + BT / 64 - 1) / (BT / 64) * (BT / 64));

@@ -502,20 +502,13 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
 
 template <int BT, int CAP, int SPILL, int UNROLL>
 uint32_t lt_grid(uint64_t count) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void *>(&lt_walk_kernel<BT, CAP, SPILL, UNROLL>), BT, 0) !=
-          hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
+  int per_cu =
+      blocks_per_cu(reinterpret_cast<const void *>(&lt_walk_kernel<BT, CAP, SPILL, UNROLL>), BT);
   // Walks are bound by the lines in flight, not by walker count: on R-MAT
   // scale 20 two blocks per CU (eight waves) measured 0.93 ms per launch, one
   // block 1.03 ms and the four that fit 1.05 ms.
   per_cu = std::min(per_cu, kLtBlocksPerCu);
-  int dev = 0, cus = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const uint64_t resident = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(cus);
+  const uint64_t resident = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(cu_count());
   const uint64_t needed = (count + BT - 1) / BT;
   return static_cast<uint32_t>(std::max<uint64_t>(1, std::min(resident, needed)));
 }
@@ -652,11 +645,10 @@ uint64_t lt_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune
   return static_cast<uint64_t>(blocks) * (kLtBT / 64) * lt_chunk(p, count, tune, blocks);
 }
 
-uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
-                   const rip_tune *tune) {
-  constexpr int BT = kLtBT, CAP = kLtCap, SPILL = kLtSpill, UNROLL = kLtUnroll;
+// Kernel parameters shared by the walk, sort and long-walk kernels.
+LtParams lt_params(rip_pool *p, uint64_t first, uint64_t count) {
   const rip_graph *g = p->g;
-  LtParams P;
+  LtParams P{};
   P.lrec = g->d_lrec;
   P.ids = g->d_ids;
   P.val = g->d_val;
@@ -671,24 +663,34 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   P.data = p->d_data;
   P.cap_data = p->cap_data;
   P.ctl = p->d_ctl;
+  P.long_ids = g->lt.d_long_ids;
+  P.spill = g->lt.d_spill;
+  return P;
+}
+
+// Enqueues the walk and sort kernels of samples [first, first+count); no
+// host synchronisation.  Walks longer than the lists are counted in the
+// pool's control words and finished by lt_finish_long.
+uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
+                   const rip_tune *tune) {
+  constexpr int BT = kLtBT, CAP = kLtCap, SPILL = kLtSpill, UNROLL = kLtUnroll;
   // long-walk ids: at most `count`
   if (p->g->lt.cap_ids < count) {
     dev_free(p->g->lt.d_long_ids);
     p->g->lt.d_long_ids = dev_alloc<uint32_t>(count);
     p->g->lt.cap_ids = count;
   }
-  P.long_ids = p->g->lt.d_long_ids;
   hipLaunchKernelGGL(reset_long_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
   const uint32_t blocks = lt_grid<BT, CAP, SPILL, UNROLL>(count);
-  P.chunk = lt_chunk(p, count, tune, blocks);
-  P.claim = lt_claim(count, blocks);
   const uint64_t spill_words = static_cast<uint64_t>(blocks) * BT * SPILL;
   if (p->g->lt.cap_spill < spill_words) {
     dev_free(p->g->lt.d_spill);
     p->g->lt.d_spill = dev_alloc<uint32_t>(spill_words);
     p->g->lt.cap_spill = spill_words;
   }
-  P.spill = p->g->lt.d_spill;
+  LtParams P = lt_params(p, first, count);
+  P.chunk = lt_chunk(p, count, tune, blocks);
+  P.claim = lt_claim(count, blocks);
   RIP_HIP(hipEventRecord(p->ev0, p->stream));
   hipLaunchKernelGGL((lt_walk_kernel<BT, CAP, SPILL, UNROLL>), dim3(blocks), dim3(BT), 0,
                      p->stream, P);
@@ -696,32 +698,34 @@ uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,
   RIP_HIP(hipEventRecord(p->ev1, p->stream));  // the events time the walk kernel alone
   hipLaunchKernelGGL(lt_sort_kernel, dim3(1024), dim3(256), 0, p->stream, P);
   RIP_HIP(hipGetLastError());
-  uint32_t launches = 2;
-  unsigned long long nlong = 0;
-  RIP_HIP(hipMemcpyAsync(&nlong, p->d_ctl + kCtlLongCount, sizeof(nlong),
-                         hipMemcpyDeviceToHost, p->stream));
-  RIP_HIP(hipStreamSynchronize(p->stream));
-  if (nlong) {
-    // lanes of unbounded walks: a visited list (n words) and bitmap each
-    const uint64_t per_lane = 4ull * g->n + 4ull * ((g->n + 31) / 32);
-    const uint64_t lanes = std::max<uint64_t>(
-        1, std::min<uint64_t>({nlong, 256, (static_cast<uint64_t>(1) << 29) / per_lane}));
-    uint32_t *lists = dev_alloc<uint32_t>(lanes * g->n);
-    uint32_t *bms = dev_alloc<uint32_t>(lanes * ((g->n + 31) / 32));
-    RIP_HIP(hipMemsetAsync(bms, 0, sizeof(uint32_t) * lanes * ((g->n + 31) / 32),
-                           p->stream));
-    for (uint64_t lo = 0; lo < nlong; lo += lanes) {
-      const uint32_t blocks_l = static_cast<uint32_t>((lanes + 63) / 64);
-      hipLaunchKernelGGL(lt_long_kernel, dim3(blocks_l), dim3(64), 0, p->stream,
-                         P, static_cast<uint32_t>(nlong), static_cast<uint32_t>(lo),
-                         static_cast<uint32_t>(lanes), lists, bms);
-      RIP_HIP(hipGetLastError());
-      ++launches;
-    }
-    RIP_HIP(hipStreamSynchronize(p->stream));
-    dev_free(lists);
-    dev_free(bms);
+  return 2;
+}
+
+// Walks that outgrew the walk kernel's 128-entry lists (nlong of them, from
+// the pool's control words after lt_launch): one lane per walk with an HBM
+// list and bitmap each.  Returns the launches made.
+uint32_t lt_finish_long(rip_pool *p, uint64_t first, uint64_t count, uint64_t nlong) {
+  const rip_graph *g = p->g;
+  const LtParams P = lt_params(p, first, count);
+  // lanes of unbounded walks: a visited list (n words) and bitmap each
+  const uint64_t per_lane = 4ull * g->n + 4ull * ((g->n + 31) / 32);
+  const uint64_t lanes = std::max<uint64_t>(
+      1, std::min<uint64_t>({nlong, 256, (static_cast<uint64_t>(1) << 29) / per_lane}));
+  uint32_t *lists = dev_alloc<uint32_t>(lanes * g->n);
+  uint32_t *bms = dev_alloc<uint32_t>(lanes * ((g->n + 31) / 32));
+  RIP_HIP(hipMemsetAsync(bms, 0, sizeof(uint32_t) * lanes * ((g->n + 31) / 32), p->stream));
+  uint32_t launches = 0;
+  for (uint64_t lo = 0; lo < nlong; lo += lanes) {
+    const uint32_t blocks_l = static_cast<uint32_t>((lanes + 63) / 64);
+    hipLaunchKernelGGL(lt_long_kernel, dim3(blocks_l), dim3(64), 0, p->stream, P,
+                       static_cast<uint32_t>(nlong), static_cast<uint32_t>(lo),
+                       static_cast<uint32_t>(lanes), lists, bms);
+    RIP_HIP(hipGetLastError());
+    ++launches;
   }
+  RIP_HIP(hipStreamSynchronize(p->stream));
+  dev_free(lists);
+  dev_free(bms);
   return launches;
 }
 
