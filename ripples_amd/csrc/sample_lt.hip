@@ -62,7 +62,7 @@ constexpr int kLtBlocksPerCu = 2;
 // Most sample ids a wave claims from the work counter at a time; the batch
 // (LtParams::claim) is at least a wave's worth, since one refill hands out
 // up to 64 ids.
-constexpr unsigned long long kClaim = 1024;
+constexpr unsigned long long kClaim = 512;
 
 // LT threshold of the walk at a vertex (ja, jb from its record): U = k * 2^-24.
 __device__ inline uint32_t lt_threshold(uint64_t ja, uint64_t jb, uint64_t z) {
@@ -194,7 +194,7 @@ __device__ inline void wave_sort(uint32_t &x0, uint32_t &x1, uint32_t w) {
 // together.  A walk's visited list lives in LDS for its first CAP vertices
 // and in a per-lane HBM spill area for the next SPILL, with a 128-bit filter
 // in registers so the list is searched only when the filter has the vertex;
-// longer walks are handed to lt_long_kernel.  Finished lists up to CAP are
+// longer walks are handed to lt_long_kernel.  Finished lists up to 16 are
 // sorted four at a time in 16-lane register groups; longer ones are copied
 // by the whole wave and sorted by lt_sort_kernel.
 // The compiler's own register budget (about 100 registers, four waves per
@@ -203,7 +203,8 @@ __device__ inline void wave_sort(uint32_t &x0, uint32_t &x1, uint32_t w) {
 template <int BT, int CAP, int SPILL, int UNROLL>
 __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   static_assert(UNROLL % 4 == 0, "groups of four weights per load");
-  static_assert(CAP <= 16 && CAP + SPILL <= 128, "16-lane group sort and 128-key wave sort");
+  static_assert(CAP >= 16 && CAP + SPILL <= 128, "16-lane group sort and 128-key wave sort");
+  constexpr uint32_t kSort = 16;  // lists this short are sorted in registers
   __shared__ uint32_t s_list[CAP * BT];
   // s_pow[r] = M^(2r): from the stream start of a batch's first sample id to
   // that of the id r places later (element 0 of sample s is M^(2s+2)(seed))
@@ -428,7 +429,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         } else if (fail) {
           P.set_len[s] = kLenFail;
         } else {
-          if (len > static_cast<uint32_t>(CAP)) {
+          if (len > kSort) {
             // written unsorted below; lt_sort_kernel sorts it in place
             const unsigned long long idx = atomicAdd(&P.ctl[kCtlSortCount], 1ull);
             P.long_ids[P.count - 1 - idx] = static_cast<uint32_t>(gid);
@@ -439,9 +440,9 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         }
       }
       const uint32_t tid0 = threadIdx.x & ~63u;
-      // lists of at most CAP ids: four at a time, one per 16-lane group,
+      // lists of at most kSort ids: four at a time, one per 16-lane group,
       // sorted in registers and written in order
-      uint64_t sm = __ballot(emit && !fail && len <= static_cast<uint32_t>(CAP));
+      uint64_t sm = __ballot(emit && !fail && len <= kSort);
       const uint32_t g = lane >> 4, gi = lane & 15u;
       while (sm) {
         uint32_t pick = 64;  // this group's lane, 64 = none
@@ -468,7 +469,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         if (have) P.data[off_g + gi] = x;
       }
       // longer lists: copied by the whole wave, sorted by lt_sort_kernel
-      uint64_t lm = __ballot(emit && !fail && len > static_cast<uint32_t>(CAP));
+      uint64_t lm = __ballot(emit && !fail && len > kSort);
       while (lm) {
         const uint32_t L = __ffsll(static_cast<unsigned long long>(lm)) - 1;
         lm &= lm - 1;
@@ -615,7 +616,11 @@ __global__ void reset_long_kernel(unsigned long long *ctl) {
 
 }  // namespace
 
-constexpr int kLtBT = 256, kLtCap = 16, kLtSpill = 112, kLtUnroll = 16;
+// 64 list entries per lane in LDS (64 KiB per block, two blocks per CU):
+// long walks fill the 128-bit filter, so their revisit checks search the
+// list most steps, and entries past the LDS part are HBM reads (16 entries
+// in LDS measured 0.93 ms per launch on R-MAT scale 20, 64 entries 0.90 ms).
+constexpr int kLtBT = 256, kLtCap = 64, kLtSpill = 64, kLtUnroll = 16;
 
 // Ids per claim: an even share of the launch per wave, so every wave gets
 // one batch, rounded up to whole waves of lanes.
