@@ -153,8 +153,43 @@ __device__ inline uint32_t lt_cross(const LtRec *lrec, uint32_t v, const uint32_
   return lo + c;
 }
 
-// Bit of a vertex in a walk's 128-bit visited filter.
-__device__ inline uint32_t lt_filter_bit(uint32_t v) { return (v * 0x9E3779B1u) >> 25; }
+// A walk's visited filter: 512 bits in eight 64-bit registers (a 64-vertex
+// walk meets a false positive about one step in nine, where 128 bits gave
+// one in three and sent long walks to their list search most steps).  Words
+// are picked with masks on named members: selects between them were turned
+// into an indexed stack array by the compiler.
+struct LtFilter {
+  uint64_t w0, w1, w2, w3, w4, w5, w6, w7;
+};
+
+__device__ inline uint32_t lt_filter_bit(uint32_t v) { return (v * 0x9E3779B1u) >> 23; }
+
+// all ones when word w is the one, else zero
+__device__ inline uint64_t lt_word_mask(uint32_t q, uint32_t w) {
+  return 0ull - static_cast<uint64_t>(q == w);
+}
+
+__device__ inline void lt_filter_add(LtFilter &f, uint32_t v) {
+  const uint32_t b = lt_filter_bit(v), q = b >> 6;
+  const uint64_t bit = 1ull << (b & 63u);
+  f.w0 |= bit & lt_word_mask(q, 0);
+  f.w1 |= bit & lt_word_mask(q, 1);
+  f.w2 |= bit & lt_word_mask(q, 2);
+  f.w3 |= bit & lt_word_mask(q, 3);
+  f.w4 |= bit & lt_word_mask(q, 4);
+  f.w5 |= bit & lt_word_mask(q, 5);
+  f.w6 |= bit & lt_word_mask(q, 6);
+  f.w7 |= bit & lt_word_mask(q, 7);
+}
+
+__device__ inline bool lt_filter_has(const LtFilter &f, uint32_t v) {
+  const uint32_t b = lt_filter_bit(v), q = b >> 6;
+  const uint64_t word = (f.w0 & lt_word_mask(q, 0)) | (f.w1 & lt_word_mask(q, 1)) |
+                        (f.w2 & lt_word_mask(q, 2)) | (f.w3 & lt_word_mask(q, 3)) |
+                        (f.w4 & lt_word_mask(q, 4)) | (f.w5 & lt_word_mask(q, 5)) |
+                        (f.w6 & lt_word_mask(q, 6)) | (f.w7 & lt_word_mask(q, 7));
+  return ((word >> (b & 63u)) & 1ull) != 0;
+}
 
 // Ascending bitonic sort of the first w keys of a wave (w = 32, 64 or 128,
 // wave-uniform), held two per lane as a[lane], a[lane + 64]; for w < 128
@@ -192,7 +227,7 @@ __device__ inline void wave_sort(uint32_t &x0, uint32_t &x1, uint32_t w) {
 // takes one step of every busy lane (a breakpoint search), or scans up to
 // UNROLL in-edges of a row without breakpoints with 16-byte loads issued
 // together.  A walk's visited list lives in LDS for its first CAP vertices
-// and in a per-lane HBM spill area for the next SPILL, with a 128-bit filter
+// and in a per-lane HBM spill area for the next SPILL, with a 512-bit filter
 // in registers so the list is searched only when the filter has the vertex;
 // longer walks are handed to lt_long_kernel.  Finished lists up to 16 are
 // sorted four at a time in 16-lane register groups; longer ones are copied
@@ -246,22 +281,12 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
   // the batch [claim_base, claim_hi) and the stream start of its first id
   uint64_t claim_base = 0, z_base = 0;
   uint32_t len = 0, e = 0, end = 0, rflags = 0, kthr = 0, ridx = 0;
-  uint64_t filt0 = 0, filt1 = 0;  // visited filter, bits 0..63 and 64..127
+  LtFilter filt{};  // visited filter
   // the wave's reserved output [out_cur, out_end) (wave-uniform): finished
   // walks are written there, and a new chunk is reserved only when they do
   // not fit, so waves rarely meet on the pool's one allocation counter
   unsigned long long out_cur = 0, out_end = 0;
-  auto filter_add = [&](uint32_t v) {
-    const uint32_t b = lt_filter_bit(v);
-    if (b < 64)
-      filt0 |= 1ull << b;
-    else
-      filt1 |= 1ull << (b - 64);
-  };
-  auto filter_has = [&](uint32_t v) -> bool {
-    const uint32_t b = lt_filter_bit(v);
-    return ((b < 64 ? filt0 >> b : filt1 >> (b - 64)) & 1ull) != 0;
-  };
+
   float t = 0.0f;                 // float chain of rows without breakpoints
   // work counters: in-edges the subtraction loop scans, index nodes and
   // breakpoint blocks read, vertices written out (every visited vertex is)
@@ -320,8 +345,8 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
           const uint32_t root = uniform_index(z, P.n);
           list[0] = root;
           len = 1;
-          filt0 = filt1 = 0;
-          filter_add(root);
+          filt = LtFilter{};
+          lt_filter_add(filt, root);
           enter(root);
           busy = true;
         } else {
@@ -379,7 +404,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         if (found) u = P.ids[e - 1];
       }
       if (found) {
-        const bool seen = filter_has(u) && listed(u, len);
+        const bool seen = lt_filter_has(filt, u) && listed(u, len);
         if (seen) {
           done = true;
         } else if (len == CAP + SPILL) {
@@ -391,7 +416,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
           else
             spill[len - CAP] = u;
           ++len;
-          filter_add(u);
+          lt_filter_add(filt, u);
           enter(u);
         }
       }
@@ -617,7 +642,7 @@ __global__ void reset_long_kernel(unsigned long long *ctl) {
 }  // namespace
 
 // 64 list entries per lane in LDS (64 KiB per block, two blocks per CU):
-// long walks fill the 128-bit filter, so their revisit checks search the
+// long walks fill their filter, so their revisit checks search the
 // list most steps, and entries past the LDS part are HBM reads (16 entries
 // in LDS measured 0.93 ms per launch on R-MAT scale 20, 64 entries 0.90 ms).
 constexpr int kLtBT = 256, kLtCap = 64, kLtSpill = 64, kLtUnroll = 16;
