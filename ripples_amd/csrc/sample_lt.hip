@@ -191,6 +191,33 @@ __device__ inline bool lt_filter_has(const LtFilter &f, uint32_t v) {
   return ((word >> (b & 63u)) & 1ull) != 0;
 }
 
+// x of lane (l ^ j) for j = 1, 2, 4 or 8, within each 16-lane row, from DPP
+// moves (a few cycles each, where a general shuffle goes through the LDS
+// crossbar): quad permutes give xor 1 and 2; xor 4 is the half-row mirror
+// (l -> 7 - l within 8) followed by reversing each quad, and xor 8 the row
+// mirror (l -> 15 - l) followed by the half-row mirror.
+__device__ inline uint32_t xor16(uint32_t x, uint32_t j) {
+  constexpr int kQuadSwap1 = 0xB1;    // quad_perm [1, 0, 3, 2]
+  constexpr int kQuadSwap2 = 0x4E;    // quad_perm [2, 3, 0, 1]
+  constexpr int kQuadReverse = 0x1B;  // quad_perm [3, 2, 1, 0]
+  constexpr int kRowMirror = 0x140;
+  constexpr int kRowHalfMirror = 0x141;
+  const int v = static_cast<int>(x);
+  int r;
+  if (j == 1) {
+    r = __builtin_amdgcn_mov_dpp(v, kQuadSwap1, 0xF, 0xF, false);
+  } else if (j == 2) {
+    r = __builtin_amdgcn_mov_dpp(v, kQuadSwap2, 0xF, 0xF, false);
+  } else if (j == 4) {
+    r = __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(v, kRowHalfMirror, 0xF, 0xF, false),
+                                 kQuadReverse, 0xF, 0xF, false);
+  } else {
+    r = __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(v, kRowMirror, 0xF, 0xF, false),
+                                 kRowHalfMirror, 0xF, 0xF, false);
+  }
+  return static_cast<uint32_t>(r);
+}
+
 // Ascending bitonic sort of the first w keys of a wave (w = 32, 64 or 128,
 // wave-uniform), held two per lane as a[lane], a[lane + 64]; for w < 128
 // only x0 takes part, and for w == 32 lanes 32..63 sort a sequence of their
@@ -486,7 +513,7 @@ __global__ void __launch_bounds__(BT) lt_walk_kernel(LtParams P) {
         for (uint32_t k = 2; k <= 16; k <<= 1) {
 #pragma unroll
           for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            const uint32_t y = __shfl_xor(x, static_cast<int>(j), 64);
+            const uint32_t y = xor16(x, j);
             const bool lower = (gi & j) == 0, asc = (gi & k) == 0;
             x = (lower == asc) ? min(x, y) : max(x, y);
           }
