@@ -64,6 +64,7 @@ struct IcParams {
   uint64_t hub_cap;
   uint32_t jlevels;     // 6-bit digits of the run index covered by the jump tables
   const uint32_t *sample_ids;  // optional: local index -> sample index (minus first)
+  uint32_t precheck;    // read visited masks before their atomics (giant components)
 };
 
 __device__ inline uint64_t sample_id(const IcParams &P, uint64_t local) {
@@ -72,12 +73,22 @@ __device__ inline uint64_t sample_id(const IcParams &P, uint64_t local) {
 
 constexpr int kJumpLevels = 5;
 
-// Mark (u, bits) discovered.  Every lane of the wave must call it.
-__device__ inline void discover(const IcParams &P, unsigned long long *vp,
-                                uint32_t *bm0, uint32_t *bm1, uint32_t u,
-                                unsigned long long bits, uint32_t *s_union,
-                                uint32_t *s_qn, uint32_t *qnext) {
+// Mark (u, bits) discovered.  Every lane of the wave must call it.  With
+// precheck the visited mask is read first and the atomic skipped when no
+// colour is new: within a group masks only gain bits, so a read that lags
+// can only cost an extra atomic.  In a giant component most edges reach
+// vertices every colour has visited, and their atomics, many on the same hub
+// words, were most of the expansion time; elsewhere the read is one more
+// round trip.  The read is a device-scope atomic load, served by L2 like the
+// atomics: a plain load could meet a vector-L1 line from an earlier group,
+// whose masks have since been cleared.
+template <bool kPrecheck>
+__device__ inline void discover(unsigned long long *vp, uint32_t *bm0, uint32_t *bm1, uint32_t u,
+                                unsigned long long bits, uint32_t *s_union, uint32_t *s_qn,
+                                uint32_t *qnext) {
   bool first = false, push = false;
+  if (kPrecheck && bits)
+    bits &= ~__hip_atomic_load(&vp[2ull * u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (bits) {
     unsigned long long old = atomicOr(&vp[2ull * u], bits);
     unsigned long long fresh = bits & ~old;
@@ -97,59 +108,7 @@ __device__ inline void discover(const IcParams &P, unsigned long long *vp,
   wave_append(push, u, s_qn, qnext);
 }
 
-// discover() for N (vertex, colours) pairs of each lane at once: every kind
-// of atomic is issued for all N before any of its results is used, so a run
-// waits for three rounds of atomics rather than 3N.  A lane's pairs that
-// share a vertex are ordered as in N calls: atomics from one lane to one
-// address complete in issue order.  Within a group visited masks only gain
-// bits, so a read that already shows every colour makes the atomic
-// unnecessary (a read that lags can only cost an extra atomic): in a giant
-// component most edges lead to vertices every colour has reached, and their
-// atomics, many on the same hub words, were most of the expansion time.  The
-// read is a device-scope atomic load, served by L2 like the atomics: a plain
-// load could meet a vector-L1 line from an earlier group, whose masks have
-// since been cleared.
-template <int N>
-__device__ inline void discover_n(unsigned long long *vp, uint32_t *bm0, uint32_t *bm1,
-                                  const uint32_t *u, const unsigned long long *bits,
-                                  uint32_t *s_union, uint32_t *s_qn, uint32_t *qnext) {
-  unsigned long long need[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-    need[i] = bits[i] ? bits[i] & ~__hip_atomic_load(&vp[2ull * u[i]], __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT)
-                      : 0ull;
-  unsigned long long fresh[N];
-  bool first[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const unsigned long long old = need[i] ? atomicOr(&vp[2ull * u[i]], need[i]) : ~0ull;
-    fresh[i] = need[i] & ~old;
-    first[i] = fresh[i] != 0 && old == 0;
-  }
-  bool push[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const unsigned long long pold = fresh[i] ? atomicOr(&vp[2ull * u[i] + 1], fresh[i]) : 1ull;
-    push[i] = fresh[i] != 0 && pold == 0;
-  }
-  uint32_t o0[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-    o0[i] = first[i] ? atomicOr(&bm0[u[i] >> 5], 1u << (u[i] & 31)) : 1u;
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-    if (first[i] && o0[i] == 0) atomicOr(&bm1[u[i] >> 10], 1u << ((u[i] >> 5) & 31));
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const uint64_t fm = __ballot(first[i]);
-    if (fm && lane_id() == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(fm)) - 1))
-      atomicAdd(s_union, static_cast<uint32_t>(__popcll(fm)));
-    wave_append(push[i], u[i], s_qn, qnext);
-  }
-}
-
-template <int BT, int RUN>
+template <int BT, int RUN, bool kPrecheck>
 __global__ void __launch_bounds__(BT)
 ic_fused_kernel(IcParams P) {
   constexpr int NW = BT / 64;
@@ -223,7 +182,7 @@ ic_fused_kernel(IcParams P) {
     }
     __syncthreads();
     if (tid < 64)
-      discover(P, vp, bm0, bm1, root, rbits, &s_union, &s_qn[0], q0);
+      discover<kPrecheck>(vp, bm0, bm1, root, rbits, &s_union, &s_qn[0], q0);
 
     // ---------------------------------------------------------- BFS levels
     // Light levels expand the frontier queue; rows of at least heavy_deg
@@ -330,11 +289,9 @@ ic_fused_kernel(IcParams P) {
             for (int i = 0; i < RUN; ++i)
               if (acc[i]) src[i] = P.ids[e0 + i];
           }
-          // in groups of at most eight pairs (registers)
-          constexpr int G = RUN < 8 ? RUN : 8;
 #pragma unroll
-          for (int i0 = 0; i0 < RUN; i0 += G)
-            discover_n<G>(vp, bm0, bm1, src + i0, acc + i0, &s_union, &s_qn[lnext], qnext);
+          for (int i = 0; i < RUN; ++i)
+            discover<kPrecheck>(vp, bm0, bm1, src[i], acc[i], &s_union, &s_qn[lnext], qnext);
         }
         __syncthreads();
       }
@@ -431,8 +388,10 @@ ic_fused_kernel(IcParams P) {
 
 template <int BT, int RUN>
 void launch_one(const IcParams &P, uint32_t slots, hipStream_t stream) {
-  hipLaunchKernelGGL((ic_fused_kernel<BT, RUN>), dim3(slots), dim3(BT), 0,
-                     stream, P);
+  if (P.precheck)
+    hipLaunchKernelGGL((ic_fused_kernel<BT, RUN, true>), dim3(slots), dim3(BT), 0, stream, P);
+  else
+    hipLaunchKernelGGL((ic_fused_kernel<BT, RUN, false>), dim3(slots), dim3(BT), 0, stream, P);
 }
 
 template <int BT>
@@ -452,7 +411,7 @@ __global__ void reset_counter_kernel(unsigned long long *ctl) {
 // Workgroups of the persistent kernel that fit on one CU at once.
 template <int BT, int RUN>
 int occupancy_of() {
-  return blocks_per_cu(reinterpret_cast<const void *>(&ic_fused_kernel<BT, RUN>), BT);
+  return blocks_per_cu(reinterpret_cast<const void *>(&ic_fused_kernel<BT, RUN, false>), BT);
 }
 
 template <int BT>
@@ -601,6 +560,10 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
   RIP_REQUIRE((max_runs >> (6 * levels)) == 0, -1, "graph: row too long for the jump tables");
   P.jlevels = levels;
   P.sample_ids = ids;
+  // the large-set fallback walks giant components, where most edges reach
+  // vertices every colour has visited; a whole launch of mostly small sets
+  // (weighted cascade) gains nothing from the extra read
+  P.precheck = ids != nullptr;
   const uint32_t slots =
       static_cast<uint32_t>(std::min<uint64_t>(cfg.slots, P.ngroups));
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
