@@ -72,6 +72,26 @@ def test_mixed_queries_equal_solo_runs():
 
 
 @pytest.mark.gpu
+def test_dense_pools_served_equal_solo_runs():
+    """R-MAT scale 16 with U[0,0.1] weights: a fifth of the sets reach the
+    giant component, so the pools take the dense selector (no inverted index)
+    and the colour-group fallback; served answers still equal solo runs."""
+    g = R.Graph(*rmat_csr(16, R.W_UNIFORM, 0.1, 3), model=R.IC)
+    srv = R.Server(slots=2, workers=2)
+    gid = srv.add_graph(g)
+    plan = [(50, 0.5, 3), (10, 0.5, 3), (20, 0.7, 4)]
+    qids = [srv.submit(gid, k, eps, seed) for k, eps, seed in plan]
+    for (k, eps, seed), q in zip(plan, qids):
+        seeds, rec, _ = srv.poll(q, wait_ms=-1)
+        solo_seeds, solo = R.imm(g, k, eps, seed=seed)
+        assert seeds.tolist() == solo_seeds.tolist(), (k, eps, seed)
+        for key in SOLO_KEYS:
+            assert rec[key] == solo[key], (k, eps, seed, key)
+    srv.close()
+    g.close()
+
+
+@pytest.mark.gpu
 def test_budget_and_single_slot_give_same_results():
     """Admission limits and eviction change the schedule, never the answers."""
     g = R.Graph(*rmat_csr(10, R.W_UNIFORM, 0.1), model=R.IC)
