@@ -150,6 +150,7 @@ struct rip_pool {
   uint64_t num_elements = 0;  // sum of set sizes: used_data less the gaps the
                               // LT sampler leaves after its waves' output chunks
   unsigned long long *d_ctl = nullptr;  // control words, see pool.hip
+  unsigned long long *h_ctl = nullptr;  // pinned host copy of the control words
   rip::SelectScratch sel;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_kernel_ms = 0.0;

@@ -179,6 +179,10 @@ void pool_create(rip_pool *p, rip_graph *g, uint64_t seed) {
   RIP_HIP(hipEventCreate(&p->ev0));
   RIP_HIP(hipEventCreate(&p->ev1));
   p->d_ctl = dev_alloc<unsigned long long>(kCtlWords);
+  // pinned, so reading the control words back after a launch is one DMA
+  // rather than a copy staged through a driver buffer
+  RIP_HIP(hipHostMalloc(reinterpret_cast<void **>(&p->h_ctl),
+                        sizeof(unsigned long long) * kCtlWords, hipHostMallocDefault));
   RIP_HIP(hipMemsetAsync(p->d_ctl, 0, sizeof(unsigned long long) * kCtlWords, p->stream));
   RIP_HIP(hipStreamSynchronize(p->stream));
 }
@@ -201,6 +205,8 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->sel.d_partial);
   dev_free(p->sel.d_args);
   if (p->sel.graph) (void)hipGraphExecDestroy(p->sel.graph);
+  if (p->h_ctl) (void)hipHostFree(p->h_ctl);
+  p->h_ctl = nullptr;
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -313,15 +319,16 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
     else
       launches = lt_launch(p, first, count, tune);
     launch_sum_u32(p->d_len + first, count, p->d_ctl + kCtlElems, p->stream);
-    unsigned long long ctl[kCtlWords];
-    RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, p->stream));
+    unsigned long long *ctl = p->h_ctl;
+    const size_t ctl_bytes = sizeof(unsigned long long) * kCtlWords;
+    RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, ctl_bytes, hipMemcpyDeviceToHost, p->stream));
     RIP_HIP(hipStreamSynchronize(p->stream));
     if (p->g->model == RIP_LT && ctl[kCtlOverflow] == 0 && ctl[kCtlLongCount] != 0) {
       // walks longer than the walk kernel's lists, then the sizes again
       launches += lt_finish_long(p, first, count, ctl[kCtlLongCount]);
       hipLaunchKernelGGL(zero_elems_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
       launch_sum_u32(p->d_len + first, count, p->d_ctl + kCtlElems, p->stream);
-      RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, p->stream));
+      RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, ctl_bytes, hipMemcpyDeviceToHost, p->stream));
       RIP_HIP(hipStreamSynchronize(p->stream));
     }
     if (ctl[kCtlOverflow] == 0) {
