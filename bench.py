@@ -1,29 +1,42 @@
 #!/usr/bin/env python3
 """IMM hot-path benchmark on one MI355X; prints one JSON line.
 
-Workload (default): IMM under the LT model on a directed R-MAT graph of scale
-20 (2^20 vertices, edge factor 16, a/b/c/d = .57/.19/.19/.05, deduplicated,
-no self loops), in-weights drawn U(0,1] and normalised per target to 0.9,
-k = 100, epsilon = 0.13.  A step is one sampling pass of `batch` RRR sets
-(batch = the theta the IMM run ends with) into a reset HBM pool; the graph
-is resident in HBM before the timed region starts.  IMM wall time is one full
-rip_imm call (theta estimation + final sampling + selection).
+Workloads (R-MAT a/b/c/d = .57/.19/.19/.05, edge factor 16, deduplicated, no
+self loops, R-MAT seed = scale; weights from lcg64(seed).split(2,0)):
+  rmat24_ic  IMM IC, scale 24, w ~ U[0,0.02], k=100, eps=0.1
+  rmat22_wc  IMM IC, scale 22, weighted cascade w = 1/indeg, k=200, eps=0.1
+  rmat20_lt  IMM LT, scale 20, in-weights U(0,1] normalised to 0.9, k=100,
+             eps=0.13
+  rmat16_ic  IMM IC, scale 16, w ~ U[0,0.1], k=50, eps=0.5
+The headline line is the largest workload that runs on one GPU.  A step is
+one sampling pass of `batch` RRR sets (batch = the theta its IMM run ends
+with) into a reset HBM pool; the graph is resident in HBM before the timed
+region starts.  IMM wall time is one full rip_imm call (theta estimation +
+final sampling + selection) after a warm-up call.
 
-Serving (the "serving" object): IMM queries on resident R-MAT graphs (scales
-18/20/22; per scale one LT graph with in-weights normalised to 0.9 and one IC
-graph, U[0,0.1] at scale 18 and weighted cascade above, where U[0,0.1] pools
-outgrow HBM), k in {10,50,100,200}, epsilon in
-{0.5,0.2,0.13}, seed = arrival counter, answered by the continuous-batching
-server, against the same queries solved back to back with rip_imm: queries/s
-with every query submitted at once, and p50/p99 latency under Poisson arrivals
-(the back-to-back figures are those of a FIFO queue in front of the measured
-solve times), once with seed = arrival counter (no stream shared) and once
-with seed 0 for every query (the reference imm tool's fixed seeding, so
-queries on a graph share its RRR sets).  Every served answer is checked
+Roofline: the dominant kernel's algorithmic bytes per launch (the bytes its
+own algorithm reads and writes, counted on the device: IC gap samplers 64 per
+row expanded, 4 per candidate threshold of rows with mixed thresholds, 4 per
+live in-neighbour, 4 per output id; LT breakpoint walk 32 per visited vertex,
+64 per index node or breakpoint block searched, 8 per output id) over its
+launch time from HIP events on the pool's stream, against 8 TB/s.  The
+reference's per-edge figures (SURVEY 8d formulas) are reported beside it,
+labelled, never as the fraction.
+
+CPU baselines are the oracle's OpenMP restatements of the reference's CPU
+algorithms (IC: the 64-colour BatchedBFSNeighborColor sweep,
+batched_add_rrrset.h:184-257; LT: one walk per sample), timed on a bounded
+sample of the same workload on this host with the thread share it is given
+(OMP_NUM_THREADS, else every CPU of the process's affinity); CPU IMM wall
+times (orc_imm) for the smaller workloads.
+
+Serving (the "serving" object): IMM queries on resident R-MAT graphs
+(scales 18/20/22), k in {10,50,100,200}, eps in {0.5,0.2,0.13}, answered by
+the continuous-batching server against the same queries solved back to back:
+queries/s with every query submitted at once and p50/p99 latency under
+Poisson arrivals, with seed = arrival counter (the served workload's own
+seeding) and with seed 0 for every query.  Every served answer is checked
 against its solo solve.
-
-The CPU baseline is the oracle's OpenMP restatement of the reference's CPU
-path, timed on a bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -38,15 +51,15 @@ import ripples_amd as R
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    "rmat20_lt": dict(scale=20, model=R.LT, scheme=R.W_LT_NORM, wscale=0.9, k=100,
-                      eps=0.13, desc="IMM LT, R-MAT scale 20, in-weights U(0,1] "
-                                     "normalised to 0.9, k=100, eps=0.13"),
-    "rmat22_wc": dict(scale=22, model=R.IC, scheme=R.W_CASCADE, wscale=1.0, k=200,
-                      eps=0.1, desc="IMM IC, R-MAT scale 22, weighted cascade "
-                                    "1/indeg, k=200, eps=0.1"),
-    "rmat16_ic": dict(scale=16, model=R.IC, scheme=R.W_UNIFORM, wscale=0.1, k=50,
-                      eps=0.5, desc="IMM IC, R-MAT scale 16, w ~ U[0,0.1], k=50, "
-                                    "eps=0.5"),
+    "rmat24_ic": dict(scale=24, model=R.IC, scheme=R.W_UNIFORM, wscale=0.02, k=100, eps=0.1,
+                      desc="IMM IC, R-MAT scale 24, w ~ U[0,0.02], k=100, eps=0.1"),
+    "rmat22_wc": dict(scale=22, model=R.IC, scheme=R.W_CASCADE, wscale=1.0, k=200, eps=0.1,
+                      desc="IMM IC, R-MAT scale 22, weighted cascade 1/indeg, k=200, eps=0.1"),
+    "rmat20_lt": dict(scale=20, model=R.LT, scheme=R.W_LT_NORM, wscale=0.9, k=100, eps=0.13,
+                      desc="IMM LT, R-MAT scale 20, in-weights U(0,1] normalised to 0.9, "
+                           "k=100, eps=0.13"),
+    "rmat16_ic": dict(scale=16, model=R.IC, scheme=R.W_UNIFORM, wscale=0.1, k=50, eps=0.5,
+                      desc="IMM IC, R-MAT scale 16, w ~ U[0,0.1], k=50, eps=0.5"),
 }
 
 
@@ -54,49 +67,217 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or aff or 1)
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "threads_used": threads}
+
+
 def build_graph(wl, seed):
     t = time.time()
     s, d = R.rmat_edges(wl["scale"])
     n = 1 << wl["scale"]
     index, edges, w = R.csr_from_edges(n, s, d, wl["scheme"], wl["wscale"], seed)
-    log("graph scale %d: n=%d m=%d built in %.1fs" % (wl["scale"], n, len(edges),
-                                                         time.time() - t))
+    log("graph scale %d: n=%d m=%d built on the host in %.1fs" % (
+        wl["scale"], n, len(edges), time.time() - t))
     return index, edges, w
 
 
-def ic_union_bytes(index, off, ver):
-    """B_IC(L) = sum over the union of (8 + 8 indeg) + 4 per output id."""
+def lt_search_bytes(stats, n_elem):
+    """What the LT breakpoint walk itself moves: the 32 bytes of a vertex
+    record it uses per visited vertex, one 64-byte index node or breakpoint
+    block per search probe, 4 for the chosen in-neighbour and 4 per output id."""
+    return 32 * n_elem + 64 * stats["coins"] + 8 * n_elem
+
+
+def ic_reference_bytes(index, off, ver):
+    """SURVEY 8d's per-edge formula: sum over the union of the launch's sets of
+    (8 + 8 indeg) + 4 per output id (every in-edge of every expanded row read
+    once, as the reference's per-edge coins need)."""
     u = np.unique(ver)
     deg = (index[u.astype(np.int64) + 1] - index[u.astype(np.int64)]).astype(np.int64)
     return int(8 * len(u) + 8 * deg.sum() + 4 * len(ver))
 
 
-def ic_plain_bytes(index, ver):
-    """Sum over samples of sum over R_s of (8 + 8 indeg + 4): every sample
-    reading its own rows, the one-BFS-per-sample form."""
-    v = ver.astype(np.int64)
-    deg = (index[v + 1] - index[v]).astype(np.int64)
-    return int(12 * len(v) + 8 * deg.sum())
-
-
-def cpu_baseline(wl, index, edges, w, seed, batch, target_s):
+def cpu_sample_rate(wl, index, edges, w, seed, target_s, threads):
+    """Oracle restatement of the reference's CPU sampler on a bounded sample."""
     from oracle import oracle as O
-    # the host's CPU share: OMP_NUM_THREADS where the environment sets it,
-    # else at most 16 threads
-    threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
-    probe = max(1000, min(batch, 20000))
+    algo = O.ALGO_BATCHED64 if wl["model"] == R.IC else O.ALGO_BFS
+    probe = 640 if algo == O.ALGO_BATCHED64 else 20000
     t = time.time()
-    O.sample(wl["model"], index, edges, w, seed, 0, probe, threads=threads)
+    O.sample(wl["model"], index, edges, w, seed, 0, probe, algo=algo, threads=threads)
     dt = max(time.time() - t, 1e-3)
     count = int(max(probe, probe * target_s / dt))
+    if algo == O.ALGO_BATCHED64:
+        count = (count + 63) // 64 * 64
     t = time.time()
-    O.sample(wl["model"], index, edges, w, seed, 0, count, threads=threads)
+    O.sample(wl["model"], index, edges, w, seed, 0, count, algo=algo, threads=threads)
     dt = time.time() - t
-    return {"value": count / dt, "unit": "RRR sets/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d RRR sets (sample ids 0..%d) of the same graph and seed, "
-                      "oracle FIFO reverse BFS/walk per sample, OpenMP over "
-                      "samples, %.1f s" % (count, count - 1, dt)}
+    what = ("64-colour BatchedBFSNeighborColor sweeps (batched_add_rrrset.h:184-257), "
+            "OpenMP over batches of 64" if algo == O.ALGO_BATCHED64 else
+            "one walk per sample (add_rrrset.h:136-150), OpenMP over samples")
+    return {"value": count / dt, "unit": "RRR sets/s", "cores": threads, "kind": "port",
+            "sample": "%d RRR sets (sample ids 0..%d) of the same graph and seed, oracle %s, "
+                      "%.1f s" % (count, count - 1, what, dt)}
+
+
+def cpu_imm_wall(wl, index, edges, w, seed, threads):
+    from oracle import oracle as O
+    t = time.time()
+    seeds, rec = O.imm(wl["model"], index, edges, w, wl["k"], wl["eps"], seed, threads=threads)
+    return time.time() - t, seeds, rec
+
+
+def spread_report(g, wl, index, edges, w, seeds, seed, gpu_replicas, cpu_replicas, threads):
+    """Monte Carlo spread of the chosen seeds (diffusion_simulation.h:73-178):
+    the device simulator over gpu_replicas, the oracle's restatement over the
+    first cpu_replicas of the same replica streams."""
+    from oracle import oracle as O
+    t = time.time()
+    act, _ = g.spread(seeds, gpu_replicas, seed=seed)
+    gdt = time.time() - t
+    t = time.time()
+    eact, _ = O.spread(wl["model"], index, edges, w, seeds, seed, 0, cpu_replicas,
+                       threads=threads)
+    cdt = time.time() - t
+    same = bool(np.array_equal(act[:cpu_replicas], eact))
+    return {"replicas": gpu_replicas, "mean": float(act.mean()),
+            "stderr": float(act.std() / np.sqrt(len(act))), "device_s": gdt,
+            "oracle_replicas": cpu_replicas, "oracle_mean": float(eact.mean()),
+            "oracle_s": cdt, "replicas_equal_oracle": same}
+
+
+def sampling_steps(pool, batch, steps, warmup, tune):
+    for _ in range(warmup):
+        pool.reset()
+        pool.generate_rrr_sets(batch, tune=tune)
+    kms, calls, kstats = [], [], []
+    t0 = time.time()
+    for _ in range(steps):
+        pool.reset()
+        pool.generate_rrr_sets(batch, tune=tune)   # returns after the device finished
+        kms.append(pool.last_kernel_ms()[0])
+        calls.append(pool.last_call_ms())
+        kstats.append(pool.last_kernel_stats())
+    wall = time.time() - t0
+    return wall, kms, calls, kstats
+
+
+def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, fused_steps,
+            spread_replicas):
+    wl = WORKLOADS[name]
+    index, edges, w = build_graph(wl, seed)
+    t = time.time()
+    g = R.Graph(index, edges, w, model=wl["model"])
+    graph_create_s = time.time() - t
+    log("%s: rip_graph_create %.2fs (device layout%s)" % (
+        name, graph_create_s, ", LT breakpoints and index" if wl["model"] == R.LT else ""))
+    seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)   # warm-up
+    t = time.time()
+    seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)
+    imm_wall = time.time() - t
+    log("%s IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
+        name, rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
+    batch = batch or int(rec["Theta"])
+    pool = g.pool(seed)
+    wall, kms, calls, kstats = sampling_steps(pool, batch, steps, warmup, tune)
+    stats = pool.last_stats()
+    n_sets, n_elem = pool.size()
+    kernel_ms = float(np.mean(kms))
+    call_ms = float(np.mean(calls))
+    ks = {k: float(np.mean([s[k] for s in kstats])) for k in kstats[0]}
+    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel_ms": kernel_ms,
+            "call_ms": call_ms}
+    if wl["model"] == R.IC:
+        per_launch = ks["algo_bytes"]
+        roof["kernel"] = "ic_lane_kernel" if (tune or {}).get("sampler", 0) in (0, 3) \
+            else "ic_wave_kernel"
+        roof["algo_bytes_formula"] = ("64 per row expanded (vertex record) + 4 per candidate "
+                                      "edge of a row with mixed thresholds + 4 per live edge "
+                                      "(in-neighbour) + 4 per output id, counted on the device")
+        off, ver = pool.export()
+        ref = ic_reference_bytes(index, off, ver)
+        roof["reference_per_edge_bytes"] = ref
+        roof["reference_per_edge_note"] = (
+            "SURVEY 8d: sum over the union of the launch's sets of (8 + 8 indeg) + 4 per "
+            "output id, the bytes of one coin per in-edge; the gap sampler draws only the "
+            "candidates, so this figure is not the fraction's numerator")
+        del off, ver
+    else:
+        per_launch = lt_search_bytes(ks, n_elem)
+        roof["kernel"] = "lt_walk_kernel"
+        roof["algo_bytes_formula"] = ("32 per visited vertex (record) + 64 per index node or "
+                                      "breakpoint block searched + 4 per chosen in-neighbour + "
+                                      "4 per output id")
+        roof["reference_subtraction_scan_bytes"] = ks["algo_bytes"]
+        roof["reference_subtraction_scan_note"] = (
+            "SURVEY 8d: 8 per visited vertex + 8 per in-edge the reference's subtraction loop "
+            "scans + 4 per output id; the walk searches precomputed breakpoints instead")
+    roof["algo_bytes_per_launch"] = per_launch
+    achieved = per_launch / (kernel_ms * 1e-3) / 1e9
+    roof["achieved"] = achieved
+    roof["frac"] = achieved / HBM_PEAK_GBS
+    roof["traffic"] = measured_traffic(roof["kernel"], name)
+    log("%s sampling: %d x %d sets in %.3fs; main kernel %.3f ms, call %.3f ms; %.1f GB/s "
+        "algorithmic; kernel counters %s" % (name, steps, batch, wall, kernel_ms, call_ms,
+                                             achieved, ks))
+    res = {
+        "value": batch * steps / wall, "ms_per_step": wall / steps * 1e3,
+        "config": {"workload": name, "description": wl["desc"],
+                   "num_nodes": int(len(index) - 1), "num_edges": int(len(edges)),
+                   "batch_sets": batch, "k": wl["k"], "epsilon": wl["eps"],
+                   "parallelism": "single GPU"},
+        "imm_wall_s": imm_wall,
+        "imm": {"theta": rec["Theta"], "rounds": rec["Rounds"], "num_seeds": rec["NumSeeds"],
+                "final_f": rec["FinalF"], "estimation_ms": rec["ThetaEstimation"],
+                "generate_ms": rec["GenerateRRRSets"], "select_ms": rec["FindMostInfluentialSet"],
+                "rrr_elements": rec["RRRElements"]},
+        "graph_create_s": graph_create_s,
+        "pool": {"sets": n_sets, "elements": n_elem},
+        "kernel_counters": ks, "call_counters": stats,
+        "roofline": roof,
+    }
+    if host is not None and cpu_s > 0:
+        res["cpu_baseline"] = cpu_sample_rate(wl, index, edges, w, seed, cpu_s,
+                                              host["threads_used"])
+        log("%s cpu baseline: %.0f sets/s on %d threads" % (
+            name, res["cpu_baseline"]["value"], host["threads_used"]))
+    if with_cpu_imm and host is not None:
+        cw, cseeds, crec = cpu_imm_wall(wl, index, edges, w, seed, host["threads_used"])
+        res["cpu_imm"] = {"wall_s": cw, "threads": host["threads_used"], "kind": "port",
+                          "seeds_equal_gpu": cseeds.tolist() == seeds.tolist(),
+                          "theta": crec.theta}
+        log("%s cpu IMM: %.2fs, seeds equal: %s" % (name, cw, res["cpu_imm"]["seeds_equal_gpu"]))
+    if spread_replicas and host is not None:
+        res["spread"] = spread_report(g, wl, index, edges, w, seeds, seed, spread_replicas[0],
+                                      spread_replicas[1], host["threads_used"])
+        log("%s spread of the seeds: device %.1f (%d replicas), oracle %.1f (%d replicas)" % (
+            name, res["spread"]["mean"], spread_replicas[0], res["spread"]["oracle_mean"],
+            spread_replicas[1]))
+    if wl["model"] == R.IC and fused_steps:
+        # the same batch with one BFS per wave (the plain one-sample-per-BFS
+        # form) and with 64-colour groups for every sample: same sets
+        for label, kind in (("plain_wave", 1), ("colour_groups", 2)):
+            tz = dict(tune or {})
+            tz["sampler"] = kind
+            pw, pk, _, _ = sampling_steps(pool, batch, fused_steps, 1, tz)
+            res[label] = {"value": batch * fused_steps / pw, "ms_per_step": pw / fused_steps * 1e3,
+                          "kernel_ms": float(np.mean(pk))}
+            res[label]["batched_lanes_over_this"] = res["value"] / res[label]["value"]
+            log("%s %s: %.3f s per step; lane sampler %.2fx faster" % (
+                name, label, pw / fused_steps, res[label]["batched_lanes_over_this"]))
+    pool.close()
+    g.close()
+    return res
 
 
 def measured_traffic(kernel, workload):
@@ -111,115 +292,13 @@ def measured_traffic(kernel, workload):
     return None if entry is None else entry.get("hbm_bytes_per_launch")
 
 
-def measure(name, seed, steps, warmup, batch, tune, skip_imm, fused_steps=0):
-    wl = WORKLOADS[name]
-    index, edges, w = build_graph(wl, seed)
-    g = R.Graph(index, edges, w, model=wl["model"])
-    imm_wall, rec = None, None
-    if not skip_imm:
-        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)   # warm-up
-        t = time.time()
-        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)
-        imm_wall = time.time() - t
-        log("%s IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
-            name, rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
-    batch = batch or (int(rec["Theta"]) if rec else 1 << 20)
-    pool = g.pool(seed)
-    for _ in range(warmup):
-        pool.reset()
-        pool.generate_rrr_sets(batch, tune=tune)
-    kms, algo = [], []
-    t0 = time.time()
-    for _ in range(steps):
-        pool.reset()
-        pool.generate_rrr_sets(batch, tune=tune)   # returns after the device finished
-        kms.append(pool.last_kernel_ms()[0])
-        algo.append(pool.last_algo_bytes())
-    wall = time.time() - t0
-    stats = pool.last_stats()
-    n_sets, n_elem = pool.size()
-    kernel_ms = float(np.mean(kms))
-    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel_ms": kernel_ms}
-    if wl["model"] == R.IC:
-        off, ver = pool.export()
-        per_launch = ic_union_bytes(index, off, ver)
-        plain = ic_plain_bytes(index, ver)
-        roof["kernel"] = "ic_wave_kernel"
-        roof["algo_bytes_per_launch"] = per_launch
-        roof["algo_bytes_formula"] = ("sum over the union of the launch's sets of "
-                                      "(8 + 8 indeg) + 4 per output id")
-        roof["plain_form_bytes_per_launch"] = plain
-        roof["plain_form_achieved"] = plain / (kernel_ms * 1e-3) / 1e9
-        roof["edges_read"] = stats["edges"]
-    else:
-        per_launch = float(np.mean(algo))
-        roof["kernel"] = "lt_walk_kernel"
-        roof["algo_bytes_per_launch"] = per_launch
-        roof["algo_bytes_formula"] = ("sum over walks of 8 per visited vertex + 8 per "
-                                      "in-edge the subtraction loop scans + 4 per output id")
-        # what the breakpoint search itself reads: the 32-byte vertex record,
-        # one 64-byte node per index level or breakpoint block (the `coins`
-        # counter), 4 per chosen id, 4 per output id
-        search = 32 * n_elem + 64 * stats["coins"] + 8 * n_elem
-        roof["search_form_bytes_per_launch"] = search
-        roof["search_form_achieved"] = search / (kernel_ms * 1e-3) / 1e9
-        roof["note"] = ("algorithmic bytes are those of the reference's per-edge "
-                        "subtraction scan; the kernel finds each crossing edge through "
-                        "per-row breakpoints instead, so frac can pass 1: traffic is "
-                        "the measured HBM bytes")
-    achieved = per_launch / (kernel_ms * 1e-3) / 1e9
-    roof["achieved"] = achieved
-    roof["frac"] = achieved / HBM_PEAK_GBS
-    roof["traffic"] = measured_traffic(roof["kernel"], name)
-    log("%s sampling: %d x %d sets in %.3fs; kernel %.3f ms; %.1f GB/s algorithmic; "
-        "counters %s" % (name, steps, batch, wall, kernel_ms, achieved, stats))
-    res = {
-        "value": batch * steps / wall, "ms_per_step": wall / steps * 1e3,
-        "config": {"workload": name, "description": wl["desc"],
-                   "num_nodes": int(len(index) - 1), "num_edges": int(len(edges)),
-                   "batch_sets": batch, "k": wl["k"], "epsilon": wl["eps"],
-                   "parallelism": "single GPU"},
-        "imm_wall_s": imm_wall,
-        "imm": None if rec is None else {
-            "theta": rec["Theta"], "rounds": rec["Rounds"], "num_seeds": rec["NumSeeds"],
-            "final_f": rec["FinalF"], "estimation_ms": rec["ThetaEstimation"],
-            "generate_ms": rec["GenerateRRRSets"],
-            "select_ms": rec["FindMostInfluentialSet"]},
-        "pool": {"sets": n_sets, "elements": n_elem},
-        "kernel_counters": stats,
-        "roofline": roof,
-        "graph": (wl, index, edges, w, batch),
-    }
-    if wl["model"] == R.IC and fused_steps:
-        # the 64-colour fused sampler on the same batch, for the fused-versus-
-        # one-sample-per-wave comparison (the sets are the same)
-        fz = {"sampler": 2}
-        pool.reset()
-        pool.generate_rrr_sets(batch, tune=fz)
-        t0 = time.time()
-        for _ in range(fused_steps):
-            pool.reset()
-            pool.generate_rrr_sets(batch, tune=fz)
-        fwall = time.time() - t0
-        res["fused_colour_sampler"] = {
-            "value": batch * fused_steps / fwall, "ms_per_step": fwall / fused_steps * 1e3,
-            "kernel_ms": pool.last_kernel_ms()[0],
-            "wave_over_fused": (batch * steps / wall) / (batch * fused_steps / fwall)}
-        log("%s fused colour sampler: %.3f s per step (%.2fx slower than one sample per wave)"
-            % (name, fwall / fused_steps, res["fused_colour_sampler"]["wave_over_fused"]))
-    pool.close()
-    g.close()
-    return res
-
-
 SERVE_K = (10, 50, 100, 200)
 SERVE_EPS = (0.5, 0.2, 0.13)
-
 
 # Largest scale whose IC graph keeps the U[0,0.1] in-weights in the serving
 # mix: measured mean RRR set sizes of 7.4K (scale 18), 25K (20) and 83K (22)
 # vertices put one k=200, eps=0.13 query's pool at 25 GB, 107 GB and 457 GB
-# (beyond HBM), so the larger IC graphs use weighted-cascade weights.
+# as sorted lists, so the larger IC graphs use weighted-cascade weights.
 SERVE_IC_UNIFORM_MAX_SCALE = 18
 
 
@@ -323,7 +402,7 @@ def serve_bench(scales, n_queries, seed, slots, workers, load, budget, modes):
     """Continuous-batching serving against back-to-back plain solves.
     Queries: graph uniform over the resident graphs, k in {10,50,100,200},
     eps in {0.5,0.2,0.13}, Poisson arrivals.  Seed modes: "arrival" gives
-    query i the sampling seed i (no two queries share a stream); "shared"
+    query i the sampling seed i (the served workload's seeding); "shared"
     gives every query seed 0, the fixed seeding of the reference's imm tool,
     so queries on one graph reuse each other's RRR sets."""
     graphs = serve_graphs(scales, seed)
@@ -363,22 +442,27 @@ def copy_probe(gib):
     return 0
 
 
+SECONDARY_KEYS = ("value", "ms_per_step", "config", "imm_wall_s", "imm", "graph_create_s",
+                  "pool", "kernel_counters", "roofline", "cpu_baseline", "cpu_imm", "spread",
+                  "plain_wave", "colour_groups")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat20_lt", choices=sorted(WORKLOADS))
-    ap.add_argument("--secondary", default="rmat22_wc,rmat16_ic",
+    ap.add_argument("--workload", default="rmat22_wc", choices=sorted(WORKLOADS))
+    ap.add_argument("--secondary", default="rmat20_lt,rmat16_ic",
                     help="workloads reported beside the first, comma-separated ('' for none)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0, help="sets per step (0 = theta)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=30.0,
-                    help="CPU-baseline budget as estimated from a small probe (the full "
-                         "run usually takes a third of it)")
-    ap.add_argument("--tune", default="", help='sampler tuning, e.g. "sampler=2,block=256"')
-    ap.add_argument("--skip-imm", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="CPU-baseline budget per workload as estimated from a small probe")
+    ap.add_argument("--tune", default="", help='sampler tuning, e.g. "sampler=1,queue_cap=256"')
+    ap.add_argument("--fused-steps", type=int, default=2,
+                    help="steps of the plain and colour-group samplers on the headline batch")
     ap.add_argument("--serve-scales", default="18,20,22",
                     help="R-MAT scales of the serving workload ('' skips it)")
     ap.add_argument("--serve-queries", type=int, default=48)
@@ -406,8 +490,10 @@ def main():
     tune = None
     if a.tune:
         tune = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in a.tune.split(",")}
-    res = measure(a.workload, a.seed, a.steps, a.warmup, a.batch, tune, a.skip_imm)
-    wl, index, edges, w, batch = res.pop("graph")
+    host = host_info()
+    cpu_s = 0.0 if a.no_cpu else a.cpu_seconds
+    res = measure(a.workload, a.seed, a.steps, a.warmup, a.batch, tune, host, cpu_s,
+                  WORKLOADS[a.workload]["scale"] <= 20, a.fused_steps, (4096, 64))
     out = {
         "metric": "RRR sets/s + IMM wall s at fixed k,eps; p50/p99 latency of served IMM queries",
         "value": res["value"],
@@ -419,31 +505,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 ids, f32 weights, u64 lcg64",
+        "dtype": "u32 vertex ids, integer coins (u64 lcg64 draws, fixed-point gaps)",
         "data": "synthetic R-MAT (generated in-process, seed %d)" % a.seed,
         "config": res["config"],
-        "imm_wall_s": res["imm_wall_s"],
-        "imm": res["imm"],
-        "pool": res["pool"],
-        "kernel_counters": res["kernel_counters"],
         "tune": tune,
-        "roofline": res["roofline"],
-        "cpu_baseline": None,
+        "host": host,
     }
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, index, edges, w, a.seed, batch, a.cpu_seconds)
-        log("cpu baseline: %.0f sets/s on %d threads" % (out["cpu_baseline"]["value"],
-                                                          out["cpu_baseline"]["cores"]))
-    del index, edges, w
-    others = [w for w in a.secondary.split(",") if w and w != a.workload]
-    for i, name in enumerate(others):
-        sec = measure(name, a.seed, max(1, a.steps // 3), 1, 0, None, False,
-                      fused_steps=2 if i == 0 else 0)
-        sec.pop("graph")
-        out["secondary" if i == 0 else "secondary_" + name] = {
-            k: sec[k] for k in ("value", "ms_per_step", "config", "imm_wall_s", "imm", "pool",
-                                "kernel_counters", "roofline", "fused_colour_sampler")
-            if k in sec}
+    for k in ("imm_wall_s", "imm", "graph_create_s", "pool", "kernel_counters", "call_counters",
+              "roofline", "cpu_baseline", "cpu_imm", "spread", "plain_wave", "colour_groups"):
+        if k in res:
+            out[k] = res[k]
+    out.setdefault("cpu_baseline", None)
+    others = [x for x in a.secondary.split(",") if x and x != a.workload]
+    for name in others:
+        sec = measure(name, a.seed, max(1, a.steps // 2), 1, 0, None, host, cpu_s,
+                      WORKLOADS[name]["scale"] <= 20, 0, None)
+        out["secondary_" + name] = {k: sec[k] for k in SECONDARY_KEYS if k in sec}
     if a.serve_scales:
         out["serving"] = serve_bench([int(x) for x in a.serve_scales.split(",")],
                                      a.serve_queries, a.seed, a.serve_slots, a.serve_workers,

@@ -49,7 +49,9 @@ typedef struct {
   uint32_t colors;       /* IC: samples fused per colour group, 1..64 */
   uint32_t block;        /* IC: threads per workgroup: 256, 512 or 1024 */
   uint32_t slots;        /* IC: resident colour groups (persistent workgroups) */
-  uint32_t edge_run;     /* IC: consecutive in-edges per lane: 4, 8 or 16 */
+  uint32_t queue_cap;    /* IC lane sampler: vertices one lane's sample may
+                            reach before the wave sampler takes it over:
+                            64, 256 or 1024 (0 = 1024) */
   uint32_t heavy_degree; /* IC: rows this long wait for the hub phase */
   uint32_t sampler;      /* IC: RIP_SAMPLER_AUTO, _WAVE or _COLOUR */
   uint32_t out_chunk;    /* LT and IC wave sampler: pool elements a wave
@@ -58,13 +60,18 @@ typedef struct {
   uint32_t reserved;     /* must be zero */
 } rip_tune;
 
-/* IC sampler kinds: one reverse BFS per wave, with sets larger than 4096
- * vertices finished by 64-sample colour groups (AUTO) or by a bitmap-visited
- * BFS per wave (WAVE), or colour groups for every sample (COLOUR).  All give
- * the same sets. */
+/* IC sampler kinds (all give the same sets):
+ *   AUTO    one reverse BFS per lane (64 samples per wave); samples passing
+ *           queue_cap vertices go to one BFS per wave, those passing 4096 to
+ *           64-sample colour groups
+ *   WAVE    one reverse BFS per wave for every sample (the plain form), sets
+ *           passing 4096 vertices finished by a bitmap-visited BFS per wave
+ *   COLOUR  64-sample colour groups for every sample
+ *   LANE    as AUTO, with the bitmap-visited BFS instead of colour groups */
 #define RIP_SAMPLER_AUTO 0
 #define RIP_SAMPLER_WAVE 1
 #define RIP_SAMPLER_COLOUR 2
+#define RIP_SAMPLER_LANE 3
 
 /* Timing and size record of one IMM run; times in milliseconds.  Mirrors
  * IMMExecutionRecord (include/ripples/imm_execution_record.h:55-92). */
@@ -173,27 +180,34 @@ int rip_pool_count(rip_pool *p, uint64_t num_sets, uint32_t *counts);
  * num_seeds) -- FindCoverage (include/ripples/opimc.h:100-118). */
 int rip_pool_coverage(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
                       uint32_t num_seeds, uint64_t *covered);
-/* Device time of the last rip_pool_sample call's main sampling kernel (the
- * IC wave kernel or the LT walk kernel; the colour fallback, the LT list
- * sort and the long-walk kernel are not included), measured with HIP events
- * on the pool's stream, and the number of kernel launches of the call. */
+/* Device time of the last rip_pool_sample call's main sampling kernel (IC:
+ * the lane kernel, or the wave kernel when the sampler kind has no lane
+ * stage; LT: the walk kernel), measured with HIP events on the pool's
+ * stream, and the number of kernel launches of the call. */
 int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
-/* Work counters of the last rip_pool_sample call's main kernel, written to
+/* Device time from the start of the last rip_pool_sample call's main kernel
+ * to the end of its last kernel (fallback kernels, sorts and the host round
+ * trips between them included), measured with HIP events. */
+int rip_pool_last_call_ms(const rip_pool *p, double *ms);
+/* Work counters of the last rip_pool_sample call's kernels, written to
  * stats[0 .. count) in this order (no counterpart in the reference; used for
  * roofline accounting):
- *   RIP_STAT_ALGO_BYTES  LT: sum over walks of 8 per visited vertex (its CSR
- *                        offsets), 8 per in-edge the subtraction loop scans
- *                        (id + weight) and 4 per output id; 0 for IC, whose
- *                        per-launch figure depends on the union of the sets
- *   RIP_STAT_EDGES       IC: in-edges read; LT: in-edges the subtraction
+ *   RIP_STAT_ALGO_BYTES  IC: bytes the gap samplers' algorithm reads and
+ *                        writes: 64 per row expanded (its vertex record),
+ *                        4 per candidate edge of a row whose edges do not
+ *                        share one threshold, 4 per live edge (the
+ *                        in-neighbour), 4 per output id;
+ *                        LT: the reference's subtraction scan, 8 per visited
+ *                        vertex (its CSR offsets), 8 per in-edge the loop
+ *                        scans (id + weight) and 4 per output id
+ *   RIP_STAT_EDGES       IC: candidate edges; LT: in-edges the subtraction
  *                        loop scans (up to and including the chosen one)
- *   RIP_STAT_COINS       IC: (edge, sample) coin evaluations;
+ *   RIP_STAT_COINS       IC: stream draws (gaps and acceptances);
  *                        LT: row breakpoints read
- *   RIP_STAT_EXPANSIONS  IC: frontier entries expanded (vertex, colour group);
- *                        LT: walk steps (rounds of a busy lane)
- *   RIP_STAT_LEVELS      IC: BFS levels summed over colour groups;
- *                        LT: loop rounds summed over waves (64 x this over
- *                        steps is the lane utilisation of the walk kernel) */
+ *   RIP_STAT_EXPANSIONS  IC: rows expanded; LT: walk steps
+ *   RIP_STAT_LEVELS      IC: live edges; LT: loop rounds summed over waves
+ *                        (64 x this over steps is the walk kernel's lane
+ *                        utilisation) */
 #define RIP_STAT_ALGO_BYTES 0
 #define RIP_STAT_EDGES 1
 #define RIP_STAT_COINS 2
@@ -201,6 +215,9 @@ int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
 #define RIP_STAT_LEVELS 4
 #define RIP_STAT_COUNT 5
 int rip_pool_last_stats(const rip_pool *p, uint64_t *stats, uint32_t count);
+/* The same counters for the main kernel alone (the one rip_pool_last_kernel_ms
+ * times). */
+int rip_pool_last_kernel_stats(const rip_pool *p, uint64_t *stats, uint32_t count);
 
 /* ---------------------------------------------------------------- IMM */
 /* The IMM algorithm -- IMM(G, CFG, l=1, gen, record, model, omp) of

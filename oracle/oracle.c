@@ -128,22 +128,137 @@ uint32_t orc_sample_root(uint64_t seed, uint64_t s, uint32_t n) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* IC row sampling by geometric gaps (the IC part of the stream contract).   */
+/* Row v = in-edges [b, b+d) with integer thresholds t_e = floor(w 2^24)     */
+/* and T = max t_e.  T = 0: nothing is live.  Block size 2^beta, beta the    */
+/* least value in [4, 20] with T 2^beta >= 2^25; block k = edges b + kB +    */
+/* [0, len), len = min(B, d - kB), draws elements 1 + 2(b + kB) + j of the   */
+/* sample's stream: candidate i uses element 2i for its gap and 2i + 1 for  */
+/* its acceptance.  gap = floor(NL(hi32(x) + 1) / q), NL(U) = -log2(U 2^-32)*/
+/* in units of 2^-58, q = NL((2^24 - T) 2^8) (q = 0, gap 0, when T = 2^24). */
+/* A candidate e is live iff hi32(a) T < t_e 2^32.  Each edge is then live   */
+/* independently with probability t_e / 2^24, the reference's u <= w test    */
+/* (add_rrrset.h:128-134) with u on the 2^-24 grid.                         */
+/* ------------------------------------------------------------------------ */
+#include "ic_gap_tables.inc"
+
+static uint64_t mulhi64(uint64_t a, uint64_t b) {
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+}
+
+uint64_t orc_ic_neg_log2(uint64_t u) {
+  if (u >= (1ull << 32)) return 0;
+  int e = 63 - __builtin_clzll(u);
+  uint64_t m = u << (63 - e);
+  uint32_t i = (uint32_t)(m >> 55) & 255u;
+  uint64_t r = m & ((1ull << 55) - 1);
+  /* x = r 2^-63 / (1 + i/256) as X = x 2^64 = (r * kRecipC[i]) >> 62 */
+  uint64_t X = (uint64_t)(((unsigned __int128)r * kRecipC[i]) >> 62);
+  uint64_t X2 = mulhi64(X, X), X3 = mulhi64(X2, X), X4 = mulhi64(X3, X);
+  uint64_t ln = X - X2 / 2 + X3 / 3 - X4 / 4;
+  uint64_t frac = kLog2C[i] + (mulhi64(ln, RIP_INV_LN2_Q63) << 1);
+  return ((uint64_t)(32 - e) << 58) - (frac >> 6);
+}
+
+uint32_t orc_ic_threshold(float w) {
+  if (!(w > 0.0f)) return 0;
+  if (w >= 1.0f) return 1u << 24;
+  return (uint32_t)(w * 16777216.0f);
+}
+
+uint64_t orc_ic_gap_q(uint32_t T) {
+  if (T >= (1u << 24)) return 0;
+  return orc_ic_neg_log2((uint64_t)((1u << 24) - T) << 8);
+}
+
+uint32_t orc_ic_block_log2(uint32_t T) {
+  uint32_t beta = 4;
+  while (beta < 20 && ((uint64_t)T << beta) < (1ull << 25)) ++beta;
+  return beta;
+}
+
+uint64_t orc_ic_gap(uint64_t x, uint64_t q) {
+  return q == 0 ? 0 : orc_ic_neg_log2((x >> 32) + 1) / q;
+}
+
+/* ------------------------------------------------------------------------ */
 /* fast evaluation of the same stream: per-vertex jump + stepping            */
 /* ------------------------------------------------------------------------ */
 typedef struct {
   aff P;         /* one step of a sample stream: M^(2S) */
-  aff *vjump;    /* vjump[v] = P^(1 + index[v]) */
+  aff P2[32];    /* P^(2^j) */
+  aff *vjump;    /* IC: P^(1 + 2 index[v]) (the row's first element);
+                    LT: P^(1 + index[v]) (the threshold element) */
+  uint32_t *thr; /* IC: integer threshold of every edge */
+  uint32_t *rowT;/* IC: the row's largest threshold */
 } stream_ctx;
 
-static void stream_ctx_init(stream_ctx *c, uint32_t n, const uint64_t *index) {
+static void stream_ctx_init(stream_ctx *c, int model, uint32_t n, const uint64_t *index,
+                            const float *w) {
   c->P = aff_pow(MASTER, 2 * ORC_SAMPLE_STRIDE);
+  c->P2[0] = c->P;
+  for (int j = 1; j < 32; ++j) c->P2[j] = aff_then(c->P2[j - 1], c->P2[j - 1]);
   c->vjump = (aff *)malloc(sizeof(aff) * (size_t)(n ? n : 1));
+  c->thr = NULL;
+  c->rowT = NULL;
 #pragma omp parallel for schedule(static)
   for (int64_t v = 0; v < (int64_t)n; ++v)
-    c->vjump[v] = aff_pow(c->P, 1 + index[v]);
+    c->vjump[v] = aff_pow(c->P, model == ORC_IC ? 1 + 2 * index[v] : 1 + index[v]);
+  if (model == ORC_IC) {
+    uint64_t m = index[n];
+    c->thr = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(m ? m : 1));
+    c->rowT = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < (int64_t)n; ++v) {
+      uint32_t T = 0;
+      for (uint64_t e = index[v]; e < index[v + 1]; ++e) {
+        c->thr[e] = orc_ic_threshold(w[e]);
+        if (c->thr[e] > T) T = c->thr[e];
+      }
+      c->rowT[v] = T;
+    }
+  }
 }
 
-static void stream_ctx_free(stream_ctx *c) { free(c->vjump); }
+static void stream_ctx_free(stream_ctx *c) {
+  free(c->vjump);
+  free(c->thr);
+  free(c->rowT);
+}
+
+/* Live in-edges of row v for one sample: x0 is the sample's element
+ * 1 + 2 index[v] (NULL-free fast path) or, with definitional set, every
+ * element is recomputed from (seed, s).  Writes the live edge positions in
+ * increasing order to live[] and returns their count (at most the degree). */
+static uint64_t ic_row_live(const stream_ctx *c, const uint64_t *index, uint32_t v, uint64_t x0,
+                            int definitional, uint64_t seed, uint64_t s, uint64_t *live) {
+  uint32_t T = c->rowT[v];
+  uint64_t b = index[v], d = index[v + 1] - b, nl = 0;
+  if (T == 0 || d == 0) return 0;
+  uint64_t q = orc_ic_gap_q(T);
+  uint32_t beta = orc_ic_block_log2(T);
+  uint64_t B = 1ull << beta;
+  aff blk_step = c->P2[beta + 1]; /* P^(2B): one block further */
+  uint64_t xb = x0;
+  for (uint64_t k0 = 0; k0 < d; k0 += B, xb = aff_apply(blk_step, xb)) {
+    uint64_t len = d - k0 < B ? d - k0 : B, pos = 0, i = 0;
+    uint64_t x = xb;
+    while (pos < len) {
+      uint64_t gx = definitional ? orc_sample_element(seed, s, 1 + 2 * (b + k0) + 2 * i) : x;
+      uint64_t g = orc_ic_gap(gx, q);
+      if (g >= len - pos) break;
+      pos += g;
+      uint64_t e = b + k0 + pos;
+      uint64_t y = definitional ? orc_sample_element(seed, s, 1 + 2 * (b + k0) + 2 * i + 1)
+                                : aff_apply(c->P, x);
+      if ((y >> 32) * T < ((uint64_t)c->thr[e] << 32)) live[nl++] = e;
+      pos += 1;
+      ++i;
+      x = aff_apply(c->P, aff_apply(c->P, x));
+    }
+  }
+  return nl;
+}
 
 static uint64_t sample_z(uint64_t seed, uint64_t s) {
   return aff_apply(aff_pow(MASTER, 2 * s + 2), seed);
@@ -181,24 +296,25 @@ static int cmp_u32(const void *x, const void *y) {
 static void rrr_one(int model, uint32_t n, const uint64_t *index,
                     const uint32_t *src, const float *w, const stream_ctx *c,
                     uint64_t seed, uint64_t s, uint8_t *visited, arena *out,
-                    int definitional) {
+                    int definitional, uint64_t *live) {
   uint64_t start = out->len;
   uint64_t z = sample_z(seed, s);
   uint32_t root = orc_uniform_index(z, n);
   arena_push(out, root);
   visited[root] = 1;
   if (model == ORC_IC) {
+    /* FIFO reverse BFS; the live in-edges of a row depend only on (seed, s,
+     * row), so the set is the reverse closure of the root over live edges */
     for (uint64_t head = start; head < out->len; ++head) {
       uint32_t v = out->buf[head];
-      uint64_t x = aff_apply(c->vjump[v], z);
-      for (uint64_t e = index[v]; e < index[v + 1]; ++e) {
-        uint64_t coin = definitional ? orc_sample_element(seed, s, 1 + e) : x;
-        uint32_t u = src[e];
-        if (!visited[u] && orc_u01(coin) <= w[e]) {
+      uint64_t nl = ic_row_live(c, index, v, aff_apply(c->vjump[v], z), definitional, seed, s,
+                                live);
+      for (uint64_t j = 0; j < nl; ++j) {
+        uint32_t u = src[live[j]];
+        if (!visited[u]) {
           visited[u] = 1;
           arena_push(out, u);
         }
-        x = aff_apply(c->P, x);
       }
     }
   } else {
@@ -228,11 +344,12 @@ static void rrr_one(int model, uint32_t n, const uint64_t *index,
 }
 
 /* ---- 64-colour sweep, as BatchedBFSNeighborColor (batched_add_rrrset.h:184-257):
- * full-vertex sweeps until no mask changes; colour i of the batch is bit 63-i. */
+ * full-vertex sweeps until no mask changes; colour i of the batch is bit 63-i.
+ * Each newly active colour at v draws its own live in-edges of v's row. */
 static void rrr_batched64(uint32_t n, const uint64_t *index, const uint32_t *src,
                           const float *w, const stream_ctx *c, uint64_t seed,
                           uint64_t s0, uint32_t nb, uint64_t *oldm,
-                          uint64_t *newm, arena *out, uint64_t *sizes) {
+                          uint64_t *newm, arena *out, uint64_t *sizes, uint64_t *live) {
   uint64_t z[64];
   memset(oldm, 0, sizeof(uint64_t) * n);
   memset(newm, 0, sizeof(uint64_t) * n);
@@ -248,25 +365,13 @@ static void rrr_batched64(uint32_t n, const uint64_t *index, const uint32_t *src
       found = 1;
       uint64_t colors = newm[v] ^ oldm[v];
       oldm[v] |= newm[v];
-      int ids[64];
-      uint64_t xs[64];
-      int nc = 0;
       while (colors) {
         int i = __builtin_clzll(colors); /* bit 63-i <-> colour i */
-        colors &= ~(1ull << (63 - i));
-        ids[nc] = i;
-        xs[nc] = aff_apply(c->vjump[v], z[i]);
-        ++nc;
-      }
-      for (uint64_t e = index[v]; e < index[v + 1]; ++e) {
-        uint32_t u = src[e];
-        uint64_t have = newm[u], add = 0;
-        for (int j = 0; j < nc; ++j) {
-          uint64_t bit = 1ull << (63 - ids[j]);
-          if (!(have & bit) && orc_u01(xs[j]) <= w[e]) add |= bit;
-          xs[j] = aff_apply(c->P, xs[j]);
-        }
-        newm[u] |= add;
+        uint64_t bit = 1ull << (63 - i);
+        colors &= ~bit;
+        uint64_t nl = ic_row_live(c, index, v, aff_apply(c->vjump[v], z[i]), 0, seed, s0 + i,
+                                  live);
+        for (uint64_t j = 0; j < nl; ++j) newm[src[live[j]]] |= bit;
       }
     }
   }
@@ -303,7 +408,10 @@ orc_sets *orc_sample(int model, int algo, uint32_t n, const uint64_t *index,
   if (n == 0 || (algo == 1 && model != ORC_IC)) return NULL;
   if (nthreads < 1) nthreads = 1;
   stream_ctx c;
-  stream_ctx_init(&c, n, index);
+  stream_ctx_init(&c, model, n, index, w);
+  uint32_t maxdeg = 0;
+  for (uint32_t v = 0; v < n; ++v)
+    if (index[v + 1] - index[v] > maxdeg) maxdeg = (uint32_t)(index[v + 1] - index[v]);
   /* record which arena holds each sample */
   uint32_t *owner = (uint32_t *)malloc(sizeof(uint32_t) * (count ? count : 1));
   uint64_t *where = (uint64_t *)malloc(sizeof(uint64_t) * (count ? count : 1));
@@ -318,6 +426,7 @@ orc_sets *orc_sample(int model, int algo, uint32_t n, const uint64_t *index,
     tid = omp_get_thread_num();
 #endif
     arena *a = &arenas[tid];
+    uint64_t *live = (uint64_t *)malloc(sizeof(uint64_t) * ((size_t)maxdeg + 1));
     if (algo == 1) {
       uint64_t *oldm = (uint64_t *)malloc(sizeof(uint64_t) * n);
       uint64_t *newm = (uint64_t *)malloc(sizeof(uint64_t) * n);
@@ -328,7 +437,7 @@ orc_sets *orc_sample(int model, int algo, uint32_t n, const uint64_t *index,
         uint64_t sizes[64];
         uint64_t base = a->len;
         rrr_batched64(n, index, src, w, &c, seed, first + s0, nb, oldm, newm, a,
-                      sizes);
+                      sizes, live);
         for (uint32_t i = 0; i < nb; ++i) {
           owner[s0 + i] = (uint32_t)tid;
           where[s0 + i] = base;
@@ -344,13 +453,14 @@ orc_sets *orc_sample(int model, int algo, uint32_t n, const uint64_t *index,
       for (int64_t i = 0; i < (int64_t)count; ++i) {
         uint64_t base = a->len;
         rrr_one(model, n, index, src, w, &c, seed, first + (uint64_t)i,
-                visited, a, algo == 2);
+                visited, a, algo == 2, live);
         owner[i] = (uint32_t)tid;
         where[i] = base;
         size[i] = a->len - base;
       }
       free(visited);
     }
+    free(live);
   }
 
   orc_sets *h = (orc_sets *)calloc(1, sizeof(orc_sets));

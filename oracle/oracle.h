@@ -19,11 +19,16 @@
  *     sampling stream = master.split(2, 1)   (tools/imm.cc:212-214)
  *   - sample s draws from sampling.split(S, s) with S = 2^32 - 1:
  *       element 0      -> the root,  root = (hi32(x) * n) >> 32
- *       element 1 + e  -> IC coin of transposed-CSR edge e
+ *       elements 1 + 2 index[v] + j -> IC draws of vertex v's row (gaps
+ *                         between candidate edges and their acceptances,
+ *                         see the IC section of oracle.c)
  *       element 1 + index[v] -> LT threshold of vertex v
  *     so element j of sample s is master output 1 + 2*(s + j*S).
  *   - uniform01: u = ((x >> 40) + 1) * 2^-24, a float in (0, 1].
- *     An IC edge is live when u <= w (add_rrrset.h:130).
+ *   - An IC edge of weight w is live with probability t / 2^24, t =
+ *     floor(w 2^24) (u <= w on the 2^-24 grid, add_rrrset.h:130), each edge
+ *     independently: a row's candidates are drawn by geometric gaps at the
+ *     row's largest threshold and thinned to each edge's own.
  *
  * Parity status of each function is listed in the header of oracle.c.
  */
@@ -54,6 +59,18 @@ int orc_lcg64_split(orc_lcg64 *g, uint64_t s, uint64_t i);
 uint64_t orc_master_output(uint64_t seed, uint64_t t);
 float orc_u01(uint64_t x);
 uint32_t orc_uniform_index(uint64_t x, uint32_t n);
+
+/* ---- IC gap sampling (the contract's integer functions, exposed for tests) ----
+ * orc_ic_neg_log2(u): -log2(u / 2^32) for u in [1, 2^32], units of 2^-58.
+ * orc_ic_threshold(w): floor(w 2^24) clamped to [0, 2^24] (0 for NaN).
+ * orc_ic_gap_q(T): NL((2^24 - T) 2^8), 0 for T = 2^24.
+ * orc_ic_block_log2(T): least beta in [4, 20] with T 2^beta >= 2^25.
+ * orc_ic_gap(x, q): floor(NL(hi32(x) + 1) / q), 0 for q = 0. */
+uint64_t orc_ic_neg_log2(uint64_t u);
+uint32_t orc_ic_threshold(float w);
+uint64_t orc_ic_gap_q(uint32_t T);
+uint32_t orc_ic_block_log2(uint32_t T);
+uint64_t orc_ic_gap(uint64_t x, uint64_t q);
 
 /* element j of sample s's stream, computed from the definition */
 uint64_t orc_sample_element(uint64_t seed, uint64_t s, uint64_t j);

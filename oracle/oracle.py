@@ -67,6 +67,16 @@ def lib():
     L.orc_sample_element.restype = C.c_uint64
     L.orc_sample_root.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
     L.orc_sample_root.restype = C.c_uint32
+    L.orc_ic_neg_log2.argtypes = [C.c_uint64]
+    L.orc_ic_neg_log2.restype = C.c_uint64
+    L.orc_ic_threshold.argtypes = [C.c_float]
+    L.orc_ic_threshold.restype = C.c_uint32
+    L.orc_ic_gap_q.argtypes = [C.c_uint32]
+    L.orc_ic_gap_q.restype = C.c_uint64
+    L.orc_ic_block_log2.argtypes = [C.c_uint32]
+    L.orc_ic_block_log2.restype = C.c_uint32
+    L.orc_ic_gap.argtypes = [C.c_uint64, C.c_uint64]
+    L.orc_ic_gap.restype = C.c_uint64
     L.orc_sample.argtypes = [C.c_int, C.c_int, C.c_uint32, _u64p, _u32p, _f32p,
                              C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]
     L.orc_sample.restype = C.c_void_p
@@ -256,6 +266,26 @@ def uniform_index(x, n):
 
 def sample_element(seed, s, j):
     return lib().orc_sample_element(seed, s, j)
+
+
+def ic_neg_log2(u):
+    return lib().orc_ic_neg_log2(u)
+
+
+def ic_threshold(w):
+    return lib().orc_ic_threshold(w)
+
+
+def ic_gap_q(T):
+    return lib().orc_ic_gap_q(T)
+
+
+def ic_block_log2(T):
+    return lib().orc_ic_block_log2(T)
+
+
+def ic_gap(x, q):
+    return lib().orc_ic_gap(x, q)
 
 
 def sample_root(seed, s, n):

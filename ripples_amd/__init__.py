@@ -39,7 +39,7 @@ class RipplesError(RuntimeError):
 
 class Tune(C.Structure):
     _fields_ = [("colors", C.c_uint32), ("block", C.c_uint32),
-                ("slots", C.c_uint32), ("edge_run", C.c_uint32),
+                ("slots", C.c_uint32), ("queue_cap", C.c_uint32),
                 ("heavy_degree", C.c_uint32), ("sampler", C.c_uint32),
                 ("out_chunk", C.c_uint32), ("reserved", C.c_uint32)]
 
@@ -135,6 +135,8 @@ def lib():
     L.rip_pool_count.argtypes = [vp, C.c_uint64, _u32p]
     L.rip_pool_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double),
                                           C.POINTER(C.c_uint32)]
+    L.rip_pool_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
+    L.rip_pool_last_kernel_stats.argtypes = [vp, _u64p, C.c_uint32]
     L.rip_pool_last_stats.argtypes = [vp, _u64p, C.c_uint32]
     L.rip_imm.argtypes = [vp, C.c_uint32, C.c_double, C.c_uint64, C.POINTER(Tune),
                           _u32p, C.POINTER(ImmRecord)]
@@ -412,12 +414,24 @@ class Pool:
         _check(lib().rip_pool_last_kernel_ms(self._h, C.byref(ms), C.byref(launches)))
         return ms.value, launches.value
 
+    def last_call_ms(self):
+        """Device time of the whole last sampling call (every kernel)."""
+        ms = C.c_double(0)
+        _check(lib().rip_pool_last_call_ms(self._h, C.byref(ms)))
+        return ms.value
+
     STAT_NAMES = ("algo_bytes", "edges", "coins", "expansions", "levels")
 
     def last_stats(self):
         """Work counters of the last sampling call's main kernel."""
         out = np.zeros(len(self.STAT_NAMES), np.uint64)
         _check(lib().rip_pool_last_stats(self._h, out, len(out)))
+        return {k: int(v) for k, v in zip(self.STAT_NAMES, out)}
+
+    def last_kernel_stats(self):
+        """Work counters of the last sampling call's main kernel alone."""
+        out = np.zeros(len(self.STAT_NAMES), np.uint64)
+        _check(lib().rip_pool_last_kernel_stats(self._h, out, len(out)))
         return {k: int(v) for k, v in zip(self.STAT_NAMES, out)}
 
     def last_algo_bytes(self):

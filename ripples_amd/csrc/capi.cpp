@@ -195,6 +195,23 @@ int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches) {
   });
 }
 
+int rip_pool_last_call_ms(const rip_pool *p, double *ms) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    RIP_REQUIRE(ms, RIP_EINVAL, "ms is NULL");
+    *ms = p->last_call_ms;
+  });
+}
+
+int rip_pool_last_kernel_stats(const rip_pool *p, uint64_t *stats, uint32_t count) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    RIP_REQUIRE(count == 0 || stats, RIP_EINVAL, "stats is NULL");
+    for (uint32_t i = 0; i < count; ++i)
+      stats[i] = i < RIP_STAT_COUNT ? p->main_stats[i] : 0;
+  });
+}
+
 int rip_pool_last_stats(const rip_pool *p, uint64_t *stats, uint32_t count) {
   return guarded([&] {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");

@@ -7,11 +7,13 @@
 //   The sampling stream is master.split(2, 1); sample s owns the leapfrog
 //   sub-stream sampling.split(S, s) with S = 2^32 - 1.  Element j of sample s
 //   is master output 1 + 2(s + jS):
-//     j = 0          -> root      = (hi32(x) * n) >> 32
-//     j = 1 + e      -> IC coin of transposed-CSR edge e, live iff u <= w
-//     j = 1 + off[v] -> LT threshold at vertex v
-//   with u = ((x >> 40) + 1) * 2^-24 in (0, 1].  The IC test u <= w is done
-//   as the exact integer test (x >> 40) < floor(w * 2^24).
+//     j = 0               -> root = (hi32(x) * n) >> 32
+//     j = 1 + 2 off[v] + i -> IC draws of v's in-edge row: gaps between
+//                           candidate edges and their acceptances (ic_gap.h)
+//     j = 1 + off[v]      -> LT threshold at vertex v
+//   with u = ((x >> 40) + 1) * 2^-24 in (0, 1] for LT thresholds.  An IC edge
+//   of weight w is live with probability floor(w 2^24) / 2^24, independently
+//   of every other edge (the reference's u <= w, add_rrrset.h:128-134).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -78,14 +80,22 @@ inline uint32_t ic_threshold(float w) {
   return static_cast<uint32_t>(scaled);  // floor for positive values
 }
 
-// Device vertex record: jump to the vertex's first coin + its CSR row.
-struct alignas(32) VertexRec {
-  uint64_t ja, jb;       // P^(1 + begin) as an affine map
+// Device vertex record, one aligned 64-byte line: the jump to the vertex's
+// first stream element (IC: P^(1 + 2 begin), the row's first draw; LT:
+// P^(1 + begin), the threshold) and its CSR row; for IC the row's gap
+// parameters (ic_gap.h): largest threshold T, q, log2 of the block size and
+// the full-block skip threshold.
+struct alignas(64) VertexRec {
+  uint64_t ja, jb;       // jump to the first element, as an affine map
+  uint64_t q;            // IC: gap divisor of the row (0 when T = 2^24)
   uint32_t begin, end;   // transposed-CSR row [begin, end)
-  uint32_t uthr;         // IC: the row's threshold when all its edges share it;
+  uint32_t thr;          // IC: the row's largest threshold T (0: nothing live);
                          // LT: offset of the row's breakpoint index
-  uint32_t flags;        // kRowUniform, kRowScalar
+  uint32_t ufull;        // IC: a full block has no candidate iff hi32(x) + 1 <= ufull
+  uint32_t flags;        // kRowUniform, kRowScalar; IC: beta << kBetaShift
+  uint32_t pad[5];
 };
+static_assert(sizeof(VertexRec) == 64, "one line per vertex record");
 
 // LT walk record of a vertex: one aligned 128-byte line holds what a walk
 // step reads before its breakpoint block.  The row's (breakpoint,
@@ -105,7 +115,8 @@ struct alignas(128) LtRec {
 };
 static_assert(sizeof(LtRec) == 128, "one line per LT record");
 
-constexpr uint32_t kRowUniform = 1u;  // IC: uthr holds the row's threshold
+constexpr uint32_t kRowUniform = 1u;  // IC: every edge of the row has threshold T
+constexpr uint32_t kBetaShift = 8;     // IC: flags bits 8..12 hold log2 of the block size
 constexpr uint32_t kRowScalar = 2u;   // LT: no breakpoints, scan the float chain
 // Extra zero elements after the per-edge arrays, so 16-byte loads of an
 // aligned group of four never leave the allocation.

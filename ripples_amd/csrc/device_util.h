@@ -60,6 +60,38 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_tmp,
   return off + x - v;
 }
 
+// The same in 64 bits (scans whose tile sums may pass 2^32).
+template <int BT>
+__device__ inline unsigned long long block_exclusive_scan_u64(unsigned long long v,
+                                                              unsigned long long *s_tmp,
+                                                              unsigned long long *total) {
+  constexpr int NW = BT / 64;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x >> 6;
+  unsigned long long x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long y = __shfl_up(x, d, 64);
+    if (lane >= static_cast<uint32_t>(d)) x += y;
+  }
+  if (lane == 63) s_tmp[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    unsigned long long w = lane < NW ? s_tmp[lane] : 0ull;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned long long y = __shfl_up(w, d, 64);
+      if (lane >= static_cast<uint32_t>(d)) w += y;
+    }
+    if (lane < NW) s_tmp[lane] = w;
+  }
+  __syncthreads();
+  const unsigned long long off = wid ? s_tmp[wid - 1] : 0ull;
+  *total = s_tmp[NW - 1];
+  __syncthreads();
+  return off + x - v;
+}
+
 // Append `val` to out[] for every lane with pred set; one LDS atomic per wave.
 // All active lanes of the wave must call it together.
 __device__ inline void wave_append(bool pred, uint32_t val, uint32_t *s_count,

@@ -1,9 +1,9 @@
 // Device copy of a transposed CSR graph (GraphBwd), laid out for the reverse
 // traversals of the samplers:
-//   d_vrec[v] : 32-byte record {P^(1+begin) jump, begin, end, uniform IC
-//               threshold, flags} -- one aligned load gives a vertex's row,
-//               the jump to its first coin and, for rows whose edges share a
-//               threshold (weighted cascade), that threshold
+//   d_vrec[v] : 64-byte record {jump to the vertex's first stream element,
+//               begin, end, IC gap parameters of the row (ic_gap.h), flags}
+//               -- one aligned line gives everything a row expansion needs
+//               before its candidate edges
 //   d_val[e]  : IC integer threshold | LT float weight bits
 //   d_ids[e]  : in-neighbour (read only for live / chosen edges)
 //   d_lrec[v] : LT walk record (common.h), one 128-byte line per vertex
@@ -18,26 +18,38 @@
 #include <cstring>
 #include <vector>
 
+#include "ic_gap.h"
 #include "internal.h"
 #include "lt_scan.h"
 
 namespace rip {
 namespace {
 
-__global__ void build_vrec_kernel(const unsigned long long *index, const uint32_t *uthr,
-                                  const uint32_t *flags, uint32_t n, Affine step,
+// IC: the row's gap parameters from its largest threshold (ic_gap.h);
+// LT: the jump to the threshold element and the breakpoint index offset.
+__global__ void build_vrec_kernel(const unsigned long long *index, const uint32_t *thr,
+                                  const uint32_t *flags, uint32_t n, Affine step, int model,
                                   VertexRec *vrec) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   uint64_t b = index[v], e = index[v + 1];
-  Affine j = aff_pow(step, 1 + b);
+  Affine j = aff_pow(step, model == RIP_IC ? 1 + 2 * b : 1 + b);
   VertexRec r;
   r.ja = j.a;
   r.jb = j.b;
+  r.q = 0;
   r.begin = static_cast<uint32_t>(b);
   r.end = static_cast<uint32_t>(e);
-  r.uthr = uthr[v];
+  r.thr = thr[v];
+  r.ufull = 0;
   r.flags = flags[v];
+  if (model == RIP_IC && r.thr != 0) {
+    const uint32_t beta = ic_block_log2(r.thr);
+    r.q = ic_gap_q(r.thr);
+    r.ufull = ic_full_skip(r.q, beta);
+    r.flags |= beta << kBetaShift;
+  }
+  for (int i = 0; i < 5; ++i) r.pad[i] = 0;
   vrec[v] = r;
 }
 
@@ -51,7 +63,7 @@ __global__ void build_lrec_kernel(const VertexRec *vrec, const uint32_t *boff, u
   o.jb = r.jb;
   o.boff = boff[v];
   o.d = r.end - r.begin;
-  o.ioff = r.uthr;
+  o.ioff = r.thr;
   o.flags = r.flags;
   for (int i = 0; i < 16; ++i) o.top[i] = 0xFFFFFFFFu;
   o.begin = r.begin;
@@ -131,19 +143,22 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
       std::memcpy(&val[e], &weights[e], 4);
     }
   }
-  // IC rows whose edges all share one threshold keep it in the vertex record;
-  // LT rows holding a weight without a breakpoint form are marked
-  std::vector<uint32_t> uthr(n, 0xFFFFFFFFu), flags(n, 0u);
+  // IC: each row's largest threshold, and whether every edge has it (then
+  // a candidate needs no acceptance draw); LT rows holding a weight without
+  // a breakpoint form are marked
+  std::vector<uint32_t> uthr(n, 0u), flags(n, 0u);
   for (uint32_t v = 0; v < n; ++v) {
     const uint64_t b = index[v], e = index[v + 1];
     if (b == e) continue;
     if (model == RIP_IC) {
+      uint32_t T = 0;
       bool same = true;
-      for (uint64_t i = b + 1; i < e && same; ++i) same = val[i] == val[b];
-      if (same) {
-        uthr[v] = val[b];
-        flags[v] |= kRowUniform;
+      for (uint64_t i = b; i < e; ++i) {
+        T = std::max(T, val[i]);
+        same = same && val[i] == val[b];
       }
+      uthr[v] = T;
+      if (same) flags[v] |= kRowUniform;
     } else {
       for (uint64_t i = b; i < e; ++i)
         if (lt_weight_irregular(val[i])) {
@@ -200,7 +215,7 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
                       hipMemcpyHostToDevice));
     dim3 grid((n + 255) / 256);
     hipLaunchKernelGGL(build_vrec_kernel, grid, dim3(256), 0, 0, d_index, d_uthr, d_flags,
-                       n, g->step, g->d_vrec);
+                       n, g->step, model, g->d_vrec);
     RIP_HIP(hipGetLastError());
     if (model == RIP_LT && m > 0) {
       // padded so a search may read 16-byte groups past the last row
@@ -269,6 +284,11 @@ void graph_destroy(rip_graph *g) {
   dev_free(g->wave.d_bbits);
   dev_free(g->wave.d_bsum);
   dev_free(g->wave.d_blist);
+  dev_free(g->wave.d_ltable);
+  dev_free(g->wave.d_llist);
+  dev_free(g->wave.d_lepoch);
+  dev_free(g->wave.d_over);
+  dev_free(g->wave.d_lsort);
   dev_free(g->lt.d_long_ids);
   dev_free(g->lt.d_spill);
   dev_free(g->d_bp);

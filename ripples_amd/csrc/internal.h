@@ -50,6 +50,16 @@ struct WaveScratch {
   uint32_t *d_bbits = nullptr;
   uint32_t *d_bsum = nullptr;
   uint32_t *d_blist = nullptr;
+  // lane kernel: per lane a visited table of 2 cap entries, a list of cap
+  // vertices and an epoch; samples that outgrew a lane, sets to sort
+  uint64_t lane_lanes = 0;
+  uint32_t lane_cap = 0;
+  unsigned long long *d_ltable = nullptr;
+  uint32_t *d_llist = nullptr;
+  uint32_t *d_lepoch = nullptr;
+  uint64_t cap_over = 0;
+  uint32_t *d_over = nullptr;
+  uint32_t *d_lsort = nullptr;
 };
 
 // Forward CSR and per-workgroup scratch of the spread simulator, built on
@@ -152,10 +162,12 @@ struct rip_pool {
   unsigned long long *d_ctl = nullptr;  // control words, see pool.hip
   unsigned long long *h_ctl = nullptr;  // pinned host copy of the control words
   rip::SelectScratch sel;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double last_kernel_ms = 0.0;
+  double last_call_ms = 0.0;
   uint32_t last_launches = 0;
   uint64_t last_stats[RIP_STAT_COUNT] = {};
+  uint64_t main_stats[RIP_STAT_COUNT] = {};  // the main kernel's share of last_stats
   double avg_set_size = 8.0;            // running estimate for capacity planning
 };
 
@@ -189,6 +201,7 @@ void graph_destroy(rip_graph *g);
 
 // pool.hip
 void pool_create(rip_pool *p, rip_graph *g, uint64_t seed);
+void pool_create_resources(rip_pool *p);  // stream, events, control words
 void pool_destroy(rip_pool *p);
 void pool_reserve_sets(rip_pool *p, uint64_t cap_sets);
 void pool_reserve_data(rip_pool *p, uint64_t cap_data);
@@ -200,6 +213,9 @@ int cu_count();
 // *out += sum of v[0, n) on stream st (out is device memory)
 void launch_sum_u32(const uint32_t *v, uint64_t n, unsigned long long *out, hipStream_t st);
 void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune);
+// Reads the control words into p->h_ctl (synchronising the pool's stream);
+// with main set, also keeps the sampling counters as the main kernel's.
+void pool_read_ctl(rip_pool *p, bool main);
 void pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
                       const uint32_t *vertices);
 void pool_export(rip_pool *p, uint64_t first, uint64_t count,
@@ -207,7 +223,7 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count,
 
 // sample_ic.hip
 struct IcLaunch {
-  uint32_t colors, block, slots, edge_run, heavy_degree, sampler;
+  uint32_t colors, block, slots, queue_cap, heavy_degree, sampler;
   bool colors_auto;    // colours not set by the caller
   uint32_t out_chunk;  // wave sampler's output chunk, 0 = automatic
   bool block_auto;     // block size not set by the caller
@@ -232,15 +248,27 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
                    bool record_events = true);
 
 // IC sampling of samples [first, first+count) with the configured sampler
-uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &cfg);
+uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &cfg,
+                   const rip_tune *tune);
 
-// sample_ic_wave.hip: returns how many samples were left for the colour
-// sampler; their indices (relative to first) are in g->wave.d_big
-uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, uint32_t out_chunk);
-// Most pool elements a wave-sampler launch of `count` samples can leave
+// sample_ic_wave.hip: one BFS per wave over samples first + i, i in
+// [0, count), or first + ids[i]; returns how many samples outgrew the wave's
+// list, their indices (relative to first) in g->wave.d_big.  record_events:
+// bracket the kernel with the pool's ev0 / ev1.
+uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
+                        uint32_t out_chunk, bool record_events);
+// Most pool elements an IC sampling call of `count` samples can leave
 // unused (the rest of each wave's last output chunk).
 uint64_t ic_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune);
 void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count);
+
+// sample_ic_lane.hip: one BFS per lane over samples [first, first + count);
+// returns how many outgrew a lane's queue, their indices (relative to first)
+// in g->wave.d_over.  Brackets its main kernel with the pool's ev0 / ev1.
+constexpr uint32_t kLaneCapDefault = 1024;
+uint32_t ic_lane_cap(const rip_tune *tune);
+uint64_t ic_lane_launch(rip_pool *p, uint64_t first, uint64_t count, const rip_tune *tune);
+uint64_t ic_lane_slack(const rip_pool *p, uint64_t count, const rip_tune *tune);
 
 // sample_lt.hip
 uint32_t lt_launch(rip_pool *p, uint64_t first, uint64_t count,

@@ -24,17 +24,19 @@ constexpr uint32_t kScanBT = 256;
 constexpr uint32_t kScanItems = 8;
 constexpr uint32_t kScanTile = kScanBT * kScanItems;
 
+// The scans add in 64 bits: a tile of 2048 counts of up to 2^32 - 1 each
+// (vertex occurrence counts, set sizes) can pass 2^32.
 __global__ void scan_tile_sums(const uint32_t *in, uint64_t n,
                                unsigned long long *tile_sum) {
-  __shared__ uint32_t s_tmp[kScanBT / 64];
+  __shared__ unsigned long long s_tmp[kScanBT / 64];
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
-  uint32_t acc = 0;
+  unsigned long long acc = 0;
   for (uint32_t i = 0; i < kScanItems; ++i) {
     const uint64_t j = base + static_cast<uint64_t>(i) * kScanBT + threadIdx.x;
     if (j < n) acc += in[j];
   }
-  uint32_t total;
-  (void)block_exclusive_scan<kScanBT>(acc, s_tmp, &total);
+  unsigned long long total;
+  (void)block_exclusive_scan_u64<kScanBT>(acc, s_tmp, &total);
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
 }
 
@@ -67,18 +69,18 @@ __global__ void scan_tile_prefix(unsigned long long *tile_sum, uint64_t tiles) {
 __global__ void scan_apply(const uint32_t *in, uint64_t n,
                            const unsigned long long *tile_prefix,
                            unsigned long long *out) {
-  __shared__ uint32_t s_tmp[kScanBT / 64];
+  __shared__ unsigned long long s_tmp[kScanBT / 64];
   // each thread owns kScanItems consecutive elements of the tile
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile +
                         static_cast<uint64_t>(threadIdx.x) * kScanItems;
   uint32_t v[kScanItems];
-  uint32_t acc = 0;
+  unsigned long long acc = 0;
   for (uint32_t i = 0; i < kScanItems; ++i) {
     v[i] = (base + i < n) ? in[base + i] : 0u;
     acc += v[i];
   }
-  uint32_t total;
-  uint32_t pre = block_exclusive_scan<kScanBT>(acc, s_tmp, &total);
+  unsigned long long total;
+  const unsigned long long pre = block_exclusive_scan_u64<kScanBT>(acc, s_tmp, &total);
   unsigned long long run = tile_prefix[blockIdx.x] + pre;
   for (uint32_t i = 0; i < kScanItems; ++i) {
     if (base + i < n) out[base + i] = run;
@@ -175,9 +177,22 @@ void pool_create(rip_pool *p, rip_graph *g, uint64_t seed) {
   RIP_HIP(hipSetDevice(g->device));
   p->g = g;
   p->seed = seed;
+  try {
+    pool_create_resources(p);
+  } catch (...) {
+    // release whatever was allocated before the failure; the caller may
+    // retry on the same object
+    pool_destroy(p);
+    p->g = g;
+    throw;
+  }
+}
+
+void pool_create_resources(rip_pool *p) {
   RIP_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
   RIP_HIP(hipEventCreate(&p->ev0));
   RIP_HIP(hipEventCreate(&p->ev1));
+  RIP_HIP(hipEventCreate(&p->ev2));
   p->d_ctl = dev_alloc<unsigned long long>(kCtlWords);
   // pinned, so reading the control words back after a launch is one DMA
   // rather than a copy staged through a driver buffer
@@ -209,8 +224,12 @@ void pool_destroy(rip_pool *p) {
   p->h_ctl = nullptr;
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
+  if (p->ev2) (void)hipEventDestroy(p->ev2);
+  p->ev0 = p->ev1 = p->ev2 = nullptr;
   if (p->stream) (void)hipStreamDestroy(p->stream);
   p->stream = nullptr;
+  p->sel = SelectScratch{};
+  p->cap_sets = p->cap_data = p->used_data = p->num_sets = p->num_elements = 0;
 }
 
 void pool_reserve_sets(rip_pool *p, uint64_t cap) {
@@ -292,6 +311,14 @@ void pool_reserve_data(rip_pool *p, uint64_t cap) {
   p->cap_data = cap;
 }
 
+void pool_read_ctl(rip_pool *p, bool main) {
+  RIP_HIP(hipMemcpyAsync(p->h_ctl, p->d_ctl, sizeof(unsigned long long) * kCtlWords,
+                         hipMemcpyDeviceToHost, p->stream));
+  RIP_HIP(hipStreamSynchronize(p->stream));
+  if (main)
+    for (int i = 0; i < RIP_STAT_COUNT; ++i) p->main_stats[i] = p->h_ctl[kCtlStatBytes + i];
+}
+
 void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
   if (count == 0) return;
   RIP_HIP(hipSetDevice(p->g->device));
@@ -315,30 +342,42 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
     // the launchers bracket their main kernel with ev0 / ev1
     uint32_t launches;
     if (p->g->model == RIP_IC)
-      launches = ic_sample(p, first, count, ic);
+      launches = ic_sample(p, first, count, ic, tune);
     else
       launches = lt_launch(p, first, count, tune);
+    RIP_HIP(hipEventRecord(p->ev2, p->stream));
     launch_sum_u32(p->d_len + first, count, p->d_ctl + kCtlElems, p->stream);
     unsigned long long *ctl = p->h_ctl;
     const size_t ctl_bytes = sizeof(unsigned long long) * kCtlWords;
     RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, ctl_bytes, hipMemcpyDeviceToHost, p->stream));
     RIP_HIP(hipStreamSynchronize(p->stream));
+    // LT: the walk kernel is the main one and the stats so far are its own;
+    // IC colour groups alone: one kernel
+    if (p->g->model == RIP_LT || ic.sampler == RIP_SAMPLER_COLOUR)
+      for (int i = 0; i < RIP_STAT_COUNT; ++i) p->main_stats[i] = ctl[kCtlStatBytes + i];
     if (p->g->model == RIP_LT && ctl[kCtlOverflow] == 0 && ctl[kCtlLongCount] != 0) {
       // walks longer than the walk kernel's lists, then the sizes again
       launches += lt_finish_long(p, first, count, ctl[kCtlLongCount]);
+      RIP_HIP(hipEventRecord(p->ev2, p->stream));
       hipLaunchKernelGGL(zero_elems_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
       launch_sum_u32(p->d_len + first, count, p->d_ctl + kCtlElems, p->stream);
       RIP_HIP(hipMemcpyAsync(ctl, p->d_ctl, ctl_bytes, hipMemcpyDeviceToHost, p->stream));
       RIP_HIP(hipStreamSynchronize(p->stream));
     }
     if (ctl[kCtlOverflow] == 0) {
-      float ms = 0.0f;
+      float ms = 0.0f, call = 0.0f;
       RIP_HIP(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+      RIP_HIP(hipEventElapsedTime(&call, p->ev0, p->ev2));
       p->last_kernel_ms = ms;
+      p->last_call_ms = call;
       p->last_launches = launches;
       for (int i = 0; i < RIP_STAT_COUNT; ++i) p->last_stats[i] = ctl[kCtlStatBytes + i];
-      const uint64_t added = ctl[kCtlDataBump] - p->used_data;
-      p->used_data = ctl[kCtlDataBump];
+      // a wave's last output chunk may reach past cap_data with no write
+      // there (only writes past it flag an overflow): every set lies below
+      // cap_data, so the cursor is clamped to it
+      const uint64_t bump = std::min<uint64_t>(ctl[kCtlDataBump], p->cap_data);
+      const uint64_t added = bump - p->used_data;
+      p->used_data = bump;
       p->num_elements += ctl[kCtlElems];
       p->avg_set_size = std::max(1.0, static_cast<double>(added) / static_cast<double>(count));
       p->num_sets += count;

@@ -16,11 +16,12 @@
 //                             went 0 -> nonzero (each vertex at most once per
 //                             level); masks are read and cleared when a chunk
 //                             of the queue is staged into LDS
-//   expansion               : the chunk's rows are split into runs of RUN
-//                             consecutive edges, balanced over all lanes by a
-//                             workgroup scan + binary search in LDS; a run
-//                             jumps to its first coin with one table lookup
-//                             and steps the lcg64 per edge
+//   expansion               : the chunk's rows are split into their gap
+//                             blocks (ic_gap.h), balanced over all lanes by a
+//                             workgroup scan + binary search in LDS; a lane
+//                             draws the candidates of its block for each
+//                             colour pending at the row, one live edge per
+//                             step of the wave
 //   discovery               : atomicOr on visited, then on pending; queue and
 //                             union bookkeeping are wave-aggregated
 //                             (ballot/popcount) into LDS counters
@@ -32,6 +33,8 @@
 #include <algorithm>
 
 #include "device_util.h"
+#include "ic_common.h"
+#include "ic_gap.h"
 #include "internal.h"
 
 namespace rip {
@@ -62,7 +65,6 @@ struct IcParams {
   uint32_t heavy_deg;   // rows at least this long wait for the hub phase
   uint32_t *hub;        // slots x 2 x hub_cap deferred-hub lists
   uint64_t hub_cap;
-  uint32_t jlevels;     // 6-bit digits of the run index covered by the jump tables
   const uint32_t *sample_ids;  // optional: local index -> sample index (minus first)
   uint32_t precheck;    // read visited masks before their atomics (giant components)
 };
@@ -70,8 +72,6 @@ struct IcParams {
 __device__ inline uint64_t sample_id(const IcParams &P, uint64_t local) {
   return P.first + (P.sample_ids ? P.sample_ids[local] : local);
 }
-
-constexpr int kJumpLevels = 5;
 
 // Mark (u, bits) discovered.  Every lane of the wave must call it.  With
 // precheck the visited mask is read first and the atomic skipped when no
@@ -108,7 +108,7 @@ __device__ inline void discover(unsigned long long *vp, uint32_t *bm0, uint32_t 
   wave_append(push, u, s_qn, qnext);
 }
 
-template <int BT, int RUN, bool kPrecheck>
+template <int BT, bool kPrecheck>
 __global__ void __launch_bounds__(BT)
 ic_fused_kernel(IcParams P) {
   constexpr int NW = BT / 64;
@@ -122,11 +122,14 @@ ic_fused_kernel(IcParams P) {
   __shared__ uint32_t s_qn[2];   // light frontier queues
   __shared__ uint32_t s_hn[2];   // deferred-hub lists
   __shared__ uint32_t s_union;
-  __shared__ Affine s_jt[kJumpLevels][64];  // step^(RUN * d * 64^l)
+  __shared__ Affine s_p2[64];      // P^(2^j): jumps between gap blocks
   __shared__ uint32_t s_ebeg[BT];
   __shared__ uint32_t s_edeg[BT];
-  __shared__ uint32_t s_eu[BT];   // uniform row threshold or 0xFFFFFFFF
+  __shared__ uint32_t s_eT[BT];    // the row's largest threshold
+  __shared__ uint32_t s_eflags[BT];
+  __shared__ uint32_t s_eufull[BT];
   __shared__ uint32_t s_epre[BT];
+  __shared__ unsigned long long s_eq[BT];
   __shared__ unsigned long long s_ea[BT];
   __shared__ unsigned long long s_eb[BT];
   __shared__ unsigned long long s_em[BT];
@@ -146,12 +149,11 @@ ic_fused_kernel(IcParams P) {
   unsigned long long *masks = reinterpret_cast<unsigned long long *>(q1);
   uint32_t *const h0 = P.hub + slot * 2ull * P.hub_cap;
   uint32_t *const h1 = h0 + P.hub_cap;
-  // work counters (edges read, coins, expansions, levels)
-  unsigned long long st_edges = 0, st_coins = 0, st_exp = 0, st_levels = 0;
-  // jump tables: run k of a row starts RUN*k coins after the row's first
-  for (uint32_t i = tid; i < 64u * P.jlevels; i += BT) {
-    const uint32_t l = i >> 6, d = i & 63;
-    s_jt[l][d] = aff_pow(P.step, static_cast<uint64_t>(RUN) * d << (6 * l));
+  // work counters (candidate edges, draws, expansions, live edges, bytes)
+  unsigned long long st_edges = 0, st_coins = 0, st_exp = 0, st_live = 0, st_bytes = 0;
+  if (tid == 0) {
+    s_p2[0] = P.step;
+    for (int j = 1; j < 64; ++j) s_p2[j] = aff_then(s_p2[j - 1], s_p2[j - 1]);
   }
 
   for (;;) {
@@ -203,7 +205,6 @@ ic_fused_kernel(IcParams P) {
       if (tid == 0) {
         s_qn[lnext] = 0;
         if (hub_phase) s_hn[hfill ^ 1] = 0;
-        ++st_levels;
       }
       uint32_t *const hcur = hfill ? h1 : h0;
       const uint32_t *qsrc = hub_phase ? hcur : (lcur ? q1 : q0);
@@ -225,19 +226,19 @@ ic_fused_kernel(IcParams P) {
           }
           s_ebeg[tid] = r.begin;
           s_edeg[tid] = deg;
-          s_eu[tid] = (r.flags & kRowUniform) ? r.uthr : 0xFFFFFFFFu;
+          s_eT[tid] = r.thr;
+          s_eflags[tid] = r.flags;
+          s_eufull[tid] = r.ufull;
+          s_eq[tid] = r.q;
           s_ea[tid] = r.ja;
           s_eb[tid] = r.jb;
           s_em[tid] = M;
-          // one work item per run of RUN edges, for all its colours (items
-          // of at most 8 colours each, which balance the colour loop across
-          // lanes, measured 1.3x slower on R-MAT scale 18 U[0,0.1]: every
-          // item repeats the run's jump, thresholds and discovery atomics)
-          runs = M ? (deg + RUN - 1) / RUN : 0;
+          // one work item per gap block of the row, for all its colours; a
+          // row no edge of which can be live (T = 0) makes none
+          runs = (M && r.thr) ? ((deg - 1) >> ((r.flags >> kBetaShift) & 31u)) + 1 : 0;
           if (M) {
-            st_edges += deg;
-            st_coins += static_cast<unsigned long long>(deg) * __popcll(M);
             ++st_exp;
+            st_bytes += 64;
           }
         }
         wave_append(defer, v, &s_hn[hfill], hdefer);
@@ -247,13 +248,12 @@ ic_fused_kernel(IcParams P) {
         __syncthreads();
         for (uint32_t rb = 0; rb < total; rb += BT) {
           const uint32_t rr = rb + tid;
-          uint32_t src[RUN];
-          unsigned long long acc[RUN];
-#pragma unroll
-          for (int i = 0; i < RUN; ++i) {
-            src[i] = 0;
-            acc[i] = 0;
-          }
+          // this lane's item: block k of entry ent, for the colours in mm
+          unsigned long long mm = 0;
+          Affine E{1, 0};
+          uint32_t e0 = 0, len = 0, B = 0, T = 0, ufull = 0;
+          uint64_t q = 0;
+          bool uniform = true;
           if (rr < total) {
             uint32_t lo = 0, hi = cl;
             while (lo < hi) {
@@ -263,35 +263,68 @@ ic_fused_kernel(IcParams P) {
             }
             const uint32_t ent = lo - 1;
             const uint32_t k = rr - s_epre[ent];
-            const uint32_t j0 = k * RUN;
-            const uint32_t deg = s_edeg[ent];
-            const uint32_t e0 = s_ebeg[ent] + j0;
-            const unsigned long long M = s_em[ent];
-            Affine E{s_ea[ent], s_eb[ent]};
-            for (uint32_t l = 0; l < P.jlevels; ++l)
-              E = aff_then(E, s_jt[l][(k >> (6 * l)) & 63]);
-            const uint32_t uthr = s_eu[ent];
-            uint32_t thr[RUN];
-#pragma unroll
-            for (int i = 0; i < RUN; ++i)
-              thr[i] = j0 + i < deg ? (uthr != 0xFFFFFFFFu ? uthr : P.val[e0 + i]) : 0u;
-            for (unsigned long long mm = M; mm; mm &= mm - 1) {
-              const int c = __ffsll(mm) - 1;
-              uint64_t x = E.a * s_z[c] + E.b;
-#pragma unroll
-              for (int i = 0; i < RUN; ++i) {
-                if (static_cast<uint32_t>(x >> 40) < thr[i]) acc[i] |= 1ull << c;
-                if (i + 1 < RUN) x = P.step.a * x + P.step.b;
+            const uint32_t fl = s_eflags[ent];
+            const uint32_t beta = (fl >> kBetaShift) & 31u;
+            B = 1u << beta;
+            e0 = s_ebeg[ent] + (k << beta);
+            len = min(s_edeg[ent] - (k << beta), B);
+            T = s_eT[ent];
+            q = s_eq[ent];
+            ufull = s_eufull[ent];
+            uniform = (fl & kRowUniform) != 0;
+            mm = s_em[ent];
+            // block k's first element: P^(2 B k) after the row's first
+            E = Affine{s_ea[ent], s_eb[ent]};
+            for (uint32_t kk = k, bit = beta + 1; kk; kk >>= 1, ++bit)
+              if (kk & 1) E = aff_then(E, s_p2[bit]);
+          }
+          // one live (edge, colour) per lane per step; every lane takes part
+          // in each discovery
+          uint32_t pos = len, c = 0;
+          uint64_t x = 0;
+          for (;;) {
+            uint32_t src = 0;
+            unsigned long long bits = 0;
+            for (;;) {
+              if (pos >= len) {
+                if (mm == 0) break;
+                c = static_cast<uint32_t>(__ffsll(mm) - 1);
+                mm &= mm - 1;
+                x = E.a * s_z[c] + E.b;
+                pos = 0;
+                ++st_coins;
+                // a full block without candidates costs one compare
+                if (len == B && (x >> 32) + 1 <= ufull) pos = len;
+                continue;
+              }
+              const uint64_t gap = ic_gap(x, q);
+              if (gap >= len - pos) {
+                pos = len;
+                continue;
+              }
+              pos += static_cast<uint32_t>(gap);
+              const uint32_t e = e0 + pos;
+              const uint64_t a = P.step.a * x + P.step.b;   // acceptance element
+              ++st_edges;
+              if (!uniform) {
+                ++st_coins;
+                st_bytes += 4;
+              }
+              const bool live = uniform || ic_accept(a, T, P.val[e]);
+              ++pos;
+              x = P.step.a * a + P.step.b;                  // next gap element
+              if (pos < len) ++st_coins;
+              if (live) {
+                src = P.ids[e];
+                bits = 1ull << c;
+                ++st_live;
+                st_bytes += 4;
+                break;
               }
             }
-            // in-neighbour ids only for edges live for some colour
-#pragma unroll
-            for (int i = 0; i < RUN; ++i)
-              if (acc[i]) src[i] = P.ids[e0 + i];
+            if (__ballot(bits != 0) == 0) break;
+            discover<kPrecheck>(vp, bm0, bm1, src, bits, &s_union, &s_qn[lnext], qnext);
           }
-#pragma unroll
-          for (int i = 0; i < RUN; ++i)
-            discover<kPrecheck>(vp, bm0, bm1, src[i], acc[i], &s_union, &s_qn[lnext], qnext);
         }
         __syncthreads();
       }
@@ -366,42 +399,22 @@ ic_fused_kernel(IcParams P) {
         const uint64_t s = sample_id(P, s0 + tid);
         P.set_off[s] = base + s_cofs[tid];
         P.set_len[s] = s_cnt[tid];
+        st_bytes += 4ull * s_cnt[tid];
       }
     } else if (tid < nb) {
       P.set_len[sample_id(P, s0 + tid)] = 0xFFFFFFFFu;
     }
     __syncthreads();
   }
-  for (int d = 32; d > 0; d >>= 1) {
-    st_edges += __shfl_down(st_edges, d, 64);
-    st_coins += __shfl_down(st_coins, d, 64);
-    st_exp += __shfl_down(st_exp, d, 64);
-    st_levels += __shfl_down(st_levels, d, 64);
-  }
-  if (lane == 0) {
-    if (st_edges) atomicAdd(&P.ctl[kCtlStatEdges], st_edges);
-    if (st_coins) atomicAdd(&P.ctl[kCtlStatCoins], st_coins);
-    if (st_exp) atomicAdd(&P.ctl[kCtlStatExpansions], st_exp);
-    if (st_levels) atomicAdd(&P.ctl[kCtlStatLevels], st_levels);
-  }
-}
-
-template <int BT, int RUN>
-void launch_one(const IcParams &P, uint32_t slots, hipStream_t stream) {
-  if (P.precheck)
-    hipLaunchKernelGGL((ic_fused_kernel<BT, RUN, true>), dim3(slots), dim3(BT), 0, stream, P);
-  else
-    hipLaunchKernelGGL((ic_fused_kernel<BT, RUN, false>), dim3(slots), dim3(BT), 0, stream, P);
+  report_stats(P.ctl, st_bytes, st_edges, st_coins, st_exp, st_live);
 }
 
 template <int BT>
-void launch_run(const IcParams &P, uint32_t slots, uint32_t run,
-                hipStream_t stream) {
-  switch (run) {
-    case 4: launch_one<BT, 4>(P, slots, stream); break;
-    case 8: launch_one<BT, 8>(P, slots, stream); break;
-    default: launch_one<BT, 16>(P, slots, stream); break;
-  }
+void launch_one(const IcParams &P, uint32_t slots, hipStream_t stream) {
+  if (P.precheck)
+    hipLaunchKernelGGL((ic_fused_kernel<BT, true>), dim3(slots), dim3(BT), 0, stream, P);
+  else
+    hipLaunchKernelGGL((ic_fused_kernel<BT, false>), dim3(slots), dim3(BT), 0, stream, P);
 }
 
 __global__ void reset_counter_kernel(unsigned long long *ctl) {
@@ -409,25 +422,16 @@ __global__ void reset_counter_kernel(unsigned long long *ctl) {
 }
 
 // Workgroups of the persistent kernel that fit on one CU at once.
-template <int BT, int RUN>
-int occupancy_of() {
-  return blocks_per_cu(reinterpret_cast<const void *>(&ic_fused_kernel<BT, RUN, false>), BT);
-}
-
 template <int BT>
-int occupancy_run(uint32_t run) {
-  switch (run) {
-    case 4: return occupancy_of<BT, 4>();
-    case 8: return occupancy_of<BT, 8>();
-    default: return occupancy_of<BT, 16>();
-  }
+int occupancy_of() {
+  return blocks_per_cu(reinterpret_cast<const void *>(&ic_fused_kernel<BT, false>), BT);
 }
 
-int resident_blocks(uint32_t block, uint32_t run) {
+int resident_blocks(uint32_t block) {
   switch (block) {
-    case 256: return occupancy_run<256>(run);
-    case 512: return occupancy_run<512>(run);
-    default: return occupancy_run<1024>(run);
+    case 256: return occupancy_of<256>();
+    case 512: return occupancy_of<512>();
+    default: return occupancy_of<1024>();
   }
 }
 
@@ -452,7 +456,7 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
   // defaults from a sweep on R-MAT scale 22 weighted cascade: 8-edge runs and
   // 256-thread groups (more independent groups per CU); hub deferral off,
   // because colours of one group rarely meet at the same hub there
-  IcLaunch c{64, 256, 0, 8, 0xFFFFFFFFu, RIP_SAMPLER_AUTO, true, 0, true, true};
+  IcLaunch c{64, 256, 0, 0, 0xFFFFFFFFu, RIP_SAMPLER_AUTO, true, 0, true, true};
   if (tune) {
     if (tune->colors) {
       c.colors = tune->colors;
@@ -466,16 +470,16 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
       c.slots = tune->slots;
       c.slots_auto = false;
     }
-    if (tune->edge_run) c.edge_run = tune->edge_run;
+    c.queue_cap = tune->queue_cap;
     if (tune->heavy_degree) c.heavy_degree = tune->heavy_degree;
     c.sampler = tune->sampler;
     c.out_chunk = tune->out_chunk;
   }
   RIP_REQUIRE(c.colors >= 1 && c.colors <= 64, -1, "tune: colors must be in [1, 64]");
+  RIP_REQUIRE(c.sampler <= RIP_SAMPLER_LANE, -1, "tune: unknown sampler kind");
+  (void)ic_lane_cap(tune);  // validates queue_cap
   RIP_REQUIRE(c.block == 256 || c.block == 512 || c.block == 1024, -1,
               "tune: block must be 256, 512 or 1024");
-  RIP_REQUIRE(c.edge_run == 4 || c.edge_run == 8 || c.edge_run == 16, -1,
-              "tune: edge_run must be 4, 8 or 16");
   if (c.slots == 0) {
     size_t free_b = 0, total_b = 0;
     RIP_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -486,7 +490,7 @@ IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune) {
     if (p->g->ic.slots && p->g->ic.n == p->g->n) held = p->g->ic.slot_bytes * p->g->ic.slots;
     const size_t budget = std::min((free_b + held) / 8, static_cast<size_t>(24) << 30);
     uint64_t by_mem = std::max<uint64_t>(1, budget / per);
-    uint64_t by_cu = static_cast<uint64_t>(cu_count()) * resident_blocks(c.block, c.edge_run);
+    uint64_t by_cu = static_cast<uint64_t>(cu_count()) * resident_blocks(c.block);
     c.slots = static_cast<uint32_t>(std::min(by_mem, by_cu));
   }
   return c;
@@ -552,13 +556,6 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
   P.heavy_deg = cfg.heavy_degree;
   P.hub = p->g->ic.d_hub;
   P.hub_cap = p->g->ic.hub_cap;
-  // 6-bit digits needed for the largest run index of a row
-  const uint64_t max_runs = (static_cast<uint64_t>(g->max_degree) + cfg.edge_run - 1) /
-                            cfg.edge_run;
-  uint32_t levels = 1;
-  while (levels < kJumpLevels && (max_runs >> (6 * levels)) != 0) ++levels;
-  RIP_REQUIRE((max_runs >> (6 * levels)) == 0, -1, "graph: row too long for the jump tables");
-  P.jlevels = levels;
   P.sample_ids = ids;
   // the large-set fallback walks giant components, where most edges reach
   // vertices every colour has visited; a whole launch of mostly small sets
@@ -569,24 +566,38 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
   if (record_events) RIP_HIP(hipEventRecord(p->ev0, p->stream));
   switch (cfg.block) {
-    case 256: launch_run<256>(P, slots, cfg.edge_run, p->stream); break;
-    case 1024: launch_run<1024>(P, slots, cfg.edge_run, p->stream); break;
-    default: launch_run<512>(P, slots, cfg.edge_run, p->stream); break;
+    case 256: launch_one<256>(P, slots, p->stream); break;
+    case 1024: launch_one<1024>(P, slots, p->stream); break;
+    default: launch_one<512>(P, slots, p->stream); break;
   }
   RIP_HIP(hipGetLastError());
   if (record_events) RIP_HIP(hipEventRecord(p->ev1, p->stream));
   return 2;
 }
 
-uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &cfg) {
+uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &cfg,
+                   const rip_tune *tune) {
   if (cfg.sampler == RIP_SAMPLER_COLOUR) return ic_launch(p, first, count, cfg);
-  const uint64_t big = ic_wave_launch(p, first, count, cfg.out_chunk);
-  if (big == 0) return 1;
+  uint32_t launches = 0;
+  uint64_t wave_count = count;
+  const uint32_t *wave_ids = nullptr;
+  const bool lanes = cfg.sampler == RIP_SAMPLER_AUTO || cfg.sampler == RIP_SAMPLER_LANE;
+  if (lanes) {
+    // one sample per lane first; the few that outgrow a lane's queue go to
+    // one BFS per wave
+    wave_count = ic_lane_launch(p, first, count, tune);
+    launches += 3;
+    if (wave_count == 0) return launches;
+    wave_ids = p->g->wave.d_over;
+  }
+  const uint64_t big = ic_wave_launch(p, first, wave_count, wave_ids, cfg.out_chunk, !lanes);
+  launches += 2;
+  if (big == 0) return launches;
   // sets that outgrew a wave's list: by default colour groups over the listed
   // sample indices (samples that reach a giant component share most of their
-  // rows, which the groups read once for 64 of them); WAVE finishes them with
-  // the bitmap-visited BFS per wave instead
-  if (cfg.sampler == RIP_SAMPLER_WAVE) {
+  // rows, which the groups read once for 64 of them); WAVE and LANE finish
+  // them with the bitmap-visited BFS per wave instead
+  if (cfg.sampler == RIP_SAMPLER_WAVE || cfg.sampler == RIP_SAMPLER_LANE) {
     ic_big_launch(p, first, big);
   } else {
     // the groups are latency-bound, one level at a time: 512-thread groups
@@ -600,7 +611,7 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
       fb.block = 512;
       if (cfg.slots_auto) {
         fb.slots = std::min<uint32_t>(
-            cfg.slots, static_cast<uint32_t>(cu_count()) * resident_blocks(fb.block, fb.edge_run));
+            cfg.slots, static_cast<uint32_t>(cu_count()) * resident_blocks(fb.block));
       }
     }
     if (cfg.colors_auto) {
@@ -611,7 +622,7 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
     }
     ic_launch(p, first, big, fb, p->g->wave.d_big, false);
   }
-  return 3;
+  return launches + 2;
 }
 
 }  // namespace rip

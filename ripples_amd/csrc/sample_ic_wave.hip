@@ -2,31 +2,32 @@
 // workgroup barriers.
 //
 // Semantics are those of AddRRRSet (include/ripples/add_rrrset.h:105-157)
-// under the per-sample stream contract of common.h (the coin of edge e in
-// sample s is element 1 + e of s's stream), so the set is the reverse
-// closure of the root over live edges and equals the colour sampler's and
-// the oracle's bit for bit.
+// under the per-sample stream contract of common.h and ic_gap.h: the live
+// in-edges of a row depend only on (seed, sample, row), so the set is the
+// reverse closure of the root over live edges and equals the other
+// samplers' and the oracle's bit for bit.
 //
 // Layout per resident wave (HBM):
 //   table[TCAP]  visited set: open addressing, 64-bit entries
 //                (epoch << 32 | vertex); an entry counts only if its epoch
 //                is the wave's current one, so the table is never cleared
 //   list[CAP]    BFS queue and output list (FIFO in discovery order)
-// Expansion of a vertex: lane l reads in-edges begin + l + 64 i (coalesced)
-// and starts from coin P^l(x0), x0 the row's first coin, stepping P^64 per
-// round.  Only live edges touch the table.  The finished list is sorted in
-// LDS (bitonic, 1024 entries per wave) or, if longer, in its HBM buffer.
-// Samples whose sets outgrow CAP are listed for the colour sampler.
+// Expansion of a vertex: lane l draws the candidates of the row's gap
+// blocks l, l + 64, ... (one compare for a full block without candidates);
+// only live edges read their in-neighbour and touch the table.  The
+// finished list is sorted in LDS (bitonic, 1024 entries per wave) or, if
+// longer, in its HBM buffer.  Samples whose sets outgrow CAP are listed for
+// the large-set kernels.
 #include <algorithm>
 
 #include "device_util.h"
+#include "ic_common.h"
+#include "ic_gap.h"
 #include "internal.h"
 
 namespace rip {
 namespace {
 
-constexpr uint32_t kEmptyVertex = 0xFFFFFFFFu;
-constexpr uint32_t kLenFail = 0xFFFFFFFFu;
 constexpr uint32_t kSortCap = 1024;
 
 struct WaveParams {
@@ -49,16 +50,8 @@ struct WaveParams {
   uint64_t cap_data;
   unsigned long long *ctl;
   uint32_t chunk;              // pool elements a wave reserves at a time
+  const uint32_t *sample_ids;  // optional: local index -> sample index (minus first)
 };
-
-__device__ inline uint32_t hash_vertex(uint32_t v) {
-  v ^= v >> 16;
-  v *= 0x7feb352du;
-  v ^= v >> 15;
-  v *= 0x846ca68bu;
-  v ^= v >> 16;
-  return v;
-}
 
 // Insert v into the wave's table; true if it was not there yet.
 __device__ inline bool table_insert(unsigned long long *table, uint32_t mask,
@@ -82,105 +75,98 @@ __device__ inline bool table_insert(unsigned long long *table, uint32_t mask,
   }
 }
 
-// Bitonic sort of a[0, len) (len <= cap2, a power of two) padded with
-// kEmptyVertex; one wave, lanes over compare-exchange pairs.
-template <typename Ptr>
-__device__ inline void wave_bitonic(Ptr a, uint32_t len, uint32_t cap2) {
-  const uint32_t lane = lane_id();
-  for (uint32_t i = len + lane; i < cap2; i += 64) a[i] = kEmptyVertex;
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  for (uint32_t k = 2; k <= cap2; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t t = lane; t < cap2 / 2; t += 64) {
-        // pair (i, i ^ j) with i having bit j clear
-        const uint32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-        const uint32_t p = i | j;
-        const uint32_t x = a[i], y = a[p];
-        const bool up = (i & k) == 0;
-        if ((x > y) == up) {
-          a[i] = y;
-          a[p] = x;
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    }
-  }
-}
-
-// Expands row v of the sample whose stream starts at z: lane l evaluates
-// the coins of in-edges begin + l + 64 i (the row's first coin is P^(1+begin)
-// applied to z; jl = P^l, j64 = P^64), and the in-neighbours of live edges for
-// which visit(src) reports a first visit are appended to list[tail ...].
-// Returns false, leaving the row unfinished, when the live edges of a round
-// (an upper bound on the new vertices) would take the list past cap.
-template <int U, typename Visit>
+// Expands row v of the sample whose element 0 is z (ic_gap.h): lane l draws
+// the candidates of gap blocks l, l + 64, ... (block k's first element is
+// P^(2 B k) applied to the row's first, P^(1 + 2 begin) z); the in-neighbours
+// of live edges for which visit(src) reports a first visit are appended to
+// list[tail ...].  Every lane of the wave must call it.  Returns false,
+// leaving the row unfinished, when the live edges of a step (an upper bound
+// on the new vertices) would take the list past cap.
+template <typename Visit>
 __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
                                   const uint32_t *val, uint32_t v, uint64_t z,
-                                  const Affine &jl, const Affine &j64, uint32_t *list,
-                                  uint32_t &tail, uint32_t cap,
-                                  unsigned long long &st_edges, Visit &&visit) {
+                                  const Affine *p2, uint32_t *list, uint32_t &tail, uint32_t cap,
+                                  unsigned long long &st_cand, unsigned long long &st_draws,
+                                  unsigned long long &st_bytes, unsigned long long &st_live,
+                                  Visit &&visit) {
   const uint32_t lane = lane_id();
   // the row is the same for every lane: keep it in scalar registers
   const VertexRec rv = vrec[v];
+  const uint32_t T = uniform_u32(rv.thr);
+  if (T == 0) return true;                         // no edge of the row can be live
   const uint32_t rbegin = uniform_u32(rv.begin);
   const uint32_t deg = uniform_u32(rv.end) - rbegin;
-  const uint32_t uthr = uniform_u32(rv.uthr);
-  const bool uniform = (uniform_u32(rv.flags) & kRowUniform) != 0;
-  st_edges += deg;
-  if (uniform && uthr == 0) return true;           // weight 0: no edge of the row is live
-  const uint64_t x0 = uniform_u64(rv.ja) * z + uniform_u64(rv.jb);  // row's first coin
-  uint64_t x = jl.a * x0 + jl.b;                    // coin of edge begin + lane
-  for (uint32_t j0 = 0; j0 < deg; j0 += 64 * U) {
-    // thresholds: from the vertex record for uniform rows, else one
-    // coalesced 4-byte load per edge; thr 0 past the row end (only the last
-    // chunk of a row needs the bounds test)
-    const uint32_t e0 = rbegin + j0 + lane;
-    uint32_t thr[U];
-    if (j0 + 64 * U <= deg) {
-      if (uniform) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) thr[u] = uthr;
-      } else {
-#pragma unroll
-        for (int u = 0; u < U; ++u) thr[u] = val[e0 + 64 * u];
-      }
-    } else {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t j = j0 + lane + 64 * u;
-        thr[u] = j < deg ? (uniform ? uthr : val[e0 + 64 * u]) : 0u;
-      }
+  const uint32_t flags = uniform_u32(rv.flags);
+  const bool uniform = (flags & kRowUniform) != 0;
+  const uint32_t beta = (flags >> kBetaShift) & 31u;
+  const uint32_t B = 1u << beta;
+  const uint64_t q = uniform_u64(rv.q);
+  const uint64_t ufull = uniform_u32(rv.ufull);
+  const uint32_t nblk = ((deg - 1) >> beta) + 1;
+  const Affine P = p2[0];
+  const uint64_t x0 = uniform_u64(rv.ja) * z + uniform_u64(rv.jb);  // row's first element
+  // this lane's first block: P^(2B lane) from the binary digits of lane
+  Affine jl{1, 0};
+  for (uint32_t k = lane, bit = beta + 1; k; k >>= 1, ++bit)
+    if (k & 1) jl = aff_then(jl, p2[bit]);
+  const Affine j64 = p2[beta + 7];                 // 64 blocks further
+  uint64_t xb = aff_apply(jl, x0);
+  for (uint32_t k0 = 0; k0 < nblk; k0 += 64, xb = aff_apply(j64, xb)) {
+    const uint32_t k = k0 + lane;
+    uint32_t len = 0;
+    if (k < nblk) {
+      len = min(deg - (k << beta), B);
+      ++st_draws;
+      // a full block without candidates costs one compare
+      if (len == B && (xb >> 32) + 1 <= ufull) len = 0;
     }
-    // all U coins first; one ballot tells whether any edge of the chunk is
-    // live (rare: about one edge per row under weighted cascade)
-    uint32_t bits = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (j0 + 64 * u >= deg) break;  // past the row end (last chunk only)
-      bits |= static_cast<uint32_t>(x >> 40) < thr[u] ? (1u << u) : 0u;
-      x = j64.a * x + j64.b;
-    }
-    if (__ballot(bits != 0) == 0) continue;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const bool live = ((bits >> u) & 1u) != 0;
+    uint32_t pos = 0;
+    uint64_t x = xb;
+    bool first = true;
+    for (;;) {
+      bool live = false;
+      uint32_t e = 0;
+      if (pos < len) {
+        if (!first) ++st_draws;
+        first = false;
+        const uint64_t gap = ic_gap(x, q);
+        if (gap >= len - pos) {
+          pos = len;
+        } else {
+          pos += static_cast<uint32_t>(gap);
+          e = rbegin + (k << beta) + pos;
+          const uint64_t a = aff_apply(P, x);     // acceptance element
+          ++st_cand;
+          if (!uniform) {
+            ++st_draws;
+            st_bytes += 4;
+          }
+          live = uniform || ic_accept(a, T, val[e]);
+          if (live) {
+            ++st_live;
+            st_bytes += 4;
+          }
+          ++pos;
+          x = aff_apply(P, a);                     // next gap element
+        }
+      }
       const uint64_t lm = __ballot(live);
-      if (lm == 0) continue;
-      if (tail + __popcll(lm) > cap) return false;
-      // the in-neighbour id is read only for live edges
-      const uint32_t src = live ? ids[e0 + 64 * u] : 0u;
-      const bool fresh = live && visit(src);
-      const uint64_t fm = __ballot(fresh);
-      if (fresh) list[tail + __popcll(fm & lanes_below())] = src;
-      tail += __popcll(fm);
+      if (lm) {
+        if (tail + __popcll(lm) > cap) return false;
+        // the in-neighbour id is read only for live edges
+        const uint32_t src = live ? ids[e] : 0u;
+        const bool fresh = live && visit(src);
+        const uint64_t fm = __ballot(fresh);
+        if (fresh) list[tail + __popcll(fm & lanes_below())] = src;
+        tail += __popcll(fm);
+      }
+      if (__ballot(pos < len) == 0) break;
     }
   }
   return true;
 }
 
-template <int BT, int U>
+template <int BT>
 __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
   constexpr int NW = BT / 64;
   __shared__ uint32_t s_sort[NW][kSortCap];
@@ -194,16 +180,18 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
   // the wave's reserved output [out_cur, out_end) (wave-uniform); the rest
   // of a chunk that cannot take the next set stays a gap in the pool's data
   unsigned long long out_cur = 0, out_end = 0;
-  const Affine jl = aff_pow(P.step, lane);    // coin offset of this lane
-  const Affine j64 = aff_pow(P.step, 64);     // one round of 64 edges
-  unsigned long long st_edges = 0, st_exp = 0;
+  __shared__ Affine s_p2[64];                 // P^(2^j): jumps between gap blocks
+  if (threadIdx.x == 0) fill_step_powers(P.step, s_p2);
+  __syncthreads();
+  unsigned long long st_cand = 0, st_draws = 0, st_exp = 0, st_bytes = 0, st_live = 0;
 
   for (;;) {
     unsigned long long idx = 0;
     if (lane == 0) idx = atomicAdd(&P.ctl[kCtlGroupCounter], 1ull);
     idx = uniform_u64(__shfl(idx, 0, 64));
     if (idx >= P.count) break;
-    const uint64_t s = P.first + idx;
+    const uint32_t local = P.sample_ids ? P.sample_ids[idx] : static_cast<uint32_t>(idx);
+    const uint64_t s = P.first + local;
     ++epoch;
     const unsigned long long tag = static_cast<unsigned long long>(epoch) << 32;
     const uint64_t z = sample_z(P.seed, s);
@@ -218,14 +206,15 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       const uint32_t v = uniform_u32(list[head++]);
       ++st_exp;
-      if (!expand_row<U>(P.vrec, P.ids, P.val, v, z, jl, j64, list, tail, P.cap, st_edges,
-                         [&](uint32_t src) { return table_insert(table, tmask, tag, src); }))
+      if (!expand_row(P.vrec, P.ids, P.val, v, z, s_p2, list, tail, P.cap, st_cand, st_draws,
+                      st_bytes, st_live,
+                      [&](uint32_t src) { return table_insert(table, tmask, tag, src); }))
         big = true;
     }
     if (big) {
       if (lane == 0) {
         const unsigned long long b = atomicAdd(&P.ctl[kCtlLongCount], 1ull);
-        P.big_ids[b] = static_cast<uint32_t>(idx);
+        P.big_ids[b] = local;
       }
       continue;
     }
@@ -266,19 +255,12 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
     if (lane == 0) {
       P.set_off[s] = base;
       P.set_len[s] = len;
+      st_bytes += 4ull * len;
     }
   }
   if (lane == 0) P.epoch[wave] = epoch;
-  for (int d = 32; d > 0; d >>= 1) {
-    st_edges += __shfl_down(st_edges, d, 64);
-    st_exp += __shfl_down(st_exp, d, 64);
-  }
-  // every lane counted the same rows: report lane 0's share once
-  if (lane == 0) {
-    atomicAdd(&P.ctl[kCtlStatEdges], st_edges / 64);
-    atomicAdd(&P.ctl[kCtlStatCoins], st_edges / 64);
-    atomicAdd(&P.ctl[kCtlStatExpansions], st_exp / 64);
-  }
+  if (lane == 0) st_bytes += kIcRecBytes * st_exp;
+  report_stats(P.ctl, st_bytes, st_cand, st_draws, lane == 0 ? st_exp : 0ull, st_live);
 }
 
 // ------------------------------------------------------------ large sets
@@ -309,7 +291,7 @@ struct BigParams {
   unsigned long long *ctl;
 };
 
-template <int BT, int U>
+template <int BT>
 __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
   constexpr int NW = BT / 64;
   const uint32_t lane = lane_id();
@@ -317,9 +299,10 @@ __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
   uint32_t *bits = P.bits + wave * P.words;
   uint32_t *sum = P.sum + wave * P.swords;
   uint32_t *list = P.list + wave * P.n;
-  const Affine jl = aff_pow(P.step, lane);
-  const Affine j64 = aff_pow(P.step, 64);
-  unsigned long long st_edges = 0, st_exp = 0;
+  __shared__ Affine s_p2[64];
+  if (threadIdx.x == 0) fill_step_powers(P.step, s_p2);
+  __syncthreads();
+  unsigned long long st_cand = 0, st_draws = 0, st_exp = 0, st_bytes = 0, st_live = 0;
   auto visit = [&](uint32_t u) {
     const uint32_t w = u >> 5, bit = 1u << (u & 31);
     const uint32_t old = atomicOr(&bits[w], bit);
@@ -346,8 +329,8 @@ __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
       // every vertex enters the list once, so it never passes n entries; the
       // cap test of expand_row counts live edges, not new vertices, so it
       // gets no bound here (with cap n it could stop a row near the end)
-      expand_row<U>(P.vrec, P.ids, P.val, v, z, jl, j64, list, tail, 0xFFFFFFFFu, st_edges,
-                    visit);
+      expand_row(P.vrec, P.ids, P.val, v, z, s_p2, list, tail, 0xFFFFFFFFu, st_cand, st_draws,
+                 st_bytes, st_live, visit);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const uint32_t len = tail;
@@ -403,40 +386,34 @@ __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
       if (fits) {
         P.set_off[s] = base;
         P.set_len[s] = len;
+        st_bytes += 4ull * len;
       } else {
         atomicOr(&P.ctl[kCtlOverflow], 1ull);
         P.set_len[s] = kLenFail;
       }
     }
   }
-  for (int d = 32; d > 0; d >>= 1) {
-    st_edges += __shfl_down(st_edges, d, 64);
-    st_exp += __shfl_down(st_exp, d, 64);
-  }
-  if (lane == 0) {
-    atomicAdd(&P.ctl[kCtlStatEdges], st_edges / 64);
-    atomicAdd(&P.ctl[kCtlStatCoins], st_edges / 64);
-    atomicAdd(&P.ctl[kCtlStatExpansions], st_exp / 64);
-  }
+  if (lane == 0) st_bytes += kIcRecBytes * st_exp;
+  report_stats(P.ctl, st_bytes, st_cand, st_draws, lane == 0 ? st_exp : 0ull, st_live);
 }
 
 __global__ void reset_big_kernel(unsigned long long *ctl) { ctl[kCtlBigCounter] = 0; }
 
-template <int BT, int U>
+template <int BT>
 uint32_t wave_grid() {
   return static_cast<uint32_t>(
-      blocks_per_cu(reinterpret_cast<const void *>(&ic_wave_kernel<BT, U>), BT) * cu_count());
+      blocks_per_cu(reinterpret_cast<const void *>(&ic_wave_kernel<BT>), BT) * cu_count());
 }
 
 }  // namespace
 
 // Returns the number of samples left for the colour sampler (their local
 // indices are in g->wave.d_big).
-constexpr int kWaveBT = 256, kWaveU = 4;
+constexpr int kWaveBT = 256;
 
 uint32_t ic_wave_blocks(uint64_t count) {
   return std::max<uint32_t>(
-      1, std::min<uint64_t>(wave_grid<kWaveBT, kWaveU>(),
+      1, std::min<uint64_t>(wave_grid<kWaveBT>(),
                             (count + kWaveBT / 64 - 1) / (kWaveBT / 64)));
 }
 
@@ -451,11 +428,21 @@ uint32_t ic_chunk(const rip_pool *p, uint64_t count, uint32_t out_chunk, uint64_
 
 uint64_t ic_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune) {
   const uint64_t waves = static_cast<uint64_t>(ic_wave_blocks(count)) * (kWaveBT / 64);
-  return waves * ic_chunk(p, count, tune ? tune->out_chunk : 0, waves);
+  uint64_t slack = waves * ic_chunk(p, count, tune ? tune->out_chunk : 0, waves);
+  // the lane kernel runs first unless a sampler kind without it is asked for
+  const uint32_t kind = tune ? tune->sampler : RIP_SAMPLER_AUTO;
+  if (kind == RIP_SAMPLER_AUTO || kind == RIP_SAMPLER_LANE) slack += ic_lane_slack(p, count, tune);
+  return slack;
 }
 
-uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, uint32_t out_chunk) {
-  constexpr int BT = kWaveBT, U = kWaveU;
+__global__ void reset_wave_kernel(unsigned long long *ctl) {
+  ctl[kCtlGroupCounter] = 0;
+  ctl[kCtlLongCount] = 0;
+}
+
+uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
+                        uint32_t out_chunk, bool record_events) {
+  constexpr int BT = kWaveBT;
   constexpr uint32_t CAP = 4096, TCAP = 8192;
   rip_graph *g = p->g;
   WaveScratch &w = g->wave;
@@ -499,15 +486,14 @@ uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, uint32_t ou
   P.cap_data = p->cap_data;
   P.ctl = p->d_ctl;
   P.chunk = ic_chunk(p, count, out_chunk, waves);
-  RIP_HIP(hipEventRecord(p->ev0, p->stream));
-  hipLaunchKernelGGL((ic_wave_kernel<BT, U>), dim3(blocks), dim3(BT), 0, p->stream, P);
+  P.sample_ids = ids;
+  hipLaunchKernelGGL(reset_wave_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
+  if (record_events) RIP_HIP(hipEventRecord(p->ev0, p->stream));
+  hipLaunchKernelGGL((ic_wave_kernel<BT>), dim3(blocks), dim3(BT), 0, p->stream, P);
   RIP_HIP(hipGetLastError());
-  RIP_HIP(hipEventRecord(p->ev1, p->stream));
-  unsigned long long nbig = 0;
-  RIP_HIP(hipMemcpyAsync(&nbig, p->d_ctl + kCtlLongCount, sizeof(nbig),
-                         hipMemcpyDeviceToHost, p->stream));
-  RIP_HIP(hipStreamSynchronize(p->stream));
-  return nbig;
+  if (record_events) RIP_HIP(hipEventRecord(p->ev1, p->stream));
+  pool_read_ctl(p, record_events);
+  return p->h_ctl[kCtlLongCount];
 }
 
 }  // namespace rip
@@ -519,14 +505,14 @@ namespace rip {
 // words; the number of waves is bounded by occupancy and by 1/16 of the
 // device memory.
 void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count) {
-  constexpr int BT = 256, U = 4;
+  constexpr int BT = 256;
   rip_graph *g = p->g;
   WaveScratch &w = g->wave;
   const uint32_t words = (g->n + 31) / 32, swords = (words + 31) / 32;
   const uint64_t per_wave = 4ull * (static_cast<uint64_t>(g->n) + words + swords);
   size_t free_b = 0, total_b = 0;
   RIP_HIP(hipMemGetInfo(&free_b, &total_b));
-  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(&ic_big_kernel<BT, U>), BT);
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(&ic_big_kernel<BT>), BT);
   const int cus = cu_count();
   uint64_t waves = std::min<uint64_t>(static_cast<uint64_t>(per_cu) * cus * (BT / 64),
                                       std::max<uint64_t>(1, (total_b / 16) / per_wave));
@@ -566,7 +552,7 @@ void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count) {
   P.cap_data = p->cap_data;
   P.ctl = p->d_ctl;
   hipLaunchKernelGGL(reset_big_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
-  hipLaunchKernelGGL((ic_big_kernel<BT, U>), dim3(static_cast<uint32_t>(waves / (BT / 64))),
+  hipLaunchKernelGGL((ic_big_kernel<BT>), dim3(static_cast<uint32_t>(waves / (BT / 64))),
                      dim3(BT), 0, p->stream, P);
   RIP_HIP(hipGetLastError());
 }

@@ -70,7 +70,7 @@ def test_selection_matches_oracle(case, orc):
     p.close()
 
 
-@pytest.mark.parametrize("sampler", [0, 1])
+@pytest.mark.parametrize("sampler", [0, 1, 3])
 def test_ic_bimodal_sets_match_oracle(sampler, orc):
     """R-MAT scale 16 with U[0,0.1] weights: most sets are small, a fifth reach
     the giant component of ~12K vertices (the wave kernel hands those to the
@@ -172,14 +172,16 @@ def test_selector_graph_replays_across_pool_growth(orc):
     g.close()
 
 
-@pytest.mark.parametrize("tune", [dict(colors=1), dict(colors=17, block=256),
-                                  dict(colors=64, block=512, edge_run=4),
-                                  dict(colors=32, block=1024, edge_run=8, slots=3),
-                                  dict(colors=64, block=256, edge_run=16, slots=1),
-                                  dict(colors=64, heavy_degree=1),
-                                  dict(colors=64, heavy_degree=1 << 30),
-                                  dict(colors=7, heavy_degree=3, edge_run=4),
-                                  dict(sampler=1), dict(sampler=2),
+@pytest.mark.parametrize("tune", [dict(colors=1, sampler=2), dict(colors=17, block=256, sampler=2),
+                                  dict(colors=64, block=512, sampler=2),
+                                  dict(colors=32, block=1024, slots=3, sampler=2),
+                                  dict(colors=64, block=256, slots=1, sampler=2),
+                                  dict(colors=64, heavy_degree=1, sampler=2),
+                                  dict(colors=64, heavy_degree=1 << 30, sampler=2),
+                                  dict(colors=7, heavy_degree=3, sampler=2),
+                                  dict(sampler=1), dict(sampler=2), dict(sampler=3),
+                                  dict(queue_cap=64), dict(queue_cap=256, sampler=3),
+                                  dict(queue_cap=64, out_chunk=3),
                                   dict(sampler=2, colors=64, heavy_degree=64),
                                   dict(out_chunk=1), dict(out_chunk=5, sampler=1),
                                   dict(out_chunk=1 << 20)])
@@ -226,7 +228,7 @@ def test_pool_growth_retry(orc):
     g.close()
 
 
-@pytest.mark.parametrize("sampler", [0, 1])
+@pytest.mark.parametrize("sampler", [0, 1, 3])
 @pytest.mark.parametrize("n", [1500, 3000, 6000, 20000, 70000])
 def test_ic_large_sets(n, sampler, orc):
     """Sets of ~n vertices: LDS sort (<= 1024), HBM sort (<= 4096) and, for

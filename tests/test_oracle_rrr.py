@@ -39,6 +39,29 @@ def test_ic_traversals_agree_on_random_graphs(orc, seed):
     check_well_formed(orc, ref[0], ref[1], n, seed, first=40)
 
 
+@pytest.mark.parametrize("whi", [1.0, 0.05])
+def test_ic_traversals_agree_on_multi_block_rows(orc, whi):
+    """Hub rows of ~400 in-edges span many gap blocks (16 edges at weights
+    near 1, 512 at 0.05); some weights are exactly 0 or 1."""
+    rng = np.random.default_rng(5)
+    n = 500
+    s = np.concatenate([rng.integers(0, n, 4000), np.arange(n), np.arange(n)])
+    d = np.concatenate([rng.integers(0, n, 4000), np.full(n, 3), np.full(n, 77)])
+    keep = s != d
+    key = np.unique(s[keep] * n + d[keep])
+    s, d = key // n, key % n
+    w = rng.uniform(0.0, whi, len(s)).astype(np.float32)
+    w[rng.random(len(s)) < 0.05] = 0.0
+    if whi == 1.0:
+        w[rng.random(len(s)) < 0.05] = 1.0
+    index, src, w = transposed_csr(n, s, d, w)
+    ref = orc.sample(orc.IC, index, src, w, 4, 0, 400, algo=0)
+    for algo in (1, 2):
+        got = orc.sample(orc.IC, index, src, w, 4, 0, 400, algo=algo)
+        assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+    check_well_formed(orc, ref[0], ref[1], n, 4)
+
+
 def test_lt_definition_agrees(orc):
     n, s, d, w = random_graph(200, 2000, 3)
     w = lt_normalise(n, s, d, w, 0.9)
