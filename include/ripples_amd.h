@@ -160,6 +160,16 @@ int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
 int rip_pool_reset(rip_pool *p);
 int rip_pool_size(const rip_pool *p, uint64_t *num_sets,
                   uint64_t *num_elements);
+/* The pool's store of large sets: IC samples that outgrow one BFS per wave
+ * are drawn 64 at a time by colour groups, and a group whose sets hold more
+ * than 2 * hot_nodes members among the graph's hot vertices (a quarter of
+ * the vertices, those with the most out-edge weight) keeps those members as
+ * one bit per (hot vertex, set) in a column of 64-bit words, the rest as
+ * sorted lists (the reference keeps every set as a sorted list,
+ * rrr_sets.h:81-105).  *groups = columns in use, *hot_nodes = column height,
+ * *column_bytes = device bytes of the columns allocated. */
+int rip_pool_columns(const rip_pool *p, uint32_t *groups, uint32_t *hot_nodes,
+                     uint64_t *column_bytes);
 /* Copy sets [first, first+count) to host memory as sorted uint32 lists in
  * CSR form: offsets uint64[count+1] (offsets[0] = 0), vertices uint32[...].
  * With vertices == NULL only offsets are written (a size query).  cap is the

@@ -136,6 +136,8 @@ def lib():
     L.rip_pool_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double),
                                           C.POINTER(C.c_uint32)]
     L.rip_pool_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
+    L.rip_pool_columns.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_uint64)]
     L.rip_pool_last_kernel_stats.argtypes = [vp, _u64p, C.c_uint32]
     L.rip_pool_last_stats.argtypes = [vp, _u64p, C.c_uint32]
     L.rip_imm.argtypes = [vp, C.c_uint32, C.c_double, C.c_uint64, C.POINTER(Tune),
@@ -427,6 +429,12 @@ class Pool:
         out = np.zeros(len(self.STAT_NAMES), np.uint64)
         _check(lib().rip_pool_last_stats(self._h, out, len(out)))
         return {k: int(v) for k, v in zip(self.STAT_NAMES, out)}
+
+    def columns(self):
+        """(column groups in use, hot vertices per column, column bytes allocated)."""
+        g, h, b = C.c_uint32(0), C.c_uint32(0), C.c_uint64(0)
+        _check(lib().rip_pool_columns(self._h, C.byref(g), C.byref(h), C.byref(b)))
+        return g.value, h.value, b.value
 
     def last_kernel_stats(self):
         """Work counters of the last sampling call's main kernel alone."""

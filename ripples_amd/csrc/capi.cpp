@@ -139,6 +139,17 @@ int rip_pool_reset(rip_pool *p) {
     p->num_sets = 0;
     p->used_data = 0;
     p->num_elements = 0;
+    p->groups = 0;
+  });
+}
+
+int rip_pool_columns(const rip_pool *p, uint32_t *groups, uint32_t *hot_nodes,
+                     uint64_t *column_bytes) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    if (groups) *groups = p->groups;
+    if (hot_nodes) *hot_nodes = p->g->hot_n;
+    if (column_bytes) *column_bytes = 8ull * p->g->hot_n * p->cap_groups;
   });
 }
 

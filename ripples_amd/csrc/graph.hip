@@ -15,6 +15,7 @@
 //               padded to whole blocks (breakpoint 0xFFFFFFFF).
 //   d_bidx    : LT fanout-16 index over the breakpoints of rows longer than 16
 // The id and value arrays carry kEdgePad zero elements for 16-byte loads.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -112,7 +113,63 @@ __global__ void build_bidx_kernel(const uint32_t *rows, uint32_t nrows, LtRec *l
   for (int t = 0; t < 16; ++t) r.top[t] = bidx[r.ioff + t];
 }
 
+// key[u] += the IC threshold of every transposed entry with source u: the
+// expected number of live out-edges of u, in units of 2^-24 (integer, so
+// the sums do not depend on the order of the atomics).
+__global__ void out_weight_kernel(const uint32_t *ids, const uint32_t *val, uint64_t m,
+                                  unsigned long long *key) {
+  for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+       e += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    if (val[e]) atomicAdd(&key[ids[e]], static_cast<unsigned long long>(val[e]));
+}
+
 }  // namespace
+
+// The column store's hot vertices: a quarter of the vertices (a multiple of
+// 64, at least min(n, 64)), those most likely to be reached from others,
+// ranked by the sum of their out-edge thresholds.  On R-MAT scale 24 with
+// U[0,0.02] weights they hold 96% of the members of the giant sets.
+void graph_ensure_hot(rip_graph *g) {
+  if (g->d_hot_index) return;
+  RIP_REQUIRE(g->model == RIP_IC, -1, "graph: the column store is for IC graphs");
+  const uint32_t n = g->n;
+  const uint32_t hot = std::max<uint32_t>(std::min<uint32_t>(n, 64u), (n / 4) & ~63u);
+  std::vector<unsigned long long> key(n);
+  unsigned long long *d_key = dev_alloc<unsigned long long>(n);
+  try {
+    RIP_HIP(hipMemset(d_key, 0, sizeof(unsigned long long) * n));
+    if (g->m)
+      hipLaunchKernelGGL(out_weight_kernel, dim3(2048), dim3(256), 0, 0, g->d_ids, g->d_val,
+                         g->m, d_key);
+    RIP_HIP(hipGetLastError());
+    RIP_HIP(hipMemcpy(key.data(), d_key, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
+  } catch (...) {
+    dev_free(d_key);
+    throw;
+  }
+  dev_free(d_key);
+  std::vector<uint32_t> order(n);
+  for (uint32_t v = 0; v < n; ++v) order[v] = v;
+  auto better = [&](uint32_t a, uint32_t b) { return key[a] != key[b] ? key[a] > key[b] : a < b; };
+  if (hot < n) std::nth_element(order.begin(), order.begin() + hot, order.end(), better);
+  std::vector<uint32_t> index(n, kNotHot), vertex(order.begin(), order.begin() + hot);
+  std::sort(vertex.begin(), vertex.end());
+  for (uint32_t h = 0; h < hot; ++h) index[vertex[h]] = h;
+  uint32_t *d_index = dev_alloc<uint32_t>(n);
+  uint32_t *d_vertex = nullptr;
+  try {
+    d_vertex = dev_alloc<uint32_t>(hot);
+    RIP_HIP(hipMemcpy(d_index, index.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    RIP_HIP(hipMemcpy(d_vertex, vertex.data(), sizeof(uint32_t) * hot, hipMemcpyHostToDevice));
+  } catch (...) {
+    dev_free(d_index);
+    dev_free(d_vertex);
+    throw;
+  }
+  g->d_hot_index = d_index;
+  g->d_hot_vertex = d_vertex;
+  g->hot_n = hot;
+}
 
 void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
                   const uint64_t *index, const uint32_t *edges,
@@ -270,6 +327,8 @@ void graph_destroy(rip_graph *g) {
   if (!g) return;
   (void)hipSetDevice(g->device);
   dev_free(g->d_vrec);
+  dev_free(g->d_hot_index);
+  dev_free(g->d_hot_vertex);
   dev_free(g->d_ids);
   dev_free(g->d_val);
   dev_free(g->ic.d_vp);

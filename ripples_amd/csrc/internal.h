@@ -96,6 +96,15 @@ struct Args {
   unsigned long long *partial;  // per-block maxima of the argmax pass
   uint64_t num_sets;
   uint32_t n;
+  // column store (groups == 0: none)
+  const uint32_t *gref;
+  unsigned long long *const *colptr;
+  const uint32_t *hot_index;
+  const uint32_t *hot_vertex;
+  unsigned long long *gkill;    // per group: colours killed this round
+  uint32_t *kgroups;            // the groups with gkill != 0 (ctl[kCtlKillGroups])
+  uint32_t groups;
+  uint32_t hot_n;
 };
 
 struct SelectScratch {
@@ -118,6 +127,13 @@ struct SelectScratch {
   hipGraphExec_t graph = nullptr;
   uint32_t graph_k = 0;
   bool graph_dense = false;
+  bool graph_cols = false;
+  // column store: per group the colours of the selected prefix, the colours
+  // killed in the current round, and the list of groups with such kills
+  uint32_t cap_groups = 0;
+  unsigned long long *d_gactive = nullptr;
+  unsigned long long *d_gkill = nullptr;
+  uint32_t *d_kgroups = nullptr;
 };
 
 }  // namespace rip
@@ -137,6 +153,12 @@ struct rip_graph {
                                      // padded to multiples of 16 pairs
   uint32_t *d_bidx = nullptr;        // LT: fanout-16 index of rows longer than 16
   uint64_t deg_ge_pow2[33] = {};     // vertices with in-degree >= 2^b
+  // IC: the "hot" vertices of the column store (pool.hip): the hot_n
+  // vertices with the largest sum of out-edge thresholds (ties to the lower
+  // id), numbered in ascending vertex order; built on first use
+  uint32_t hot_n = 0;
+  uint32_t *d_hot_index = nullptr;   // n: index among the hot vertices, or kNotHot
+  uint32_t *d_hot_vertex = nullptr;  // hot_n: the hot vertices, ascending
   // Sampler scratch shared by every pool of this graph; pool_sample holds
   // sampler_mu while a launch uses it.
   std::mutex sampler_mu;
@@ -145,6 +167,12 @@ struct rip_graph {
   rip::LtScratch lt;
   rip::FwdGraph fwd;
 };
+
+namespace rip {
+constexpr uint32_t kNotHot = 0xFFFFFFFFu;
+constexpr uint32_t kNoGroup = 0xFFFFFFFFu;
+constexpr uint32_t kColBlock = 16;  // column groups per device allocation
+}  // namespace rip
 
 struct rip_pool {
   rip_graph *g = nullptr;
@@ -159,6 +187,14 @@ struct rip_pool {
   uint64_t used_data = 0;     // next free element of d_data
   uint64_t num_elements = 0;  // sum of set sizes: used_data less the gaps the
                               // LT sampler leaves after its waves' output chunks
+  // column store (pool.hip): groups of up to 64 large sets; set s of a
+  // group keeps its members among the graph's hot vertices as bit c of the
+  // group's column (one 64-bit word per hot vertex) and the rest as its list
+  uint32_t groups = 0, cap_groups = 0;
+  std::vector<unsigned long long *> col_blocks;  // kColBlock columns each
+  unsigned long long **d_colptr = nullptr;       // cap_groups column pointers
+  uint32_t *d_gsample = nullptr;                 // cap_groups x 64 sample ids (kNoGroup: none)
+  uint32_t *d_gref = nullptr;                    // cap_sets: group << 6 | colour, or kNoGroup
   unsigned long long *d_ctl = nullptr;  // control words, see pool.hip
   unsigned long long *h_ctl = nullptr;  // pinned host copy of the control words
   rip::SelectScratch sel;
@@ -190,7 +226,10 @@ enum : int {
   kCtlKillSets = 12,     // selector, dense pools: sets killed this round
   kCtlKillElems = 13,    // selector, dense pools: their elements
   kCtlElems = 14,        // elements of the sets of the last sampling launch
-  kCtlWords = 16,
+  kCtlKillGroups = 15,   // selector, column store: groups with sets killed this round
+  kCtlGroupCounter2 = 16,  // IC colour groups: columns taken by the launch
+  kCtlScratch2 = 17,     // a word kernels may count into and nobody reads
+  kCtlWords = 19,        // the last word is scratch too
 };
 
 // graph.hip
@@ -205,6 +244,11 @@ void pool_create_resources(rip_pool *p);  // stream, events, control words
 void pool_destroy(rip_pool *p);
 void pool_reserve_sets(rip_pool *p, uint64_t cap_sets);
 void pool_reserve_data(rip_pool *p, uint64_t cap_data);
+// Column store: the graph's hot vertices (built on first use), room for
+// `groups` column groups, and d_gref set to kNoGroup for sets [first, end).
+void graph_ensure_hot(rip_graph *g);
+void pool_reserve_groups(rip_pool *p, uint32_t groups);
+void pool_clear_gref(rip_pool *p, uint64_t first, uint64_t end);
 // Resident blocks per CU of a kernel at a block size (at least 1), and the
 // CU count of the current device; both cached, so launchers can ask on
 // every call.
@@ -243,9 +287,11 @@ inline uint32_t out_chunk_rule(double elems_per_wave, uint32_t max_chunk) {
 IcLaunch ic_tune(const rip_pool *p, const rip_tune *tune);
 void ic_ensure_slots(rip_pool *p, uint32_t slots, uint32_t heavy_deg);
 // Colour sampler over samples first + i, i in [0, count), or first + ids[i].
+// With columns, groups whose sets hold many hot members store them in the
+// pool's column store (pool.hip).
 uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
                    const IcLaunch &cfg, const uint32_t *ids = nullptr,
-                   bool record_events = true);
+                   bool record_events = true, bool columns = false);
 
 // IC sampling of samples [first, first+count) with the configured sampler
 uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &cfg,

@@ -95,6 +95,7 @@ __global__ void set_ctl_kernel(unsigned long long *ctl, unsigned long long bump)
   ctl[kCtlGroupCounter] = 0;
   ctl[kCtlLongCount] = 0;
   ctl[kCtlElems] = 0;
+  ctl[kCtlGroupCounter2] = 0;
   for (int i = kCtlStatBytes; i <= kCtlStatLevels; ++i) ctl[i] = 0;
 }
 
@@ -147,6 +148,45 @@ __global__ void gather_sets(const unsigned long long *set_off,
       else hi = mid;
     }
     out[obase + f] = data[s_off[wid][lo] + (f - s_pre[wid][lo])];
+  }
+}
+
+// Column store export: hot members of giant set j (entry g << 6 | c).
+constexpr uint32_t kHotBT = 1024;
+
+__global__ void __launch_bounds__(kHotBT) hot_size_kernel(unsigned long long *const *colptr,
+                                                          const uint32_t *ent, uint32_t hot_n,
+                                                          unsigned long long *sizes) {
+  __shared__ unsigned long long s_tmp[kHotBT / 64];
+  const uint32_t e = ent[blockIdx.x];
+  const unsigned long long *col = colptr[e >> 6];
+  const uint32_t c = e & 63u;
+  unsigned long long cnt = 0;
+  for (uint32_t h = threadIdx.x; h < hot_n; h += kHotBT) cnt += (col[h] >> c) & 1ull;
+  unsigned long long tot;
+  (void)block_exclusive_scan_u64<kHotBT>(cnt, s_tmp, &tot);
+  if (threadIdx.x == 0) sizes[blockIdx.x] = tot;
+}
+
+// The hot members of set j in ascending vertex order (hot vertices are
+// numbered in vertex order) at out[hoff[j] ...].
+__global__ void __launch_bounds__(kHotBT) hot_extract_kernel(unsigned long long *const *colptr,
+                                                             const uint32_t *ent, uint32_t hot_n,
+                                                             const uint32_t *hot_vertex,
+                                                             const unsigned long long *hoff,
+                                                             uint32_t *out) {
+  __shared__ unsigned long long s_tmp[kHotBT / 64];
+  const uint32_t e = ent[blockIdx.x];
+  const unsigned long long *col = colptr[e >> 6];
+  const uint32_t c = e & 63u;
+  unsigned long long run = hoff[blockIdx.x];
+  for (uint32_t h0 = 0; h0 < hot_n; h0 += kHotBT) {
+    const uint32_t h = h0 + threadIdx.x;
+    const unsigned long long bit = h < hot_n ? (col[h] >> c) & 1ull : 0ull;
+    unsigned long long tot;
+    const unsigned long long pre = block_exclusive_scan_u64<kHotBT>(bit, s_tmp, &tot);
+    if (bit) out[run + pre] = hot_vertex[h];
+    run += tot;
   }
 }
 
@@ -210,6 +250,12 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->d_len);
   dev_free(p->d_data);
   dev_free(p->d_ctl);
+  dev_free(p->d_gref);
+  dev_free(p->d_colptr);
+  dev_free(p->d_gsample);
+  for (auto *b : p->col_blocks) dev_free(b);
+  p->col_blocks.clear();
+  p->groups = p->cap_groups = 0;
   dev_free(p->sel.d_counts);
   dev_free(p->sel.d_inv_start);
   dev_free(p->sel.d_cursor);
@@ -219,6 +265,9 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->sel.d_keys);
   dev_free(p->sel.d_partial);
   dev_free(p->sel.d_args);
+  dev_free(p->sel.d_gactive);
+  dev_free(p->sel.d_gkill);
+  dev_free(p->sel.d_kgroups);
   if (p->sel.graph) (void)hipGraphExecDestroy(p->sel.graph);
   if (p->h_ctl) (void)hipHostFree(p->h_ctl);
   p->h_ctl = nullptr;
@@ -236,11 +285,13 @@ void pool_reserve_sets(rip_pool *p, uint64_t cap) {
   if (cap <= p->cap_sets) return;
   cap = std::max(cap, 2 * p->cap_sets);
   auto *off = dev_alloc<unsigned long long>(cap);
-  uint32_t *len;
+  uint32_t *len = nullptr, *gref = nullptr;
   try {
     len = dev_alloc<uint32_t>(cap);
+    if (p->d_gref) gref = dev_alloc<uint32_t>(cap);
   } catch (...) {
     dev_free(off);
+    dev_free(len);
     throw;
   }
   if (p->num_sets) {
@@ -248,13 +299,76 @@ void pool_reserve_sets(rip_pool *p, uint64_t cap) {
                            hipMemcpyDeviceToDevice, p->stream));
     RIP_HIP(hipMemcpyAsync(len, p->d_len, sizeof(uint32_t) * p->num_sets,
                            hipMemcpyDeviceToDevice, p->stream));
+    if (gref)
+      RIP_HIP(hipMemcpyAsync(gref, p->d_gref, sizeof(uint32_t) * p->num_sets,
+                             hipMemcpyDeviceToDevice, p->stream));
   }
   RIP_HIP(hipStreamSynchronize(p->stream));
   dev_free(p->d_off);
   dev_free(p->d_len);
+  dev_free(p->d_gref);
   p->d_off = off;
   p->d_len = len;
+  p->d_gref = gref;
   p->cap_sets = cap;
+}
+
+namespace {
+__global__ void fill_u32_range_kernel(uint32_t *p, uint64_t first, uint64_t end, uint32_t v) {
+  for (uint64_t i = first + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < end; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+}  // namespace
+
+void pool_clear_gref(rip_pool *p, uint64_t first, uint64_t end) {
+  if (!p->d_gref || end <= first) return;
+  const uint64_t blocks = std::min<uint64_t>((end - first + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_u32_range_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+                     p->stream, p->d_gref, first, end, kNoGroup);
+  RIP_HIP(hipGetLastError());
+}
+
+// Room for `need` column groups: columns are allocated kColBlock at a time,
+// so the store grows without moving them; the pointer and sample tables are
+// reallocated.  The first call also sets up d_gref (every set so far a plain
+// list).
+void pool_reserve_groups(rip_pool *p, uint32_t need) {
+  graph_ensure_hot(p->g);
+  if (!p->d_gref) {
+    p->d_gref = dev_alloc<uint32_t>(std::max<uint64_t>(p->cap_sets, 1));
+    pool_clear_gref(p, 0, std::max<uint64_t>(p->cap_sets, 1));
+  }
+  if (need <= p->cap_groups) return;
+  const uint32_t cap = (need + kColBlock - 1) / kColBlock * kColBlock;
+  const uint64_t hot = p->g->hot_n;
+  std::vector<unsigned long long *> ptrs(cap);
+  if (p->cap_groups)
+    RIP_HIP(hipMemcpy(ptrs.data(), p->d_colptr, sizeof(void *) * p->cap_groups,
+                      hipMemcpyDeviceToHost));
+  for (uint32_t b = p->cap_groups; b < cap; b += kColBlock) {
+    unsigned long long *blk = dev_alloc<unsigned long long>(kColBlock * hot);
+    p->col_blocks.push_back(blk);
+    for (uint32_t i = 0; i < kColBlock; ++i) ptrs[b + i] = blk + i * hot;
+  }
+  unsigned long long **colptr = dev_alloc<unsigned long long *>(cap);
+  uint32_t *gsample = nullptr;
+  try {
+    gsample = dev_alloc<uint32_t>(64ull * cap);
+    RIP_HIP(hipMemcpy(colptr, ptrs.data(), sizeof(void *) * cap, hipMemcpyHostToDevice));
+    if (p->cap_groups)
+      RIP_HIP(hipMemcpy(gsample, p->d_gsample, sizeof(uint32_t) * 64 * p->cap_groups,
+                        hipMemcpyDeviceToDevice));
+  } catch (...) {
+    dev_free(colptr);
+    dev_free(gsample);
+    throw;
+  }
+  dev_free(p->d_colptr);
+  dev_free(p->d_gsample);
+  p->d_colptr = colptr;
+  p->d_gsample = gsample;
+  p->cap_groups = cap;
 }
 
 int blocks_per_cu(const void *kernel, int block) {
@@ -339,6 +453,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
     RIP_REQUIRE(attempt < 40, -2, "pool: data capacity did not converge");
     hipLaunchKernelGGL(set_ctl_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl,
                        static_cast<unsigned long long>(p->used_data));
+    pool_clear_gref(p, first, first + count);
     // the launchers bracket their main kernel with ev0 / ev1
     uint32_t launches;
     if (p->g->model == RIP_IC)
@@ -379,6 +494,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
       const uint64_t added = bump - p->used_data;
       p->used_data = bump;
       p->num_elements += ctl[kCtlElems];
+      p->groups += static_cast<uint32_t>(ctl[kCtlGroupCounter2]);
       p->avg_set_size = std::max(1.0, static_cast<double>(added) / static_cast<double>(count));
       p->num_sets += count;
       return;
@@ -422,26 +538,72 @@ void pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
                     hipMemcpyHostToDevice));
   RIP_HIP(hipMemcpy(p->d_len + p->num_sets, len.data(), sizeof(uint32_t) * count,
                     hipMemcpyHostToDevice));
+  pool_clear_gref(p, p->num_sets, p->num_sets + count);
   p->num_sets += count;
   p->used_data += total;
   p->num_elements += total;
 }
 
+// Sets [first, first + count) in CSR form on the host.  A set held in the
+// column store is its list (the members outside the hot vertices) merged
+// with its hot members, both ascending.
 void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
                  uint32_t *vertices, uint64_t cap) {
   RIP_REQUIRE(first + count <= p->num_sets && first + count >= first, -1,
               "export: range beyond the pool");
   RIP_REQUIRE(offsets != nullptr, -1, "export: offsets is NULL");
   RIP_HIP(hipSetDevice(p->g->device));
+  // sets of the range held partly in columns: (index in the range, entry)
+  std::vector<uint64_t> gidx;
+  std::vector<uint32_t> gent;
+  if (p->groups && p->d_gref && count) {
+    std::vector<uint32_t> gref(count);
+    RIP_HIP(hipMemcpy(gref.data(), p->d_gref + first, sizeof(uint32_t) * count,
+                      hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < count; ++i)
+      if (gref[i] != kNoGroup) {
+        gidx.push_back(i);
+        gent.push_back(gref[i]);
+      }
+  }
   auto *pre = dev_alloc<unsigned long long>(count + 1);
-  uint32_t *out = nullptr;
+  uint32_t *out = nullptr, *d_ent = nullptr, *hot = nullptr;
+  unsigned long long *hsz = nullptr;
   try {
     exclusive_scan_u32(p->d_len + first, pre, count, p->stream);
-    RIP_HIP(hipMemcpy(offsets, pre, sizeof(uint64_t) * (count + 1), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> coff(count + 1);
+    RIP_HIP(hipMemcpy(coff.data(), pre, sizeof(uint64_t) * (count + 1), hipMemcpyDeviceToHost));
+    const uint64_t cold_total = coff[count];
+    // hot member counts of the column-held sets
+    const uint64_t ng = gidx.size();
+    std::vector<unsigned long long> hoff(ng + 1, 0);
+    if (ng) {
+      d_ent = dev_alloc<uint32_t>(ng);
+      hsz = dev_alloc<unsigned long long>(ng + 1);
+      RIP_HIP(hipMemcpy(d_ent, gent.data(), sizeof(uint32_t) * ng, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(hot_size_kernel, dim3(static_cast<uint32_t>(ng)), dim3(kHotBT), 0,
+                         p->stream, p->d_colptr, d_ent, p->g->hot_n, hsz);
+      RIP_HIP(hipGetLastError());
+      std::vector<unsigned long long> sz(ng);
+      RIP_HIP(hipMemcpyAsync(sz.data(), hsz, sizeof(unsigned long long) * ng,
+                             hipMemcpyDeviceToHost, p->stream));
+      RIP_HIP(hipStreamSynchronize(p->stream));
+      for (uint64_t j = 0; j < ng; ++j) hoff[j + 1] = hoff[j] + sz[j];
+    }
+    // final offsets: list part plus hot part of each set
+    offsets[0] = 0;
+    for (uint64_t i = 0, j = 0; i < count; ++i) {
+      uint64_t add = coff[i + 1] - coff[i];
+      if (j < ng && gidx[j] == i) {
+        add += hoff[j + 1] - hoff[j];
+        ++j;
+      }
+      offsets[i + 1] = offsets[i] + add;
+    }
     const uint64_t total = offsets[count];
     if (vertices) {
       RIP_REQUIRE(total <= cap, -1, "export: vertices buffer too small");
-      out = dev_alloc<uint32_t>(total);
+      out = dev_alloc<uint32_t>(cold_total);
       const uint64_t waves = (count + 63) / 64;
       if (waves) {
         hipLaunchKernelGGL(gather_sets, dim3(static_cast<uint32_t>((waves + 3) / 4)),
@@ -449,17 +611,54 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
                            first, count, pre, out);
         RIP_HIP(hipGetLastError());
       }
+      std::vector<uint32_t> hot_h;
+      if (ng) {
+        hot = dev_alloc<uint32_t>(hoff[ng]);
+        RIP_HIP(hipMemcpy(hsz, hoff.data(), sizeof(unsigned long long) * (ng + 1),
+                          hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(hot_extract_kernel, dim3(static_cast<uint32_t>(ng)), dim3(kHotBT), 0,
+                           p->stream, p->d_colptr, d_ent, p->g->hot_n, p->g->d_hot_vertex, hsz,
+                           hot);
+        RIP_HIP(hipGetLastError());
+        hot_h.resize(hoff[ng]);
+      }
       RIP_HIP(hipStreamSynchronize(p->stream));
-      if (total)
-        RIP_HIP(hipMemcpy(vertices, out, sizeof(uint32_t) * total, hipMemcpyDeviceToHost));
+      if (!ng) {
+        if (total)
+          RIP_HIP(hipMemcpy(vertices, out, sizeof(uint32_t) * total, hipMemcpyDeviceToHost));
+      } else {
+        std::vector<uint32_t> cold_h(cold_total);
+        if (cold_total)
+          RIP_HIP(hipMemcpy(cold_h.data(), out, sizeof(uint32_t) * cold_total,
+                            hipMemcpyDeviceToHost));
+        if (!hot_h.empty())
+          RIP_HIP(hipMemcpy(hot_h.data(), hot, sizeof(uint32_t) * hot_h.size(),
+                            hipMemcpyDeviceToHost));
+        for (uint64_t i = 0, j = 0; i < count; ++i) {
+          const uint32_t *c0 = cold_h.data() + coff[i], *c1 = cold_h.data() + coff[i + 1];
+          if (j < ng && gidx[j] == i) {
+            std::merge(c0, c1, hot_h.data() + hoff[j], hot_h.data() + hoff[j + 1],
+                       vertices + offsets[i]);
+            ++j;
+          } else {
+            std::copy(c0, c1, vertices + offsets[i]);
+          }
+        }
+      }
     }
   } catch (...) {
     dev_free(pre);
     dev_free(out);
+    dev_free(d_ent);
+    dev_free(hsz);
+    dev_free(hot);
     throw;
   }
   dev_free(pre);
   dev_free(out);
+  dev_free(d_ent);
+  dev_free(hsz);
+  dev_free(hot);
 }
 
 }  // namespace rip

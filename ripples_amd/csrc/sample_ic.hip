@@ -67,6 +67,15 @@ struct IcParams {
   uint64_t hub_cap;
   const uint32_t *sample_ids;  // optional: local index -> sample index (minus first)
   uint32_t precheck;    // read visited masks before their atomics (giant components)
+  // column store (nullptr: every set a list): a group whose sets hold more
+  // than 2 hot_n hot members in all takes the next column (columns
+  // group_base, group_base + 1, ... as counted in ctl[kCtlGroupCounter2])
+  unsigned long long *const *colptr;
+  uint32_t *gsample;
+  uint32_t *gref;
+  const uint32_t *hot_index;
+  uint32_t hot_n;
+  uint32_t group_base;
 };
 
 __device__ inline uint64_t sample_id(const IcParams &P, uint64_t local) {
@@ -133,6 +142,8 @@ ic_fused_kernel(IcParams P) {
   __shared__ unsigned long long s_ea[BT];
   __shared__ unsigned long long s_eb[BT];
   __shared__ unsigned long long s_em[BT];
+  __shared__ unsigned long long s_hot[BT / 64];
+  __shared__ uint32_t s_col;       // the group's column, or kNoGroup
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
@@ -364,6 +375,48 @@ ic_fused_kernel(IcParams P) {
       running += tot;
     }
     __syncthreads();
+    if (P.colptr) {
+      // per-group choice of representation: a column costs 8 bytes per hot
+      // vertex, lists 4 per member; the group's hot members decide
+      unsigned long long hot = 0;
+      for (uint32_t i = tid; i < U; i += BT)
+        if (P.hot_index[list[i]] != kNotHot) hot += __popcll(masks[i]);
+      unsigned long long tot_hot;
+      (void)block_exclusive_scan_u64<BT>(hot, s_hot, &tot_hot);
+      if (tid == 0) {
+        s_col = kNoGroup;
+        if (tot_hot > 2ull * P.hot_n)
+          s_col = P.group_base +
+                  static_cast<uint32_t>(atomicAdd(&P.ctl[kCtlGroupCounter2], 1ull));
+      }
+      __syncthreads();
+      const uint32_t colg = s_col;
+      if (colg != kNoGroup) {
+        unsigned long long *col = P.colptr[colg];
+        for (uint32_t h = tid; h < P.hot_n; h += BT) col[h] = 0ull;
+        __syncthreads();
+        // hot members go to the column, which the lists then leave out
+        for (uint32_t i = tid; i < U; i += BT) {
+          const uint32_t h = P.hot_index[list[i]];
+          if (h != kNotHot) {
+            col[h] = masks[i];
+            masks[i] = 0ull;
+          }
+        }
+        st_bytes += 8ull * ((P.hot_n + BT - 1 - tid) / BT);
+        // the pool's element count includes the members held in the column
+        if (tid == 0) atomicAdd(&P.ctl[kCtlElems], tot_hot);
+        if (tid < 64) {
+          uint32_t sid = kNoGroup;
+          if (tid < nb) {
+            sid = static_cast<uint32_t>(sample_id(P, s0 + tid));
+            P.gref[sid] = (colg << 6) | tid;
+          }
+          P.gsample[64ull * colg + tid] = sid;
+        }
+      }
+      __syncthreads();
+    }
     for (uint32_t i = tid; i < U; i += BT)
       for (unsigned long long mk = masks[i]; mk; mk &= mk - 1)
         atomicAdd(&s_cnt[__ffsll(mk) - 1], 1u);
@@ -524,7 +577,8 @@ void ic_ensure_slots(rip_pool *p, uint32_t slots, uint32_t heavy_deg) {
 }
 
 uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
-                   const IcLaunch &cfg, const uint32_t *ids, bool record_events) {
+                   const IcLaunch &cfg, const uint32_t *ids, bool record_events,
+                   bool columns) {
   // no more resident groups than there are groups; scratch only ever grows
   const uint64_t groups = (count + cfg.colors - 1) / cfg.colors;
   ic_ensure_slots(p, static_cast<uint32_t>(std::min<uint64_t>(cfg.slots, groups)),
@@ -557,6 +611,23 @@ uint32_t ic_launch(rip_pool *p, uint64_t first, uint64_t count,
   P.hub = p->g->ic.d_hub;
   P.hub_cap = p->g->ic.hub_cap;
   P.sample_ids = ids;
+  P.colptr = nullptr;
+  P.gsample = nullptr;
+  P.gref = nullptr;
+  P.hot_index = nullptr;
+  P.hot_n = 0;
+  P.group_base = 0;
+  if (columns) {
+    // room for a column per group of the launch; groups whose sets are too
+    // small for one keep lists and leave theirs unused
+    pool_reserve_groups(p, p->groups + static_cast<uint32_t>(P.ngroups));
+    P.colptr = p->d_colptr;
+    P.gsample = p->d_gsample;
+    P.gref = p->d_gref;
+    P.hot_index = g->d_hot_index;
+    P.hot_n = g->hot_n;
+    P.group_base = p->groups;
+  }
   // the large-set fallback walks giant components, where most edges reach
   // vertices every colour has visited; a whole launch of mostly small sets
   // (weighted cascade) gains nothing from the extra read
@@ -620,7 +691,7 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
       while (c < 64 && static_cast<double>(c) * 1.41421356 < want) c <<= 1;
       fb.colors = c;
     }
-    ic_launch(p, first, big, fb, p->g->wave.d_big, false);
+    ic_launch(p, first, big, fb, p->g->wave.d_big, false, true);
   }
   return launches + 2;
 }

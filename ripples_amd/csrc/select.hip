@@ -289,6 +289,7 @@ __global__ void __launch_bounds__(kBT) argmax_kernel(const Args *a) {
     if (blockIdx.x == 0) {  // the dense pools' kill list of this round
       a->ctl[kCtlKillSets] = 0;
       a->ctl[kCtlKillElems] = 0;
+      a->ctl[kCtlKillGroups] = 0;
     }
   }
 }
@@ -355,6 +356,8 @@ __global__ void __launch_bounds__(kBT) scan_kill_kernel(const Args *a, uint32_t 
   const uint32_t *data = a->data;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBT;
   const uint64_t lane = lane_id();
+  // column store: the seed's place among the hot vertices
+  const uint32_t seed_hot = a->groups ? a->hot_index[seed] : kNotHot;
   for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBT + threadIdx.x - lane; w < num_sets;
        w += stride) {
     const uint64_t s = w + lane;
@@ -365,6 +368,19 @@ __global__ void __launch_bounds__(kBT) scan_kill_kernel(const Args *a, uint32_t 
       len = set_len[s];
       const uint32_t pos = lower_bound_u32(data + off, len, seed);
       hit = pos < len && data[off + pos] == seed;
+      const uint32_t e = a->groups ? a->gref[s] : kNoGroup;
+      if (e != kNoGroup) {
+        // a set of the column store: its hot members are bit c of its
+        // group's column; its hot members leave the counts in hot_dec_kernel
+        if (!hit && seed_hot != kNotHot) hit = (a->colptr[e >> 6][seed_hot] >> (e & 63u)) & 1ull;
+        if (hit) {
+          const unsigned long long prev = atomicOr(&a->gkill[e >> 6], 1ull << (e & 63u));
+          if (prev == 0) {
+            const unsigned long long at = atomicAdd(&a->ctl[kCtlKillGroups], 1ull);
+            a->kgroups[at] = e >> 6;
+          }
+        }
+      }
       if (hit) covered[s] = 1;
     }
     const uint64_t m = __ballot(hit);
@@ -404,13 +420,107 @@ __global__ void __launch_bounds__(kBT) dec_small_kernel(const Args *a) {
   }
 }
 
+// ------------------------------------------------------------ column store
+// Sets held in the column store keep their hot members as bit c of their
+// group's column: word h of the column holds, for the group's sets, whether
+// hot vertex h is a member.
+
+// gactive[g] = the colours of group g whose sample id is below num_sets.
+__global__ void group_active_kernel(const uint32_t *gsample, uint32_t groups, uint64_t num_sets,
+                                    unsigned long long *gactive, unsigned long long *gkill) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < groups;
+       g += gridDim.x * blockDim.x) {
+    unsigned long long m = 0;
+    for (uint32_t c = 0; c < 64; ++c) {
+      const uint32_t sid = gsample[64ull * g + c];
+      if (sid != kNoGroup && sid < num_sets) m |= 1ull << c;
+    }
+    gactive[g] = m;
+    gkill[g] = 0;
+  }
+}
+
+// counts[hot vertex h] += its memberships among the active column sets.
+__global__ void __launch_bounds__(kBT) hot_count_kernel(unsigned long long *const *colptr,
+                                                        const unsigned long long *gactive,
+                                                        uint32_t groups, const uint32_t *hot_vertex,
+                                                        uint32_t hot_n, uint32_t *counts) {
+  for (uint32_t h = blockIdx.x * kBT + threadIdx.x; h < hot_n; h += gridDim.x * kBT) {
+    uint32_t acc = 0;
+    for (uint32_t g = 0; g < groups; ++g) acc += __popcll(colptr[g][h] & gactive[g]);
+    if (acc) counts[hot_vertex[h]] += acc;
+  }
+}
+
+// Seed round, column store: the hot members of the column sets killed this
+// round leave the counts (one pass over the columns of the groups with
+// kills; none in most rounds).
+__global__ void __launch_bounds__(kBT) hot_dec_kernel(const Args *a) {
+  const uint32_t nk = static_cast<uint32_t>(a->ctl[kCtlKillGroups]);
+  if (nk == 0) return;
+  const uint32_t *kg = a->kgroups;
+  for (uint32_t h = blockIdx.x * kBT + threadIdx.x; h < a->hot_n; h += gridDim.x * kBT) {
+    uint32_t d = 0;
+    for (uint32_t i = 0; i < nk; ++i) {
+      const uint32_t g = kg[i];
+      d += __popcll(a->colptr[g][h] & a->gkill[g]);
+    }
+    if (d) a->counts[a->hot_vertex[h]] -= d;
+  }
+}
+
+__global__ void clear_gkill_kernel(const Args *a) {
+  const uint32_t nk = static_cast<uint32_t>(a->ctl[kCtlKillGroups]);
+  for (uint32_t i = threadIdx.x; i < nk; i += blockDim.x) a->gkill[a->kgroups[i]] = 0;
+}
+
+// Coverage of the column sets not already hit through their lists: a hit
+// when a flagged seed is a hot member.
+__global__ void __launch_bounds__(kBT) coverage_hot_kernel(const uint32_t *gref,
+                                                           unsigned long long *const *colptr,
+                                                           const uint32_t *hot_index,
+                                                           uint64_t num_sets, const uint32_t *seeds,
+                                                           uint32_t num_seeds, uint8_t *hit) {
+  for (uint64_t s = static_cast<uint64_t>(blockIdx.x) * kBT + threadIdx.x; s < num_sets;
+       s += static_cast<uint64_t>(gridDim.x) * kBT) {
+    const uint32_t e = gref[s];
+    if (e == kNoGroup || hit[s]) continue;
+    const unsigned long long *col = colptr[e >> 6];
+    for (uint32_t j = 0; j < num_seeds; ++j) {
+      const uint32_t h = hot_index[seeds[j]];
+      if (h != kNotHot && ((col[h] >> (e & 63u)) & 1ull)) {
+        hit[s] = 1;
+        break;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBT) count_u8_kernel(const uint8_t *v, uint64_t n,
+                                                       unsigned long long *out) {
+  __shared__ unsigned long long s_w[kBT / 64];
+  unsigned long long acc = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBT + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kBT)
+    acc += v[i];
+  for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    acc = 0;
+    for (uint32_t w = 0; w < kBT / 64; ++w) acc += s_w[w];
+    if (acc) atomicAdd(out, acc);
+  }
+}
+
 // Sets [0, num_sets) that contain a flagged vertex (FindCoverage,
 // include/ripples/opimc.h:100-118): each wave walks 64 sets, marks hits in
 // LDS and adds their popcount to *out once per wave.
+template <bool kBits>
 __global__ void __launch_bounds__(kBT)
 coverage_kernel(const unsigned long long *set_off, const uint32_t *set_len,
                 const uint32_t *data, uint64_t num_sets, const uint32_t *flags,
-                unsigned long long *out) {
+                unsigned long long *out, uint8_t *hit_out = nullptr) {
   __shared__ uint32_t s_pre[kBT / 64][65];
   __shared__ unsigned long long s_off[kBT / 64][64];
   __shared__ uint32_t s_hit[kBT / 64][64];
@@ -436,10 +546,11 @@ coverage_kernel(const unsigned long long *set_off, const uint32_t *set_len,
         else hi = mid;
       }
       const uint32_t v = data[s_off[wid][lo] + (i - s_pre[wid][lo])];
-      if (flags[v]) s_hit[wid][lo] = 1;
+      if (kBits ? (flags[v >> 5] >> (v & 31)) & 1u : flags[v]) s_hit[wid][lo] = 1;
     }
     __builtin_amdgcn_wave_barrier();
     acc += __popcll(__ballot(s_hit[wid][lane] != 0));
+    if (hit_out && s < num_sets) hit_out[s] = s_hit[wid][lane] != 0;
     __builtin_amdgcn_wave_barrier();
   }
   if (lane == 0 && acc) atomicAdd(out, acc);
@@ -449,6 +560,8 @@ __global__ void flag_seeds_kernel(const uint32_t *seeds, uint32_t k, uint32_t *f
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x)
     flags[seeds[i]] = 1;
 }
+
+
 
 // inv_elems: elements of the inverted index (0: none); kill_sets: length of
 // the dense pools' kill list (0: none).
@@ -484,6 +597,17 @@ void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t inv_elems,
     dev_free(s.d_keys);
     s.cap_k = k;
     s.d_keys = dev_alloc<unsigned long long>(k);
+  }
+  if (s.cap_groups < p->groups) {
+    dev_free(s.d_gactive);
+    dev_free(s.d_gkill);
+    dev_free(s.d_kgroups);
+    s.cap_groups = 0;
+    const uint32_t cap = std::max<uint32_t>(p->groups, p->cap_groups);
+    s.d_gactive = dev_alloc<unsigned long long>(cap);
+    s.d_gkill = dev_alloc<unsigned long long>(cap);
+    s.d_kgroups = dev_alloc<uint32_t>(cap);
+    s.cap_groups = cap;
   }
   if (!s.d_args) s.d_args = dev_alloc<Args>(1);
   if (!s.d_partial) s.d_partial = dev_alloc<unsigned long long>(kArgmaxBlocks);
@@ -528,7 +652,10 @@ bool build_counts(rip_pool *p, uint64_t num_sets, uint32_t k, Need need) {
   RIP_HIP(hipMemcpyAsync(&total, p->d_ctl + kCtlWords - 1, sizeof(total),
                          hipMemcpyDeviceToHost, st));
   RIP_HIP(hipStreamSynchronize(st));
-  const bool dense = is_dense(n, num_sets, total);
+  // pools with column sets take the dense kernels, whose seed rounds scan
+  // every uncovered set and so also test the column sets' hot members
+  const bool cols = p->groups > 0;
+  const bool dense = cols ? num_sets > 0 : is_dense(n, num_sets, total);
   const bool sel = need == Need::Select;
   ensure_select_scratch(p, sel ? num_sets : 0, sel && !dense ? total : 0,
                         sel && dense ? num_sets : 0, k);
@@ -540,6 +667,13 @@ bool build_counts(rip_pool *p, uint64_t num_sets, uint32_t k, Need need) {
     hipLaunchKernelGGL(dense_count_kernel, dim3(dense_chunks(ranges, num_sets) * ranges),
                        dim3(kDenseBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets, n,
                        ranges, s.d_counts);
+    if (cols) {
+      hipLaunchKernelGGL(group_active_kernel, dim3(grid_for(p->groups, kBT, 1024)), dim3(kBT), 0,
+                         st, p->d_gsample, p->groups, num_sets, s.d_gactive, s.d_gkill);
+      hipLaunchKernelGGL(hot_count_kernel, dim3(grid_for(p->g->hot_n, kBT, 8192)), dim3(kBT), 0,
+                         st, p->d_colptr, s.d_gactive, p->groups, p->g->d_hot_vertex,
+                         p->g->hot_n, s.d_counts);
+    }
   } else if (num_sets) {
     hipLaunchKernelGGL(count_kernel, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
                        dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets,
@@ -554,7 +688,7 @@ bool build_counts(rip_pool *p, uint64_t num_sets, uint32_t k, Need need) {
 // kernels of which one does the work).  Only kernels are recorded (no memset
 // nodes); the capture mode is thread-local so other host threads' streams
 // are unaffected.
-void capture_rounds(rip_pool *p, uint32_t k, bool dense) {
+void capture_rounds(rip_pool *p, uint32_t k, bool dense, bool cols) {
   SelectScratch &s = p->sel;
   if (s.graph) {
     (void)hipGraphExecDestroy(s.graph);
@@ -576,6 +710,11 @@ void capture_rounds(rip_pool *p, uint32_t k, bool dense) {
       hipLaunchKernelGGL(dec_small_kernel, dim3(256), dim3(kBT), 0, st, s.d_args);
       hipLaunchKernelGGL(dense_dec_kernel, dim3(dec_grid), dim3(kDenseBT), 0, st, s.d_args,
                          ranges);
+      if (cols) {
+        hipLaunchKernelGGL(hot_dec_kernel, dim3(grid_for(p->g->hot_n, kBT, 8192)), dim3(kBT), 0,
+                           st, s.d_args);
+        hipLaunchKernelGGL(clear_gkill_kernel, dim3(1), dim3(256), 0, st, s.d_args);
+      }
     } else {
       hipLaunchKernelGGL(kill_kernel, dim3(256), dim3(kBT), 0, st, s.d_args, i, argmax_grid);
     }
@@ -592,6 +731,7 @@ void capture_rounds(rip_pool *p, uint32_t k, bool dense) {
   s.graph = exec;
   s.graph_k = k;
   s.graph_dense = dense;
+  s.graph_cols = cols;
 }
 
 }  // namespace
@@ -615,11 +755,15 @@ uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
                      s.d_covered, num_sets);
   // the seed rounds run as one graph; this call's buffers travel in d_args
   // (`args` stays alive until the stream is synchronised below)
+  const bool cols = p->groups > 0;
   const Args args{s.d_counts, s.d_inv_start, s.d_inv, p->d_off, p->d_len, p->d_data,
-                  s.d_covered, s.d_keys, p->d_ctl, s.d_kill, s.d_partial, num_sets, n};
+                  s.d_covered, s.d_keys, p->d_ctl, s.d_kill, s.d_partial, num_sets, n,
+                  p->d_gref, p->d_colptr, p->g->d_hot_index, p->g->d_hot_vertex, s.d_gkill,
+                  s.d_kgroups, cols ? p->groups : 0u, p->g->hot_n};
   RIP_HIP(hipMemcpyAsync(s.d_args, &args, sizeof(Args), hipMemcpyHostToDevice, st));
   if (k) {
-    if (!s.graph || s.graph_k != k || s.graph_dense != dense) capture_rounds(p, k, dense);
+    if (!s.graph || s.graph_k != k || s.graph_dense != dense || s.graph_cols != cols)
+      capture_rounds(p, k, dense, cols);
     RIP_HIP(hipGraphLaunch(s.graph, st));
   } else {
     hipLaunchKernelGGL(clear_keys_kernel, dim3(1), dim3(256), 0, st, s.d_args, 0u);
@@ -650,7 +794,8 @@ uint64_t coverage_run(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
     RIP_REQUIRE(seeds[i] < n, -1, "coverage: seed vertex out of range");
   hipStream_t st = p->stream;
   RIP_HIP(hipSetDevice(p->g->device));
-  ensure_select_scratch(p, 0, 0, 0, std::max<uint32_t>(num_seeds, 1));
+  // column pools also mark per-set hits (in the covered flags)
+  ensure_select_scratch(p, p->groups ? num_sets : 0, 0, 0, std::max<uint32_t>(num_seeds, 1));
   SelectScratch &s = p->sel;
   // the seeds travel through the key buffer (k >= num_seeds words of 8 bytes)
   if (num_seeds)
@@ -663,10 +808,22 @@ uint64_t coverage_run(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
                        reinterpret_cast<const uint32_t *>(s.d_keys), num_seeds, s.d_counts);
   unsigned long long *out = p->d_ctl + kCtlWords - 1;
   hipLaunchKernelGGL(zero_word_kernel, dim3(1), dim3(1), 0, st, out);
-  if (num_sets && num_seeds)
-    hipLaunchKernelGGL(coverage_kernel, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
+  if (num_sets && num_seeds && p->groups) {
+    // column sets: hits through their lists or their hot members, counted
+    // once per set
+    hipLaunchKernelGGL(coverage_kernel<false>, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
+                       dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets, s.d_counts,
+                       p->d_ctl + kCtlScratch2, s.d_covered);
+    hipLaunchKernelGGL(coverage_hot_kernel, dim3(grid_for(num_sets, kBT, 4096)), dim3(kBT), 0, st,
+                       p->d_gref, p->d_colptr, p->g->d_hot_index, num_sets,
+                       reinterpret_cast<const uint32_t *>(s.d_keys), num_seeds, s.d_covered);
+    hipLaunchKernelGGL(count_u8_kernel, dim3(grid_for(num_sets, kBT, 1024)), dim3(kBT), 0, st,
+                       s.d_covered, num_sets, out);
+  } else if (num_sets && num_seeds) {
+    hipLaunchKernelGGL(coverage_kernel<false>, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
                        dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets, s.d_counts,
                        out);
+  }
   RIP_HIP(hipGetLastError());
   unsigned long long cov = 0;
   RIP_HIP(hipMemcpyAsync(&cov, out, sizeof(cov), hipMemcpyDeviceToHost, st));

@@ -308,6 +308,7 @@ void worker_loop(rip_server *s) {
         e->pool.seed = e->seed;
         e->pool.num_sets = 0;
         e->pool.used_data = 0;
+        e->pool.groups = 0;
         e->pool.num_elements = 0;
       }
       e->reseed = false;
