@@ -129,7 +129,7 @@ __global__ void out_weight_kernel(const uint32_t *ids, const uint32_t *val, uint
 // 64, at least min(n, 64)), those most likely to be reached from others,
 // ranked by the sum of their out-edge thresholds.  On R-MAT scale 24 with
 // U[0,0.02] weights they hold 96% of the members of the giant sets.
-void graph_ensure_hot(rip_graph *g) {
+void graph_ensure_hot(rip_graph *g, hipStream_t st) {
   if (g->d_hot_index) return;
   RIP_REQUIRE(g->model == RIP_IC, -1, "graph: the column store is for IC graphs");
   const uint32_t n = g->n;
@@ -137,12 +137,14 @@ void graph_ensure_hot(rip_graph *g) {
   std::vector<unsigned long long> key(n);
   unsigned long long *d_key = dev_alloc<unsigned long long>(n);
   try {
-    RIP_HIP(hipMemset(d_key, 0, sizeof(unsigned long long) * n));
+    RIP_HIP(hipMemsetAsync(d_key, 0, sizeof(unsigned long long) * n, st));
     if (g->m)
-      hipLaunchKernelGGL(out_weight_kernel, dim3(2048), dim3(256), 0, 0, g->d_ids, g->d_val,
+      hipLaunchKernelGGL(out_weight_kernel, dim3(2048), dim3(256), 0, st, g->d_ids, g->d_val,
                          g->m, d_key);
     RIP_HIP(hipGetLastError());
-    RIP_HIP(hipMemcpy(key.data(), d_key, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
+    RIP_HIP(hipMemcpyAsync(key.data(), d_key, sizeof(unsigned long long) * n,
+                           hipMemcpyDeviceToHost, st));
+    RIP_HIP(hipStreamSynchronize(st));
   } catch (...) {
     dev_free(d_key);
     throw;
@@ -159,8 +161,11 @@ void graph_ensure_hot(rip_graph *g) {
   uint32_t *d_vertex = nullptr;
   try {
     d_vertex = dev_alloc<uint32_t>(hot);
-    RIP_HIP(hipMemcpy(d_index, index.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-    RIP_HIP(hipMemcpy(d_vertex, vertex.data(), sizeof(uint32_t) * hot, hipMemcpyHostToDevice));
+    RIP_HIP(hipMemcpyAsync(d_index, index.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice,
+                           st));
+    RIP_HIP(hipMemcpyAsync(d_vertex, vertex.data(), sizeof(uint32_t) * hot,
+                           hipMemcpyHostToDevice, st));
+    RIP_HIP(hipStreamSynchronize(st));
   } catch (...) {
     dev_free(d_index);
     dev_free(d_vertex);

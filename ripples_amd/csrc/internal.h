@@ -246,7 +246,7 @@ void pool_reserve_sets(rip_pool *p, uint64_t cap_sets);
 void pool_reserve_data(rip_pool *p, uint64_t cap_data);
 // Column store: the graph's hot vertices (built on first use), room for
 // `groups` column groups, and d_gref set to kNoGroup for sets [first, end).
-void graph_ensure_hot(rip_graph *g);
+void graph_ensure_hot(rip_graph *g, hipStream_t st);
 void pool_reserve_groups(rip_pool *p, uint32_t groups);
 void pool_clear_gref(rip_pool *p, uint64_t first, uint64_t end);
 // Resident blocks per CU of a kernel at a block size (at least 1), and the

@@ -334,7 +334,7 @@ void pool_clear_gref(rip_pool *p, uint64_t first, uint64_t end) {
 // reallocated.  The first call also sets up d_gref (every set so far a plain
 // list).
 void pool_reserve_groups(rip_pool *p, uint32_t need) {
-  graph_ensure_hot(p->g);
+  graph_ensure_hot(p->g, p->stream);
   if (!p->d_gref) {
     p->d_gref = dev_alloc<uint32_t>(std::max<uint64_t>(p->cap_sets, 1));
     pool_clear_gref(p, 0, std::max<uint64_t>(p->cap_sets, 1));
@@ -344,8 +344,9 @@ void pool_reserve_groups(rip_pool *p, uint32_t need) {
   const uint64_t hot = p->g->hot_n;
   std::vector<unsigned long long *> ptrs(cap);
   if (p->cap_groups)
-    RIP_HIP(hipMemcpy(ptrs.data(), p->d_colptr, sizeof(void *) * p->cap_groups,
-                      hipMemcpyDeviceToHost));
+    RIP_HIP(hipMemcpyAsync(ptrs.data(), p->d_colptr, sizeof(void *) * p->cap_groups,
+                           hipMemcpyDeviceToHost, p->stream));
+  RIP_HIP(hipStreamSynchronize(p->stream));
   for (uint32_t b = p->cap_groups; b < cap; b += kColBlock) {
     unsigned long long *blk = dev_alloc<unsigned long long>(kColBlock * hot);
     p->col_blocks.push_back(blk);
@@ -355,10 +356,12 @@ void pool_reserve_groups(rip_pool *p, uint32_t need) {
   uint32_t *gsample = nullptr;
   try {
     gsample = dev_alloc<uint32_t>(64ull * cap);
-    RIP_HIP(hipMemcpy(colptr, ptrs.data(), sizeof(void *) * cap, hipMemcpyHostToDevice));
+    RIP_HIP(hipMemcpyAsync(colptr, ptrs.data(), sizeof(void *) * cap, hipMemcpyHostToDevice,
+                           p->stream));
     if (p->cap_groups)
-      RIP_HIP(hipMemcpy(gsample, p->d_gsample, sizeof(uint32_t) * 64 * p->cap_groups,
-                        hipMemcpyDeviceToDevice));
+      RIP_HIP(hipMemcpyAsync(gsample, p->d_gsample, sizeof(uint32_t) * 64 * p->cap_groups,
+                             hipMemcpyDeviceToDevice, p->stream));
+    RIP_HIP(hipStreamSynchronize(p->stream));
   } catch (...) {
     dev_free(colptr);
     dev_free(gsample);
@@ -411,6 +414,13 @@ void launch_sum_u32(const uint32_t *v, uint64_t n, unsigned long long *out, hipS
   hipLaunchKernelGGL(sum_u32_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, st, v, n,
                      out);
   RIP_HIP(hipGetLastError());
+}
+
+// A copy on the pool's stream, finished before returning (no legacy-stream
+// operation: other pools' streams may be capturing a graph meanwhile).
+static void copy_sync(rip_pool *p, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  RIP_HIP(hipMemcpyAsync(dst, src, bytes, kind, p->stream));
+  RIP_HIP(hipStreamSynchronize(p->stream));
 }
 
 void pool_reserve_data(rip_pool *p, uint64_t cap) {
@@ -532,12 +542,12 @@ void pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
     pool_reserve_data(p, std::max<uint64_t>(p->used_data + total,
                                             p->cap_data + p->cap_data / 2));
   if (total)
-    RIP_HIP(hipMemcpy(p->d_data + p->used_data, vertices, sizeof(uint32_t) * total,
-                      hipMemcpyHostToDevice));
-  RIP_HIP(hipMemcpy(p->d_off + p->num_sets, off.data(), sizeof(unsigned long long) * count,
-                    hipMemcpyHostToDevice));
-  RIP_HIP(hipMemcpy(p->d_len + p->num_sets, len.data(), sizeof(uint32_t) * count,
-                    hipMemcpyHostToDevice));
+    copy_sync(p, p->d_data + p->used_data, vertices, sizeof(uint32_t) * total,
+                      hipMemcpyHostToDevice);
+  copy_sync(p, p->d_off + p->num_sets, off.data(), sizeof(unsigned long long) * count,
+                    hipMemcpyHostToDevice);
+  copy_sync(p, p->d_len + p->num_sets, len.data(), sizeof(uint32_t) * count,
+                    hipMemcpyHostToDevice);
   pool_clear_gref(p, p->num_sets, p->num_sets + count);
   p->num_sets += count;
   p->used_data += total;
@@ -558,8 +568,8 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
   std::vector<uint32_t> gent;
   if (p->groups && p->d_gref && count) {
     std::vector<uint32_t> gref(count);
-    RIP_HIP(hipMemcpy(gref.data(), p->d_gref + first, sizeof(uint32_t) * count,
-                      hipMemcpyDeviceToHost));
+    copy_sync(p, gref.data(), p->d_gref + first, sizeof(uint32_t) * count,
+                      hipMemcpyDeviceToHost);
     for (uint64_t i = 0; i < count; ++i)
       if (gref[i] != kNoGroup) {
         gidx.push_back(i);
@@ -572,7 +582,7 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
   try {
     exclusive_scan_u32(p->d_len + first, pre, count, p->stream);
     std::vector<uint64_t> coff(count + 1);
-    RIP_HIP(hipMemcpy(coff.data(), pre, sizeof(uint64_t) * (count + 1), hipMemcpyDeviceToHost));
+    copy_sync(p, coff.data(), pre, sizeof(uint64_t) * (count + 1), hipMemcpyDeviceToHost);
     const uint64_t cold_total = coff[count];
     // hot member counts of the column-held sets
     const uint64_t ng = gidx.size();
@@ -580,7 +590,7 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
     if (ng) {
       d_ent = dev_alloc<uint32_t>(ng);
       hsz = dev_alloc<unsigned long long>(ng + 1);
-      RIP_HIP(hipMemcpy(d_ent, gent.data(), sizeof(uint32_t) * ng, hipMemcpyHostToDevice));
+      copy_sync(p, d_ent, gent.data(), sizeof(uint32_t) * ng, hipMemcpyHostToDevice);
       hipLaunchKernelGGL(hot_size_kernel, dim3(static_cast<uint32_t>(ng)), dim3(kHotBT), 0,
                          p->stream, p->d_colptr, d_ent, p->g->hot_n, hsz);
       RIP_HIP(hipGetLastError());
@@ -614,8 +624,8 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
       std::vector<uint32_t> hot_h;
       if (ng) {
         hot = dev_alloc<uint32_t>(hoff[ng]);
-        RIP_HIP(hipMemcpy(hsz, hoff.data(), sizeof(unsigned long long) * (ng + 1),
-                          hipMemcpyHostToDevice));
+        copy_sync(p, hsz, hoff.data(), sizeof(unsigned long long) * (ng + 1),
+                          hipMemcpyHostToDevice);
         hipLaunchKernelGGL(hot_extract_kernel, dim3(static_cast<uint32_t>(ng)), dim3(kHotBT), 0,
                            p->stream, p->d_colptr, d_ent, p->g->hot_n, p->g->d_hot_vertex, hsz,
                            hot);
@@ -625,15 +635,15 @@ void pool_export(rip_pool *p, uint64_t first, uint64_t count, uint64_t *offsets,
       RIP_HIP(hipStreamSynchronize(p->stream));
       if (!ng) {
         if (total)
-          RIP_HIP(hipMemcpy(vertices, out, sizeof(uint32_t) * total, hipMemcpyDeviceToHost));
+          copy_sync(p, vertices, out, sizeof(uint32_t) * total, hipMemcpyDeviceToHost);
       } else {
         std::vector<uint32_t> cold_h(cold_total);
         if (cold_total)
-          RIP_HIP(hipMemcpy(cold_h.data(), out, sizeof(uint32_t) * cold_total,
-                            hipMemcpyDeviceToHost));
+          copy_sync(p, cold_h.data(), out, sizeof(uint32_t) * cold_total,
+                            hipMemcpyDeviceToHost);
         if (!hot_h.empty())
-          RIP_HIP(hipMemcpy(hot_h.data(), hot, sizeof(uint32_t) * hot_h.size(),
-                            hipMemcpyDeviceToHost));
+          copy_sync(p, hot_h.data(), hot, sizeof(uint32_t) * hot_h.size(),
+                            hipMemcpyDeviceToHost);
         for (uint64_t i = 0, j = 0; i < count; ++i) {
           const uint32_t *c0 = cold_h.data() + coff[i], *c1 = cold_h.data() + coff[i + 1];
           if (j < ng && gidx[j] == i) {
