@@ -190,6 +190,17 @@ int rip_pool_count(rip_pool *p, uint64_t num_sets, uint32_t *counts);
  * num_seeds) -- FindCoverage (include/ripples/opimc.h:100-118). */
 int rip_pool_coverage(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
                       uint32_t num_seeds, uint64_t *covered);
+/* rip_pool_coverage without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * call allocates and frees nothing: it works in a fixed 8.3 MB device array
+ * of the library, one per GPU, held until the process ends.  Limits (larger
+ * inputs return RIP_EINVAL): graphs of at most 2^25 vertices, at most 65536
+ * seeds, and for pools with column sets (rip_pool_columns) at most 2^25 sets.
+ * Every call on one GPU shares that array, so calls that overlap on a GPU
+ * race; calls on one pool are ordered by its stream. */
+int rip_pool_coverage_noalloc(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
+                              uint32_t num_seeds, uint64_t *covered);
 /* Device time of the last rip_pool_sample call's main sampling kernel (IC:
  * the lane kernel, or the wave kernel when the sampler kind has no lane
  * stage; LT: the walk kernel), measured with HIP events on the pool's

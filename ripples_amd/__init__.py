@@ -160,6 +160,7 @@ def lib():
                                         C.POINTER(C.POINTER(C.c_uint32))]
     L.rip_host_graph_destroy.argtypes = [vp]
     L.rip_pool_coverage.argtypes = [vp, C.c_uint64, _u32p, C.c_uint32, _u64p]
+    L.rip_pool_coverage_noalloc.argtypes = [vp, C.c_uint64, _u32p, C.c_uint32, _u64p]
     L.rip_opimc.argtypes = [vp, C.c_uint32, C.c_double, C.c_double, C.c_uint64,
                             C.POINTER(Tune), _u32p, C.POINTER(OpimcRecord)]
     L.rip_opimc_r2_seed.argtypes = [C.c_uint64]
@@ -409,6 +410,16 @@ class Pool:
         out = np.zeros(1, np.uint64)
         _check(lib().rip_pool_coverage(self._h, ns, seeds if len(seeds) else
                                        np.zeros(1, np.uint32), len(seeds), out))
+        return int(out[0])
+
+    def coverage_noalloc(self, seeds, num_sets=None):
+        """coverage() through rip_pool_coverage_noalloc: the library's fixed
+        device array instead of allocated scratch."""
+        seeds = np.ascontiguousarray(seeds, np.uint32)
+        ns = self.size()[0] if num_sets is None else num_sets
+        out = np.zeros(1, np.uint64)
+        _check(lib().rip_pool_coverage_noalloc(self._h, ns, seeds if len(seeds) else
+                                               np.zeros(1, np.uint32), len(seeds), out))
         return int(out[0])
 
     def last_kernel_ms(self):

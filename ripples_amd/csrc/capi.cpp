@@ -198,6 +198,15 @@ int rip_pool_coverage(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
   });
 }
 
+int rip_pool_coverage_noalloc(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
+                              uint32_t num_seeds, uint64_t *covered) {
+  return guarded([&] {
+    RIP_REQUIRE(p && covered, RIP_EINVAL, "pool or covered is NULL");
+    RIP_REQUIRE(num_seeds == 0 || seeds, RIP_EINVAL, "seeds is NULL");
+    *covered = rip::coverage_run_noalloc(p, num_sets, seeds, num_seeds);
+  });
+}
+
 int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches) {
   return guarded([&] {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");

@@ -332,6 +332,10 @@ uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k,
 void count_run(rip_pool *p, uint64_t num_sets, uint32_t *counts_host);
 uint64_t coverage_run(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
                       uint32_t num_seeds);
+// coverage_run in a fixed device array instead of the pool's selector
+// buffers: no device memory is allocated or freed
+uint64_t coverage_run_noalloc(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
+                              uint32_t num_seeds);
 
 // scan.hip: exclusive scan of uint32 values into uint64 prefixes (n + 1 out)
 void exclusive_scan_u32(const uint32_t *d_in, unsigned long long *d_out,

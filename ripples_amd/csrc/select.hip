@@ -515,12 +515,15 @@ __global__ void __launch_bounds__(kBT) count_u8_kernel(const uint8_t *v, uint64_
 
 // Sets [0, num_sets) that contain a flagged vertex (FindCoverage,
 // include/ripples/opimc.h:100-118): each wave walks 64 sets, marks hits in
-// LDS and adds their popcount to *out once per wave.
+// LDS and adds their popcount to *out once per wave.  Per-set hits also go
+// to hit_out (one byte per set) or hit_bits (word w = sets 64w .. 64w + 63:
+// a wave's 64 sets are one word).
 template <bool kBits>
 __global__ void __launch_bounds__(kBT)
 coverage_kernel(const unsigned long long *set_off, const uint32_t *set_len,
                 const uint32_t *data, uint64_t num_sets, const uint32_t *flags,
-                unsigned long long *out, uint8_t *hit_out = nullptr) {
+                unsigned long long *out, uint8_t *hit_out = nullptr,
+                unsigned long long *hit_bits = nullptr) {
   __shared__ uint32_t s_pre[kBT / 64][65];
   __shared__ unsigned long long s_off[kBT / 64][64];
   __shared__ uint32_t s_hit[kBT / 64][64];
@@ -549,11 +552,70 @@ coverage_kernel(const unsigned long long *set_off, const uint32_t *set_len,
       if (kBits ? (flags[v >> 5] >> (v & 31)) & 1u : flags[v]) s_hit[wid][lo] = 1;
     }
     __builtin_amdgcn_wave_barrier();
-    acc += __popcll(__ballot(s_hit[wid][lane] != 0));
+    const uint64_t hm = __ballot(s_hit[wid][lane] != 0);
+    acc += __popcll(hm);
     if (hit_out && s < num_sets) hit_out[s] = s_hit[wid][lane] != 0;
+    if (hit_bits && lane == 0) hit_bits[w] = hm;
     __builtin_amdgcn_wave_barrier();
   }
   if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
+// Column sets not already hit through their lists, with per-set hit bits
+// (word w = sets 64w .. 64w + 63, one wave per word): a hit when a seed is a
+// hot member.
+__global__ void __launch_bounds__(kBT) coverage_hot_bits_kernel(
+    const uint32_t *gref, unsigned long long *const *colptr, const uint32_t *hot_index,
+    uint64_t num_sets, const uint32_t *seeds, uint32_t num_seeds,
+    unsigned long long *hit_bits) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBT / 64);
+  for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * (kBT / 64) + (threadIdx.x >> 6);
+       w * 64 < num_sets; w += nwaves) {
+    const uint64_t s = w * 64 + lane;
+    const unsigned long long word = hit_bits[w];
+    bool hit = false;
+    const uint32_t e = s < num_sets ? gref[s] : kNoGroup;
+    if (e != kNoGroup && !((word >> lane) & 1ull)) {
+      const unsigned long long *col = colptr[e >> 6];
+      for (uint32_t j = 0; j < num_seeds && !hit; ++j) {
+        const uint32_t h = hot_index[seeds[j]];
+        hit = h != kNotHot && ((col[h] >> (e & 63u)) & 1ull);
+      }
+    }
+    const uint64_t m = __ballot(hit);
+    if (lane == 0 && m) hit_bits[w] = word | m;
+  }
+}
+
+// *out += the set bits of words[0, n).
+__global__ void __launch_bounds__(kBT) popcount_u64_kernel(const unsigned long long *words,
+                                                           uint64_t n, unsigned long long *out) {
+  __shared__ unsigned long long s_w[kBT / 64];
+  unsigned long long acc = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBT + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kBT)
+    acc += __popcll(words[i]);
+  for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    acc = 0;
+    for (uint32_t w = 0; w < kBT / 64; ++w) acc += s_w[w];
+    if (acc) atomicAdd(out, acc);
+  }
+}
+
+// Sets the bit of every seed in a vertex bitmap.
+__global__ void flag_seed_bits_kernel(const uint32_t *seeds, uint32_t k, uint32_t *bits) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x)
+    atomicOr(&bits[seeds[i] >> 5], 1u << (seeds[i] & 31u));
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = 0;
 }
 
 __global__ void flag_seeds_kernel(const uint32_t *seeds, uint32_t k, uint32_t *flags) {
@@ -823,6 +885,103 @@ uint64_t coverage_run(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
     hipLaunchKernelGGL(coverage_kernel<false>, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
                        dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets, s.d_counts,
                        out);
+  }
+  RIP_HIP(hipGetLastError());
+  unsigned long long cov = 0;
+  RIP_HIP(hipMemcpyAsync(&cov, out, sizeof(cov), hipMemcpyDeviceToHost, st));
+  RIP_HIP(hipStreamSynchronize(st));
+  return cov;
+}
+
+// ------------------------------------------------- coverage, no allocation
+// rip_pool_coverage_noalloc works in this fixed device array instead of the
+// pool's selector buffers: a vertex bitmap (kNaMaxNodes bits), the seeds
+// (kNaMaxSeeds), per-set hit bits of column pools (kNaMaxColumnSets bits) and
+// the result counter, 8.3 MB in all.  Each device holds its own copy, found
+// with hipGetSymbolAddress on the device's first call; every call on one
+// device shares it, so calls that overlap on a device race.
+constexpr uint64_t kNaMaxNodes = 1ull << 25;
+constexpr uint64_t kNaMaxSeeds = 1ull << 16;
+constexpr uint64_t kNaMaxColumnSets = 1ull << 25;
+constexpr uint64_t kNaBitmapWords = kNaMaxNodes / 64;         // u64 words
+constexpr uint64_t kNaSeedWords = kNaMaxSeeds / 2;
+constexpr uint64_t kNaHitWords = kNaMaxColumnSets / 64;
+constexpr uint64_t kNaSeedAt = kNaBitmapWords;
+constexpr uint64_t kNaHitAt = kNaSeedAt + kNaSeedWords;
+constexpr uint64_t kNaCounterAt = kNaHitAt + kNaHitWords;
+constexpr uint64_t kNaWords = kNaCounterAt + 32;
+
+static __device__ __align__(256) unsigned long long scratch[kNaWords];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+std::mutex na_mu;
+unsigned long long *na_ptr[kNaMaxDevices] = {};
+
+// The device address of `scratch` on the current device.
+unsigned long long *noalloc_scratch(int device) {
+  RIP_REQUIRE(device >= 0 && device < kNaMaxDevices, RIP_EINVAL, "coverage_noalloc: device id");
+  std::lock_guard<std::mutex> hold(na_mu);
+  if (!na_ptr[device]) {
+    void *ptr = nullptr;
+    RIP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    RIP_REQUIRE(ptr, RIP_EDEVICE, "coverage_noalloc: no device address for the scratch array");
+    na_ptr[device] = static_cast<unsigned long long *>(ptr);
+  }
+  return na_ptr[device];
+}
+}  // namespace
+
+uint64_t coverage_run_noalloc(rip_pool *p, uint64_t num_sets, const uint32_t *seeds,
+                              uint32_t num_seeds) {
+  const uint32_t n = p->g->n;
+  RIP_REQUIRE(num_sets <= p->num_sets, RIP_EINVAL, "coverage_noalloc: num_sets beyond the pool");
+  RIP_REQUIRE(n <= kNaMaxNodes, RIP_EINVAL,
+              "coverage_noalloc: graphs of at most 2^25 vertices (the fixed scratch)");
+  RIP_REQUIRE(num_seeds <= kNaMaxSeeds, RIP_EINVAL,
+              "coverage_noalloc: at most 65536 seeds (the fixed scratch)");
+  RIP_REQUIRE(p->groups == 0 || num_sets <= kNaMaxColumnSets, RIP_EINVAL,
+              "coverage_noalloc: pools with column sets: at most 2^25 sets (the fixed scratch)");
+  for (uint32_t i = 0; i < num_seeds; ++i)
+    RIP_REQUIRE(seeds[i] < n, RIP_EINVAL, "coverage_noalloc: seed vertex out of range");
+  RIP_HIP(hipSetDevice(p->g->device));
+  unsigned long long *base = noalloc_scratch(p->g->device);
+  uint32_t *bits = reinterpret_cast<uint32_t *>(base);
+  uint32_t *d_seeds = reinterpret_cast<uint32_t *>(base + kNaSeedAt);
+  unsigned long long *hit = base + kNaHitAt;
+  unsigned long long *out = base + kNaCounterAt;
+  hipStream_t st = p->stream;
+  // the previous call's contents stay in the array: clear what this call
+  // reads (kernels, not memsets, so the sequence can sit in a captured graph)
+  const uint64_t nwords = (static_cast<uint64_t>(n) + 63) / 64;
+  const uint64_t hwords = p->groups ? (num_sets + 63) / 64 : 0;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(nwords, kBT, 1024)), dim3(kBT), 0, st, base,
+                     nwords);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, st, out, 2ull);  // + a dummy
+  if (hwords)
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(hwords, kBT, 1024)), dim3(kBT), 0, st, hit,
+                       hwords);
+  if (num_seeds) {
+    RIP_HIP(hipMemcpyAsync(d_seeds, seeds, sizeof(uint32_t) * num_seeds, hipMemcpyHostToDevice,
+                           st));
+    hipLaunchKernelGGL(flag_seed_bits_kernel, dim3(grid_for(num_seeds, kBT, 64)), dim3(kBT), 0,
+                       st, d_seeds, num_seeds, bits);
+  }
+  if (num_sets && num_seeds) {
+    const dim3 grid(grid_for(num_sets, 64 * (kBT / 64), 8192));
+    if (p->groups) {
+      // column sets: hits through their lists, then through their hot
+      // members; each set counted once from the hit bits
+      hipLaunchKernelGGL(coverage_kernel<true>, grid, dim3(kBT), 0, st, p->d_off, p->d_len,
+                         p->d_data, num_sets, bits, base + kNaCounterAt + 1, nullptr, hit);
+      hipLaunchKernelGGL(coverage_hot_bits_kernel, grid, dim3(kBT), 0, st, p->d_gref,
+                         p->d_colptr, p->g->d_hot_index, num_sets, d_seeds, num_seeds, hit);
+      hipLaunchKernelGGL(popcount_u64_kernel, dim3(grid_for(hwords, kBT, 1024)), dim3(kBT), 0,
+                         st, hit, hwords, out);
+    } else {
+      hipLaunchKernelGGL(coverage_kernel<true>, grid, dim3(kBT), 0, st, p->d_off, p->d_len,
+                         p->d_data, num_sets, bits, out, nullptr, nullptr);
+    }
   }
   RIP_HIP(hipGetLastError());
   unsigned long long cov = 0;
