@@ -57,17 +57,21 @@ typedef struct {
   uint32_t out_chunk;    /* LT and IC wave sampler: pool elements a wave
                             reserves at a time (the pool's data then holds
                             gaps between chunks) */
-  uint32_t reserved;     /* must be zero */
+  uint32_t frontier_cap; /* IC grid sampler (AUTO): frontier entries a BFS
+                            level may list (0 = automatic); vertices past it
+                            are found again by a scan of the pending masks */
 } rip_tune;
 
 /* IC sampler kinds (all give the same sets):
  *   AUTO    one reverse BFS per lane (64 samples per wave); samples passing
- *           queue_cap vertices go to one BFS per wave, those passing 4096 to
- *           64-sample colour groups
+ *           queue_cap vertices go to the grid sampler: 64-sample colour groups,
+ *           many groups per batch, one BFS level of all of them per kernel
  *   WAVE    one reverse BFS per wave for every sample (the plain form), sets
  *           passing 4096 vertices finished by a bitmap-visited BFS per wave
  *   COLOUR  64-sample colour groups for every sample
- *   LANE    as AUTO, with the bitmap-visited BFS instead of colour groups */
+ *   LANE    one reverse BFS per lane, then one per wave for samples passing
+ *           queue_cap vertices and the bitmap-visited BFS per wave for those
+ *           passing 4096 */
 #define RIP_SAMPLER_AUTO 0
 #define RIP_SAMPLER_WAVE 1
 #define RIP_SAMPLER_COLOUR 2
@@ -206,6 +210,13 @@ int rip_pool_coverage_noalloc(rip_pool *p, uint64_t num_sets, const uint32_t *se
  * stage; LT: the walk kernel), measured with HIP events on the pool's
  * stream, and the number of kernel launches of the call. */
 int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
+/* The grid sampler's share of the last rip_pool_sample call (IC, AUTO: the
+ * samples that outgrew a lane): the summed device time of its BFS level
+ * kernels (HIP events on the pool's stream), how many levels ran (0 when no
+ * sample reached it), and the work counters of its kernels (RIP_STAT_*
+ * order, as rip_pool_last_stats) into stats[0 .. count). */
+int rip_pool_last_grid(const rip_pool *p, double *ms, uint32_t *levels, uint64_t *stats,
+                       uint32_t count);
 /* Device time from the start of the last rip_pool_sample call's main kernel
  * to the end of its last kernel (fallback kernels, sorts and the host round
  * trips between them included), measured with HIP events. */

@@ -41,7 +41,7 @@ class Tune(C.Structure):
     _fields_ = [("colors", C.c_uint32), ("block", C.c_uint32),
                 ("slots", C.c_uint32), ("queue_cap", C.c_uint32),
                 ("heavy_degree", C.c_uint32), ("sampler", C.c_uint32),
-                ("out_chunk", C.c_uint32), ("reserved", C.c_uint32)]
+                ("out_chunk", C.c_uint32), ("frontier_cap", C.c_uint32)]
 
 
 class LoadOptions(C.Structure):
@@ -136,6 +136,8 @@ def lib():
     L.rip_pool_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double),
                                           C.POINTER(C.c_uint32)]
     L.rip_pool_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
+    L.rip_pool_last_grid.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), _u64p,
+                                     C.c_uint32]
     L.rip_pool_columns.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint64)]
     L.rip_pool_last_kernel_stats.argtypes = [vp, _u64p, C.c_uint32]
@@ -426,6 +428,15 @@ class Pool:
         ms, launches = C.c_double(0), C.c_uint32(0)
         _check(lib().rip_pool_last_kernel_ms(self._h, C.byref(ms), C.byref(launches)))
         return ms.value, launches.value
+
+    def last_grid(self):
+        """(summed ms of the grid sampler's level kernels, levels, its work
+        counters) for the last sampling call."""
+        ms, lv = C.c_double(0), C.c_uint32(0)
+        st = np.zeros(5, np.uint64)
+        _check(lib().rip_pool_last_grid(self._h, C.byref(ms), C.byref(lv), st, 5))
+        keys = ("algo_bytes", "edges", "coins", "expansions", "levels")
+        return ms.value, lv.value, {k: int(v) for k, v in zip(keys, st)}
 
     def last_call_ms(self):
         """Device time of the whole last sampling call (every kernel)."""

@@ -215,6 +215,18 @@ int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches) {
   });
 }
 
+int rip_pool_last_grid(const rip_pool *p, double *ms, uint32_t *levels, uint64_t *stats,
+                       uint32_t count) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    RIP_REQUIRE(count == 0 || stats, RIP_EINVAL, "stats is NULL");
+    if (ms) *ms = p->grid_ms;
+    if (levels) *levels = p->grid_levels;
+    for (uint32_t i = 0; i < count; ++i)
+      stats[i] = i < RIP_STAT_COUNT && p->grid_levels ? p->last_stats[i] - p->main_stats[i] : 0;
+  });
+}
+
 int rip_pool_last_call_ms(const rip_pool *p, double *ms) {
   return guarded([&] {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");

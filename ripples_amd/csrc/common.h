@@ -83,8 +83,8 @@ inline uint32_t ic_threshold(float w) {
 // Device vertex record, one aligned 64-byte line: the jump to the vertex's
 // first stream element (IC: P^(1 + 2 begin), the row's first draw; LT:
 // P^(1 + begin), the threshold) and its CSR row; for IC the row's gap
-// parameters (ic_gap.h): largest threshold T, q, log2 of the block size and
-// the full-block skip threshold.
+// parameters (ic_gap.h): largest threshold T, q and its reciprocal, log2 of
+// the block size and the skip thresholds of a full and of the last block.
 struct alignas(64) VertexRec {
   uint64_t ja, jb;       // jump to the first element, as an affine map
   uint64_t q;            // IC: gap divisor of the row (0 when T = 2^24)
@@ -93,7 +93,9 @@ struct alignas(64) VertexRec {
                          // LT: offset of the row's breakpoint index
   uint32_t ufull;        // IC: a full block has no candidate iff hi32(x) + 1 <= ufull
   uint32_t flags;        // kRowUniform, kRowScalar; IC: beta << kBetaShift
-  uint32_t pad[5];
+  uint32_t ulast;        // IC: the same for the row's last block when it is partial
+  uint64_t rq;           // IC: floor(2^64 / q), the reciprocal of the gap divisor
+  uint32_t pad[2];
 };
 static_assert(sizeof(VertexRec) == 64, "one line per vertex record");
 

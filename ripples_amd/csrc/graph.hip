@@ -43,14 +43,21 @@ __global__ void build_vrec_kernel(const unsigned long long *index, const uint32_
   r.end = static_cast<uint32_t>(e);
   r.thr = thr[v];
   r.ufull = 0;
+  r.ulast = 0;
+  r.rq = 0;
   r.flags = flags[v];
   if (model == RIP_IC && r.thr != 0) {
     const uint32_t beta = ic_block_log2(r.thr);
     r.q = ic_gap_q(r.thr);
+    r.rq = ic_gap_recip(r.q);
     r.ufull = ic_full_skip(r.q, beta);
+    // the last block holds d - (nblk - 1) B edges
+    const uint32_t d = static_cast<uint32_t>(e - b);
+    const uint32_t last = d == 0 ? 0u : d - (((d - 1) >> beta) << beta);
+    r.ulast = ic_skip_len(r.q, last);
     r.flags |= beta << kBetaShift;
   }
-  for (int i = 0; i < 5; ++i) r.pad[i] = 0;
+  for (int i = 0; i < 2; ++i) r.pad[i] = 0;
   vrec[v] = r;
 }
 
@@ -353,6 +360,7 @@ void graph_destroy(rip_graph *g) {
   dev_free(g->wave.d_lepoch);
   dev_free(g->wave.d_over);
   dev_free(g->wave.d_lsort);
+  grid_scratch_free(g->grid);
   dev_free(g->lt.d_long_ids);
   dev_free(g->lt.d_spill);
   dev_free(g->d_bp);

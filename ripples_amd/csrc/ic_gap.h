@@ -102,14 +102,16 @@ __host__ __device__ inline bool ic_accept(uint64_t a, uint32_t T, uint32_t t) {
   return (a >> 32) * T < (static_cast<uint64_t>(t) << 32);
 }
 
-// The largest U = hi32(x) + 1 whose gap is at least 2^beta (a full block
-// without candidates), 0 if there is none.
-__host__ __device__ inline uint32_t ic_full_skip(uint64_t q, uint32_t beta) {
-  if (q == 0) return 0;
+// The largest U = hi32(x) + 1 whose gap is at least len (a block of len
+// edges without candidates), 0 if there is none.  NL is strictly decreasing,
+// so gap >= len holds exactly for U <= this value.
+__host__ __device__ inline uint32_t ic_skip_len(uint64_t q, uint32_t len) {
+  if (q == 0 || len == 0) return 0;
   const uint64_t top = ic_neg_log2(1);               // NL(1) = 32 * 2^58
-  if (q > (top >> beta)) return 0;                   // q 2^beta > NL(1)
-  const uint64_t need = q << beta;
+  if (q > top / len) return 0;                       // q len > NL(1)
+  const uint64_t need = q * len;
   uint64_t lo = 1, hi = 1ull << 32;                  // NL(lo) >= need > NL(hi)
+  if (ic_neg_log2(lo) < need) return 0;
   while (hi - lo > 1) {
     const uint64_t mid = lo + (hi - lo) / 2;
     if (ic_neg_log2(mid) >= need)
@@ -118,6 +120,32 @@ __host__ __device__ inline uint32_t ic_full_skip(uint64_t q, uint32_t beta) {
       hi = mid;
   }
   return static_cast<uint32_t>(lo);
+}
+
+// The same for a full block of 2^beta edges.
+__host__ __device__ inline uint32_t ic_full_skip(uint64_t q, uint32_t beta) {
+  return ic_skip_len(q, 1u << beta);
+}
+
+// floor(2^64 / q) for q >= 2 (0 for q = 0): the reciprocal ic_gap_fast
+// divides by.  Every row has q = 0 or q >= NL(2^32 - 2^8) > 2^34.
+__host__ __device__ inline uint64_t ic_gap_recip(uint64_t q) {
+  if (q < 2) return 0;
+  const uint64_t r0 = ~0ull / q;                     // floor((2^64 - 1) / q)
+  const uint64_t rem = ~0ull - r0 * q;               // (2^64 - 1) mod q
+  return rem == q - 1 ? r0 + 1 : r0;                 // q divides 2^64
+}
+
+// ic_gap(x, q) with the division done as a multiplication by rq =
+// ic_gap_recip(q): with N = NL(U) < 2^64, N rq / 2^64 lies in (N/q - 1, N/q],
+// so its floor is floor(N/q) or one less, and one compare settles which;
+// (g + 1) q <= N + q stays below 2^64 since N, q <= 2^63.
+__host__ __device__ inline uint64_t ic_gap_fast(uint64_t x, uint64_t q, uint64_t rq) {
+  if (q == 0) return 0;
+  const uint64_t N = ic_neg_log2((x >> 32) + 1);
+  uint64_t g = mulhi_u64(N, rq);
+  if ((g + 1) * q <= N) ++g;
+  return g;
 }
 
 }  // namespace rip

@@ -62,6 +62,23 @@ struct WaveScratch {
   uint32_t *d_lsort = nullptr;
 };
 
+// Scratch of the grid IC sampler (sample_ic_grid.hip): per colour group of a
+// batch the visited and pending colour masks of every vertex and the union
+// bitmaps; two frontier buffers of fcap items (g << 32 | v), which the output
+// reuses as staging; control words and a pinned copy for the host.
+struct GridScratch {
+  uint32_t groups = 0, n = 0;
+  uint64_t fcap = 0;
+  unsigned long long *d_vis = nullptr, *d_pend = nullptr;  // groups x n
+  uint32_t *d_bm0 = nullptr, *d_bm1 = nullptr;
+  unsigned long long *d_z = nullptr;   // groups x 64
+  uint32_t *d_sid = nullptr;           // groups x 64
+  uint32_t *d_nb = nullptr;            // groups
+  unsigned long long *d_fa = nullptr, *d_fb = nullptr;
+  unsigned long long *d_gctl = nullptr, *d_union = nullptr;
+  unsigned long long *h_ctl = nullptr;
+};
+
 // Forward CSR and per-workgroup scratch of the spread simulator, built on
 // first use (spread.hip).
 struct FwdGraph {
@@ -164,6 +181,7 @@ struct rip_graph {
   std::mutex sampler_mu;
   rip::IcSlots ic;
   rip::WaveScratch wave;
+  rip::GridScratch grid;
   rip::LtScratch lt;
   rip::FwdGraph fwd;
 };
@@ -199,6 +217,9 @@ struct rip_pool {
   unsigned long long *h_ctl = nullptr;  // pinned host copy of the control words
   rip::SelectScratch sel;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipEvent_t ev3 = nullptr, ev4 = nullptr;  // around each grid-sampler level
+  double grid_ms = 0.0;       // last sampling call: the grid sampler's level kernels
+  uint32_t grid_levels = 0;   // and how many ran
   double last_kernel_ms = 0.0;
   double last_call_ms = 0.0;
   uint32_t last_launches = 0;
@@ -307,6 +328,15 @@ uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
 // unused (the rest of each wave's last output chunk).
 uint64_t ic_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune);
 void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count);
+
+// sample_ic_grid.hip: samples first + ids[i] (first + i without ids), i in
+// [0, count), 64 per colour group and many groups per batch, one BFS level
+// of all groups per kernel; with columns, groups with many hot members go to
+// the pool's column store.  frontier_cap limits the entries a level lists
+// (0 = automatic).  Returns the kernel launches made.
+uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
+                        bool columns, uint32_t frontier_cap);
+void grid_scratch_free(GridScratch &s);
 
 // sample_ic_lane.hip: one BFS per lane over samples [first, first + count);
 // returns how many outgrew a lane's queue, their indices (relative to first)

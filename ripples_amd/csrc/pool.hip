@@ -233,6 +233,8 @@ void pool_create_resources(rip_pool *p) {
   RIP_HIP(hipEventCreate(&p->ev0));
   RIP_HIP(hipEventCreate(&p->ev1));
   RIP_HIP(hipEventCreate(&p->ev2));
+  RIP_HIP(hipEventCreate(&p->ev3));
+  RIP_HIP(hipEventCreate(&p->ev4));
   p->d_ctl = dev_alloc<unsigned long long>(kCtlWords);
   // pinned, so reading the control words back after a launch is one DMA
   // rather than a copy staged through a driver buffer
@@ -274,7 +276,9 @@ void pool_destroy(rip_pool *p) {
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->ev2) (void)hipEventDestroy(p->ev2);
-  p->ev0 = p->ev1 = p->ev2 = nullptr;
+  if (p->ev3) (void)hipEventDestroy(p->ev3);
+  if (p->ev4) (void)hipEventDestroy(p->ev4);
+  p->ev0 = p->ev1 = p->ev2 = p->ev3 = p->ev4 = nullptr;
   if (p->stream) (void)hipStreamDestroy(p->stream);
   p->stream = nullptr;
   p->sel = SelectScratch{};
@@ -459,6 +463,8 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
   if (p->g->model == RIP_IC) ic = ic_tune(p, tune);
   const uint64_t first = p->num_sets;
   std::lock_guard<std::mutex> hold(p->g->sampler_mu);  // graph-owned sampler scratch
+  p->grid_ms = 0.0;
+  p->grid_levels = 0;
   for (int attempt = 0;; ++attempt) {
     RIP_REQUIRE(attempt < 40, -2, "pool: data capacity did not converge");
     hipLaunchKernelGGL(set_ctl_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl,

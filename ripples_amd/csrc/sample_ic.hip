@@ -137,6 +137,8 @@ ic_fused_kernel(IcParams P) {
   __shared__ uint32_t s_eT[BT];    // the row's largest threshold
   __shared__ uint32_t s_eflags[BT];
   __shared__ uint32_t s_eufull[BT];
+  __shared__ uint32_t s_eulast[BT];
+  __shared__ unsigned long long s_erq[BT];
   __shared__ uint32_t s_epre[BT];
   __shared__ unsigned long long s_eq[BT];
   __shared__ unsigned long long s_ea[BT];
@@ -240,6 +242,8 @@ ic_fused_kernel(IcParams P) {
           s_eT[tid] = r.thr;
           s_eflags[tid] = r.flags;
           s_eufull[tid] = r.ufull;
+          s_eulast[tid] = r.ulast;
+          s_erq[tid] = r.rq;
           s_eq[tid] = r.q;
           s_ea[tid] = r.ja;
           s_eb[tid] = r.jb;
@@ -262,8 +266,8 @@ ic_fused_kernel(IcParams P) {
           // this lane's item: block k of entry ent, for the colours in mm
           unsigned long long mm = 0;
           Affine E{1, 0};
-          uint32_t e0 = 0, len = 0, B = 0, T = 0, ufull = 0;
-          uint64_t q = 0;
+          uint32_t e0 = 0, len = 0, B = 0, T = 0, uskip = 0;
+          uint64_t q = 0, rq = 0;
           bool uniform = true;
           if (rr < total) {
             uint32_t lo = 0, hi = cl;
@@ -281,7 +285,10 @@ ic_fused_kernel(IcParams P) {
             len = min(s_edeg[ent] - (k << beta), B);
             T = s_eT[ent];
             q = s_eq[ent];
-            ufull = s_eufull[ent];
+            // a block without candidates costs one compare: the full or
+            // the partial last block's skip threshold
+            uskip = len == B ? s_eufull[ent] : s_eulast[ent];
+            rq = s_erq[ent];
             uniform = (fl & kRowUniform) != 0;
             mm = s_em[ent];
             // block k's first element: P^(2 B k) after the row's first
@@ -304,11 +311,10 @@ ic_fused_kernel(IcParams P) {
                 x = E.a * s_z[c] + E.b;
                 pos = 0;
                 ++st_coins;
-                // a full block without candidates costs one compare
-                if (len == B && (x >> 32) + 1 <= ufull) pos = len;
+                if ((x >> 32) + 1 <= uskip) pos = len;
                 continue;
               }
-              const uint64_t gap = ic_gap(x, q);
+              const uint64_t gap = ic_gap_fast(x, q, rq);
               if (gap >= len - pos) {
                 pos = len;
                 continue;
@@ -655,11 +661,14 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
   const bool lanes = cfg.sampler == RIP_SAMPLER_AUTO || cfg.sampler == RIP_SAMPLER_LANE;
   if (lanes) {
     // one sample per lane first; the few that outgrow a lane's queue go to
-    // one BFS per wave
+    // the grid sampler (AUTO) or to one BFS per wave (LANE)
     wave_count = ic_lane_launch(p, first, count, tune);
     launches += 3;
     if (wave_count == 0) return launches;
     wave_ids = p->g->wave.d_over;
+    if (cfg.sampler == RIP_SAMPLER_AUTO)
+      return launches + ic_grid_launch(p, first, wave_count, wave_ids, true,
+                                       tune ? tune->frontier_cap : 0u);
   }
   const uint64_t big = ic_wave_launch(p, first, wave_count, wave_ids, cfg.out_chunk, !lanes);
   launches += 2;

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
   unsigned long long tag = 0;
   uint32_t head = 0, tail = 0;
   uint32_t rbegin = 0, deg = 0, T = 0, beta = 0, nblk = 0, k = 0, pos = 0, len = 0;
-  uint64_t q = 0, ufull = 0, xb = 0, x = 0;
+  uint64_t q = 0, rq = 0, ufull = 0, ulast = 0, xb = 0, x = 0;
   bool uniform = true;
   unsigned long long st_cand = 0, st_draws = 0, st_exp = 0, st_bytes = 0, st_live = 0;
 
@@ -180,6 +180,8 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
           beta = (rv.flags >> kBetaShift) & 31u;
           q = rv.q;
           ufull = rv.ufull;
+          ulast = rv.ulast;
+          rq = rv.rq;
           nblk = ((deg - 1) >> beta) + 1;
           k = 0;
           xb = rv.ja * z + rv.jb;                  // the row's first element
@@ -187,8 +189,8 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
           pos = 0;
           len = min(deg, 1u << beta);
           ++st_draws;
-          // a full block without candidates costs one compare
-          if (len == (1u << beta) && (xb >> 32) + 1 <= ufull) len = 0;
+          // a block without candidates costs one compare
+          if ((xb >> 32) + 1 <= (len == (1u << beta) ? ufull : ulast)) len = 0;
           state = kRow;
         }
       } else {
@@ -203,12 +205,12 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
           pos = 0;
           len = min(deg - (k << beta), 1u << beta);
           ++st_draws;
-          if (len == (1u << beta) && (xb >> 32) + 1 <= ufull) len = 0;
+          if ((xb >> 32) + 1 <= (len == (1u << beta) ? ufull : ulast)) len = 0;
         } else {
           state = kPop;
         }
       } else {
-        const uint64_t gap = ic_gap(x, q);
+        const uint64_t gap = ic_gap_fast(x, q, rq);
         if (gap >= len - pos) {
           pos = len;
         } else {

@@ -102,6 +102,8 @@ __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
   const uint32_t B = 1u << beta;
   const uint64_t q = uniform_u64(rv.q);
   const uint64_t ufull = uniform_u32(rv.ufull);
+  const uint64_t ulast = uniform_u32(rv.ulast);
+  const uint64_t rq = uniform_u64(rv.rq);
   const uint32_t nblk = ((deg - 1) >> beta) + 1;
   const Affine P = p2[0];
   const uint64_t x0 = uniform_u64(rv.ja) * z + uniform_u64(rv.jb);  // row's first element
@@ -117,8 +119,8 @@ __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
     if (k < nblk) {
       len = min(deg - (k << beta), B);
       ++st_draws;
-      // a full block without candidates costs one compare
-      if (len == B && (xb >> 32) + 1 <= ufull) len = 0;
+      // a block without candidates costs one compare
+      if ((xb >> 32) + 1 <= (len == B ? ufull : ulast)) len = 0;
     }
     uint32_t pos = 0;
     uint64_t x = xb;
@@ -129,7 +131,7 @@ __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
       if (pos < len) {
         if (!first) ++st_draws;
         first = false;
-        const uint64_t gap = ic_gap(x, q);
+        const uint64_t gap = ic_gap_fast(x, q, rq);
         if (gap >= len - pos) {
           pos = len;
         } else {

@@ -184,7 +184,8 @@ def test_selector_graph_replays_across_pool_growth(orc):
                                   dict(queue_cap=64, out_chunk=3),
                                   dict(sampler=2, colors=64, heavy_degree=64),
                                   dict(out_chunk=1), dict(out_chunk=5, sampler=1),
-                                  dict(out_chunk=1 << 20)])
+                                  dict(out_chunk=1 << 20),
+                                  dict(frontier_cap=7), dict(frontier_cap=64, queue_cap=64)])
 def test_ic_tuning_does_not_change_results(tune, orc):
     index, edges, w = rmat_csr(12, R.W_UNIFORM, 0.1, 1)
     g = R.Graph(index, edges, w, model=R.IC)
