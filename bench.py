@@ -160,7 +160,7 @@ def sampling_steps(pool, batch, steps, warmup, tune):
     for _ in range(warmup):
         pool.reset()
         pool.generate_rrr_sets(batch, tune=tune)
-    kms, calls, kstats = [], [], []
+    kms, calls, kstats, grids = [], [], [], []
     t0 = time.time()
     for _ in range(steps):
         pool.reset()
@@ -168,8 +168,9 @@ def sampling_steps(pool, batch, steps, warmup, tune):
         kms.append(pool.last_kernel_ms()[0])
         calls.append(pool.last_call_ms())
         kstats.append(pool.last_kernel_stats())
+        grids.append(pool.last_grid())
     wall = time.time() - t0
-    return wall, kms, calls, kstats
+    return wall, kms, calls, kstats, grids
 
 
 def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, fused_steps,
@@ -189,7 +190,7 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
         name, rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
     batch = batch or int(rec["Theta"])
     pool = g.pool(seed)
-    wall, kms, calls, kstats = sampling_steps(pool, batch, steps, warmup, tune)
+    wall, kms, calls, kstats, grids = sampling_steps(pool, batch, steps, warmup, tune)
     stats = pool.last_stats()
     n_sets, n_elem = pool.size()
     kernel_ms = float(np.mean(kms))
@@ -201,6 +202,19 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
         per_launch = ks["algo_bytes"]
         roof["kernel"] = "ic_lane_kernel" if (tune or {}).get("sampler", 0) in (0, 3) \
             else "ic_wave_kernel"
+        grid_ms = float(np.mean([gr[0] for gr in grids]))
+        grid_levels = float(np.mean([gr[1] for gr in grids]))
+        grid_bytes = float(np.mean([gr[2]["algo_bytes"] for gr in grids]))
+        roof["lane_kernel_ms"] = kernel_ms
+        roof["grid_levels_ms"] = grid_ms
+        roof["grid_levels"] = grid_levels
+        if grid_ms > kernel_ms:
+            # the samples that outgrew a lane dominate: the grid sampler's
+            # level kernels are the dominant kernel, timed over all levels
+            roof["kernel"] = "grid_expand_kernel + grid_tail_kernel (all BFS levels of the call)"
+            per_launch = grid_bytes
+            kernel_ms = grid_ms
+            roof["kernel_ms"] = grid_ms
         roof["algo_bytes_formula"] = ("64 per row expanded (vertex record) + 4 per candidate "
                                       "edge of a row with mixed thresholds + 4 per live edge "
                                       "(in-neighbour) + 4 per output id, counted on the device")
@@ -269,7 +283,7 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
         for label, kind in (("plain_wave", 1), ("colour_groups", 2)):
             tz = dict(tune or {})
             tz["sampler"] = kind
-            pw, pk, _, _ = sampling_steps(pool, batch, fused_steps, 1, tz)
+            pw, pk, _, _, _ = sampling_steps(pool, batch, fused_steps, 1, tz)
             res[label] = {"value": batch * fused_steps / pw, "ms_per_step": pw / fused_steps * 1e3,
                           "kernel_ms": float(np.mean(pk))}
             res[label]["batched_lanes_over_this"] = res["value"] / res[label]["value"]

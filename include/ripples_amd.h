@@ -184,7 +184,12 @@ int rip_pool_export(rip_pool *p, uint64_t first, uint64_t count,
  * (include/ripples/find_most_influential.h:85-196,
  * streaming_find_most_influential.h:607-643).  Ties go to the lowest vertex
  * id, the rule of GPUMaxElement (src/gpu/find_most_influential.cc:65-72).
- * Stops after k seeds or when every set is covered.  seeds: uint32[k]. */
+ * Stops after k seeds or when no vertex covers an uncovered set: every set
+ * covered, or only empty sets left.  In the latter case the reference's
+ * sequential loop (find_most_influential.h:129-160) goes on adding zero-gain
+ * vertices in its heap order; sampled pools hold no empty sets (every set
+ * holds its root), so only host-appended pools can tell the two apart.
+ * seeds: uint32[k]. */
 int rip_pool_select(rip_pool *p, uint64_t num_sets, uint32_t k,
                     uint32_t *seeds, uint32_t *num_seeds, uint64_t *covered);
 /* Occurrence counts of every vertex over sets [0, num_sets) --

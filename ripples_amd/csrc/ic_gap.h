@@ -62,20 +62,41 @@ __host__ __device__ inline int clz_u64(uint64_t x) {
 #endif
 }
 
-// -log2(u / 2^32) for u in [1, 2^32], in units of 2^-58 (0 for u >= 2^32).
-__host__ __device__ inline uint64_t ic_neg_log2(uint64_t u) {
+// -log2(u / 2^32) for u in [1, 2^32], in units of 2^-58 (0 for u >= 2^32),
+// with the tables kLog2C and kRecipC read from log2c and recipc (kernels pass
+// copies in LDS).
+__host__ __device__ inline uint64_t ic_neg_log2_tab(uint64_t u, const uint64_t *log2c,
+                                                    const uint64_t *recipc) {
   if (u >= (1ull << 32)) return 0;
   const int e = 63 - clz_u64(u);                   // floor(log2 u), 0..31
   const uint64_t m = u << (63 - e);                // mantissa in [2^63, 2^64)
   const uint32_t i = static_cast<uint32_t>(m >> 55) & 255u;
   const uint64_t r = m & ((1ull << 55) - 1);       // m = 2^63 (1 + i/256) + r
   // x = r 2^-63 / (1 + i/256) < 2^-8, held as X = x 2^64
-  const uint64_t c = kRecipC[i];
+  const uint64_t c = recipc[i];
   const uint64_t X = (mulhi_u64(r, c) << 2) | ((r * c) >> 62);
   const uint64_t X2 = mulhi_u64(X, X), X3 = mulhi_u64(X2, X), X4 = mulhi_u64(X3, X);
   const uint64_t ln = X - X2 / 2 + X3 / 3 - X4 / 4;    // ln(1 + x), Q0.64
-  const uint64_t frac = kLog2C[i] + (mulhi_u64(ln, RIP_INV_LN2_Q63) << 1);
+  const uint64_t frac = log2c[i] + (mulhi_u64(ln, RIP_INV_LN2_Q63) << 1);
   return (static_cast<uint64_t>(32 - e) << 58) - (frac >> 6);
+}
+
+__host__ __device__ inline uint64_t ic_neg_log2(uint64_t u) {
+  return ic_neg_log2_tab(u, kLog2C, kRecipC);
+}
+
+// The tables of ic_neg_log2, for a copy in LDS.
+struct IcLogTables {
+  uint64_t log2c[256], recipc[256];
+};
+
+// Fills t (LDS) from the constant tables; every thread of the workgroup calls
+// it, the caller synchronises.
+__device__ inline void ic_load_tables(IcLogTables *t) {
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+    t->log2c[i] = kLog2C[i];
+    t->recipc[i] = kRecipC[i];
+  }
 }
 
 // q of a row whose largest threshold is T (1 <= T <= 2^24).
@@ -140,9 +161,10 @@ __host__ __device__ inline uint64_t ic_gap_recip(uint64_t q) {
 // ic_gap_recip(q): with N = NL(U) < 2^64, N rq / 2^64 lies in (N/q - 1, N/q],
 // so its floor is floor(N/q) or one less, and one compare settles which;
 // (g + 1) q <= N + q stays below 2^64 since N, q <= 2^63.
-__host__ __device__ inline uint64_t ic_gap_fast(uint64_t x, uint64_t q, uint64_t rq) {
+__host__ __device__ inline uint64_t ic_gap_fast(uint64_t x, uint64_t q, uint64_t rq,
+                                                const IcLogTables *t) {
   if (q == 0) return 0;
-  const uint64_t N = ic_neg_log2((x >> 32) + 1);
+  const uint64_t N = ic_neg_log2_tab((x >> 32) + 1, t->log2c, t->recipc);
   uint64_t g = mulhi_u64(N, rq);
   if ((g + 1) * q <= N) ++g;
   return g;

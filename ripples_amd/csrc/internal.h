@@ -76,6 +76,7 @@ struct GridScratch {
   uint32_t *d_nb = nullptr;            // groups
   unsigned long long *d_fa = nullptr, *d_fb = nullptr;
   unsigned long long *d_gctl = nullptr, *d_union = nullptr;
+  unsigned long long *d_ring = nullptr;  // per-level frontier lengths and chunk counters
   unsigned long long *h_ctl = nullptr;
 };
 

@@ -463,10 +463,10 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
   if (p->g->model == RIP_IC) ic = ic_tune(p, tune);
   const uint64_t first = p->num_sets;
   std::lock_guard<std::mutex> hold(p->g->sampler_mu);  // graph-owned sampler scratch
-  p->grid_ms = 0.0;
-  p->grid_levels = 0;
   for (int attempt = 0;; ++attempt) {
     RIP_REQUIRE(attempt < 40, -2, "pool: data capacity did not converge");
+    p->grid_ms = 0.0;     // timings of the attempt that succeeds
+    p->grid_levels = 0;
     hipLaunchKernelGGL(set_ctl_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl,
                        static_cast<unsigned long long>(p->used_data));
     pool_clear_gref(p, first, first + count);

@@ -132,6 +132,7 @@ ic_fused_kernel(IcParams P) {
   __shared__ uint32_t s_hn[2];   // deferred-hub lists
   __shared__ uint32_t s_union;
   __shared__ Affine s_p2[64];      // P^(2^j): jumps between gap blocks
+  __shared__ IcLogTables s_tab;
   __shared__ uint32_t s_ebeg[BT];
   __shared__ uint32_t s_edeg[BT];
   __shared__ uint32_t s_eT[BT];    // the row's largest threshold
@@ -168,6 +169,7 @@ ic_fused_kernel(IcParams P) {
     s_p2[0] = P.step;
     for (int j = 1; j < 64; ++j) s_p2[j] = aff_then(s_p2[j - 1], s_p2[j - 1]);
   }
+  ic_load_tables(&s_tab);
 
   for (;;) {
     if (tid == 0) s_group = atomicAdd(&P.ctl[kCtlGroupCounter], 1ull);
@@ -314,7 +316,7 @@ ic_fused_kernel(IcParams P) {
                 if ((x >> 32) + 1 <= uskip) pos = len;
                 continue;
               }
-              const uint64_t gap = ic_gap_fast(x, q, rq);
+              const uint64_t gap = ic_gap_fast(x, q, rq, &s_tab);
               if (gap >= len - pos) {
                 pos = len;
                 continue;

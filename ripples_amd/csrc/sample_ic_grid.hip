@@ -45,10 +45,17 @@ namespace {
 
 constexpr int kExpBT = 256;          // expand kernel
 constexpr int kOutBT = 1024;         // output kernel, one workgroup per group
-constexpr uint32_t kUnitBlocks = 8;  // gap blocks per unit of work
+// rows pending for this many colours or more are expanded a block per wave,
+// lane c drawing colour c, so a live edge is one discovery for all colours
+constexpr uint32_t kDenseColours = 8;
+// level counters kept on the device (a ring), and levels enqueued per host
+// synchronisation: a BFS of hundreds of narrow levels costs a launch per
+// level, not a round trip
+constexpr uint32_t kRing = 64;
+constexpr uint32_t kLevelsPerSync = 16;
 
 // control words of the grid scratch
-enum : int { kGHead = 0, kGNext = 1, kGWords = 4 };
+enum : int { kGOver = 0, kGLevels = 1, kGCount = 2, kGWords = 4 };
 
 struct GridParams {
   const VertexRec *vrec;
@@ -64,8 +71,12 @@ struct GridParams {
   const unsigned long long *cur;   // frontier items g << 32 | v
   uint64_t ncur;
   unsigned long long *next;        // next frontier
+  unsigned long long *next_cnt;    // its length (counts every append, also past cap)
   uint64_t cap;                    // capacity of next
-  unsigned long long *gctl;
+  unsigned long long *gctl;        // control words: overflow flag, level count
+  unsigned long long *ring;        // per level L: ring[L % kRing] frontier length,
+                                   // ring[kRing + L % kRing] chunk counter
+  unsigned long long *buf[2];      // level L reads buf[L % 2], writes buf[(L + 1) % 2]
   unsigned long long *ctl;         // the pool's control words (work counters)
 };
 
@@ -91,27 +102,28 @@ __device__ inline uint32_t nth_bit(uint64_t m, uint32_t j) {
   return pos;
 }
 
-// Marks colour `bit` of group g discovered at u (for lanes with live set);
-// (g, u) joins the next frontier when its pend word was zero.  Every lane of
-// the wave must call it.
+// Marks the colours `bits` of group g discovered at u (for lanes with live
+// set); (g, u) joins the next frontier when its pend word was zero.  Every
+// lane of the wave must call it.
 __device__ inline void grid_discover(const GridParams &P, bool live, uint32_t g, uint32_t u,
-                                     unsigned long long bit) {
+                                     unsigned long long bits) {
   bool push = false;
   if (live) {
     const uint64_t idx = static_cast<uint64_t>(g) * P.n + u;
     // vis is written by atomics, which complete in L2; the read goes there too
     unsigned long long old = __hip_atomic_load(&P.vis[idx], __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
-    if ((old & bit) == 0) {
-      old = atomicOr(&P.vis[idx], bit);
-      if ((old & bit) == 0) {
+    if (bits & ~old) {
+      old = atomicOr(&P.vis[idx], bits);
+      const unsigned long long fresh = bits & ~old;
+      if (fresh) {
         if (old == 0) {
           uint32_t *b0 = P.bm0 + static_cast<uint64_t>(g) * P.nw0;
           const uint32_t o0 = atomicOr(&b0[u >> 5], 1u << (u & 31u));
           if (o0 == 0)
             atomicOr(&P.bm1[static_cast<uint64_t>(g) * P.nw1 + (u >> 10)], 1u << ((u >> 5) & 31u));
         }
-        push = atomicOr(&P.pend[idx], bit) == 0;
+        push = atomicOr(&P.pend[idx], fresh) == 0;
       }
     }
   }
@@ -119,12 +131,22 @@ __device__ inline void grid_discover(const GridParams &P, bool live, uint32_t g,
   if (m == 0) return;
   const uint32_t leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
   unsigned long long base = 0;
-  if (lane_id() == leader) base = atomicAdd(&P.gctl[kGNext], static_cast<unsigned long long>(__popcll(m)));
+  if (lane_id() == leader) base = atomicAdd(P.next_cnt, static_cast<unsigned long long>(__popcll(m)));
   base = __shfl(base, static_cast<int>(leader), 64);
   if (push) {
     const unsigned long long at = base + __popcll(m & lanes_below());
     if (at < P.cap) P.next[at] = (static_cast<unsigned long long>(g) << 32) | u;
   }
+}
+
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+  for (int d = 32; d > 0; d >>= 1) v = min(v, static_cast<uint32_t>(__shfl_xor(v, d, 64)));
+  return v;
+}
+
+__device__ inline unsigned long long wave_or_u64(unsigned long long v) {
+  for (int d = 32; d > 0; d >>= 1) v |= __shfl_xor(v, d, 64);
+  return v;
 }
 
 // Batch start: the root of every colour of every group (one 64-lane block
@@ -137,159 +159,331 @@ __global__ void __launch_bounds__(64) grid_seed_kernel(GridParams P, const uint3
   grid_discover(P, has, g, root, 1ull << c);
 }
 
-// One BFS level over the frontier items cur[0, ncur).
+// Shared memory of the expansion: the step powers and, per item of a chunk,
+// its row and pending colours with the prefix of its work.
 template <int BT>
-__global__ void __launch_bounds__(BT) grid_expand_kernel(GridParams P) {
+struct ExpandLds {
+  Affine p2[64];
+  IcLogTables tab;
+  uint32_t tmp[BT / 64];
+  uint32_t g[BT], beg[BT], deg[BT], T[BT], fl[BT], uf[BT], ul[BT], pre[BT], nch[BT], dpre[BT];
+  unsigned long long q[BT], rq[BT], ja[BT], jb[BT], m[BT];
+  uint32_t ucur;   // next unit of the chunk not claimed by a wave
+};
+
+struct ExpandStats {
+  unsigned long long cand = 0, draws = 0, exp = 0, bytes = 0, live = 0;
+};
+
+// Expands the frontier items cur[c0, c0 + cl), cl <= BT: every thread of
+// the workgroup calls it.  A chunk's rows are split into units of one colour
+// and up to eight gap blocks (rows pending for few colours) or into wave
+// tasks of one block for every colour (rows pending for many).
+template <int BT>
+__device__ void expand_chunk(const GridParams &P, const unsigned long long *cur, uint64_t c0,
+                             uint32_t cl, ExpandLds<BT> &S, ExpandStats &st) {
   constexpr int NW = BT / 64;
-  __shared__ Affine s_p2[64];
-  __shared__ uint32_t s_tmp[NW];
-  __shared__ unsigned long long s_chunk;
-  __shared__ uint32_t s_g[BT], s_beg[BT], s_deg[BT], s_T[BT], s_fl[BT], s_uf[BT], s_ul[BT];
-  __shared__ uint32_t s_pre[BT], s_nch[BT];
-  __shared__ unsigned long long s_q[BT], s_rq[BT], s_ja[BT], s_jb[BT], s_m[BT];
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) fill_step_powers(P.step, s_p2);
-  unsigned long long st_cand = 0, st_draws = 0, st_exp = 0, st_bytes = 0, st_live = 0;
-  for (;;) {
-    __syncthreads();
-    if (tid == 0) s_chunk = atomicAdd(&P.gctl[kGHead], static_cast<unsigned long long>(BT));
-    __syncthreads();
-    const uint64_t c0 = s_chunk;
-    if (c0 >= P.ncur) break;
-    const uint32_t cl = static_cast<uint32_t>(min(static_cast<uint64_t>(BT), P.ncur - c0));
-    uint32_t units = 0;
-    if (tid < cl) {
-      const unsigned long long item = P.cur[c0 + tid];
-      const uint32_t g = static_cast<uint32_t>(item >> 32), v = static_cast<uint32_t>(item);
-      // the colours still to expand at v; a repeated item finds none
-      const unsigned long long M = atomicExch(&P.pend[static_cast<uint64_t>(g) * P.n + v], 0ull);
-      uint32_t nch = 0;
-      if (M) {
-        const VertexRec r = P.vrec[v];
-        ++st_exp;
-        st_bytes += kIcRecBytes;
-        s_g[tid] = g;
-        s_beg[tid] = r.begin;
-        s_deg[tid] = r.end - r.begin;
-        s_T[tid] = r.thr;
-        s_fl[tid] = r.flags;
-        s_uf[tid] = r.ufull;
-        s_ul[tid] = r.ulast;
-        s_q[tid] = r.q;
-        s_rq[tid] = r.rq;
-        s_ja[tid] = r.ja;
-        s_jb[tid] = r.jb;
-        s_m[tid] = M;
-        if (r.thr) {
-          const uint32_t beta = (r.flags >> kBetaShift) & 31u;
-          const uint32_t nblk = ((r.end - r.begin - 1) >> beta) + 1;
-          nch = (nblk + kUnitBlocks - 1) / kUnitBlocks;
+  __syncthreads();   // the previous chunk's waves are done with the tables
+  Affine *s_p2 = S.p2;
+  uint32_t *s_tmp = S.tmp, *s_g = S.g, *s_beg = S.beg, *s_deg = S.deg, *s_T = S.T, *s_fl = S.fl;
+  uint32_t *s_uf = S.uf, *s_ul = S.ul, *s_pre = S.pre, *s_nch = S.nch, *s_dpre = S.dpre;
+  unsigned long long *s_q = S.q, *s_rq = S.rq, *s_ja = S.ja, *s_jb = S.jb, *s_m = S.m;
+  unsigned long long &st_cand = st.cand, &st_draws = st.draws, &st_exp = st.exp;
+  unsigned long long &st_bytes = st.bytes, &st_live = st.live;
+  uint32_t units = 0, ntask = 0;
+  if (tid < cl) {
+    const unsigned long long item = cur[c0 + tid];
+    const uint32_t g = static_cast<uint32_t>(item >> 32), v = static_cast<uint32_t>(item);
+    // the colours still to expand at v; a repeated item finds none
+    const unsigned long long M = atomicExch(&P.pend[static_cast<uint64_t>(g) * P.n + v], 0ull);
+    uint32_t nch = 0;
+    if (M) {
+      const VertexRec r = P.vrec[v];
+      ++st_exp;
+      st_bytes += kIcRecBytes;
+      s_g[tid] = g;
+      s_beg[tid] = r.begin;
+      s_deg[tid] = r.end - r.begin;
+      s_T[tid] = r.thr;
+      s_fl[tid] = r.flags;
+      s_uf[tid] = r.ufull;
+      s_ul[tid] = r.ulast;
+      s_q[tid] = r.q;
+      s_rq[tid] = r.rq;
+      s_ja[tid] = r.ja;
+      s_jb[tid] = r.jb;
+      s_m[tid] = M;
+      if (r.thr) {
+        const uint32_t beta = (r.flags >> kBetaShift) & 31u;
+        const uint32_t nblk = ((r.end - r.begin - 1) >> beta) + 1;
+        if (__popcll(M) >= kDenseColours) {
+          ntask = nblk;   // one wave per block, a lane per colour
+        } else {
+          nch = nblk;   // a unit per (colour, block)
           units = static_cast<uint32_t>(__popcll(M)) * nch;
         }
       }
-      s_nch[tid] = nch;
     }
-    uint32_t total;
-    const uint32_t pre = block_exclusive_scan<BT>(units, s_tmp, &total);
-    if (tid < cl) s_pre[tid] = pre;
-    __syncthreads();
-    for (uint32_t rb = 0; rb < total; rb += BT) {
-      const uint32_t rr = rb + tid;
-      // this lane's unit: colour c of entry ent, blocks [k, k1)
-      uint32_t g = 0, c = 0, k = 0, k1 = 0, rbeg = 0, deg = 0, T = 0, beta = 0, B = 0;
-      uint32_t uf = 0, ul = 0;
-      uint64_t q = 0, rq = 0, xb = 0;
-      bool uniform = true;
-      if (rr < total) {
-        uint32_t lo = 0, hi = cl;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_pre[mid] <= rr) lo = mid + 1;
-          else hi = mid;
+    s_nch[tid] = nch;
+  }
+  uint32_t total, dtotal;
+  const uint32_t pre = block_exclusive_scan<BT>(units, s_tmp, &total);
+  const uint32_t dpre = block_exclusive_scan<BT>(ntask, s_tmp, &dtotal);
+  if (tid < cl) {
+    s_pre[tid] = pre;
+    s_dpre[tid] = dpre;
+  }
+  if (tid == 0) S.ucur = 0;
+  __syncthreads();
+  // Rows pending for few colours: unit u is block k of colour j of entry
+  // ent.  Each wave claims 64 units at a time from the chunk's counter; a
+  // lane with no unit takes the next one, and every step each lane with a
+  // unit makes one draw (the block's first element and its first gap, or the
+  // next gap) and tests its candidate, so the lanes stay busy whatever the
+  // lengths of their units; a live edge is discovered in the same step.
+  const uint32_t lane = lane_id(), wid = tid >> 6;
+  {
+    const Affine Pst = s_p2[0];
+    uint32_t nxt = 0, hi = 0;   // the wave's claimed units [nxt, hi) (wave-uniform)
+    bool has = false, started = false;
+    uint32_t g = 0, e0 = 0, len = 0, pos = 0, T = 0, uskip = 0;
+    unsigned long long bit = 0;
+    uint64_t q = 0, rq = 0, x = 0;
+    bool uniform = true;
+    for (;;) {
+      const uint64_t idle = __ballot(!has);
+      if (idle) {
+        if (nxt >= hi) {
+          uint32_t c0 = 0;
+          if (lane == 0) c0 = atomicAdd(&S.ucur, 64u);
+          c0 = uniform_u32(__shfl(c0, 0, 64));
+          nxt = min(c0, total);
+          hi = min(c0 + 64u, total);
         }
-        // entries without units share their prefix with the next one: take
-        // the last entry whose prefix is <= rr, which has units
-        const uint32_t ent = lo - 1;
-        const uint32_t u = rr - s_pre[ent];
-        const uint32_t nch = s_nch[ent];
-        const uint32_t j = u / nch, kc = u - j * nch;
-        g = s_g[ent];
-        c = nth_bit(s_m[ent], j);
-        const uint32_t fl = s_fl[ent];
-        beta = (fl >> kBetaShift) & 31u;
-        B = 1u << beta;
-        rbeg = s_beg[ent];
-        deg = s_deg[ent];
-        T = s_T[ent];
-        uf = s_uf[ent];
-        ul = s_ul[ent];
-        q = s_q[ent];
-        rq = s_rq[ent];
-        uniform = (fl & kRowUniform) != 0;
-        const uint32_t nblk = ((deg - 1) >> beta) + 1;
-        k = kc * kUnitBlocks;
-        k1 = min(k + kUnitBlocks, nblk);
-        // block k's first element: P^(2 B k) after the row's first
-        Affine E{s_ja[ent], s_jb[ent]};
-        for (uint32_t kk = k, bit = beta + 1; kk; kk >>= 1, ++bit)
-          if (kk & 1) E = aff_then(E, s_p2[bit]);
-        xb = E.a * P.z[64ull * g + c] + E.b;
-      }
-      const Affine jB = s_p2[(beta + 1) & 63u];   // the next block: 2B elements on
-      const Affine Pst = s_p2[0];
-      const unsigned long long bit = 1ull << c;
-      uint32_t pos = 0, len = 0, e0 = 0;
-      uint64_t x = 0;
-      bool started = false;
-      // one live edge per lane per step; every lane takes part in each discovery
-      for (;;) {
-        bool live = false;
-        uint32_t src = 0;
-        for (;;) {
-          if (pos >= len) {
-            if (k >= k1) break;
-            if (started) xb = jB.a * xb + jB.b;
-            started = true;
-            x = xb;
-            e0 = rbeg + (k << beta);
-            len = min(deg - (k << beta), B);
+        if (!has) {
+          const uint32_t u = nxt + static_cast<uint32_t>(__popcll(idle & lanes_below()));
+          if (u < hi) {
+            uint32_t lo = 0, up = cl;
+            while (lo < up) {
+              const uint32_t mid = (lo + up) >> 1;
+              if (s_pre[mid] <= u) lo = mid + 1;
+              else up = mid;
+            }
+            const uint32_t ent = lo - 1;   // the last entry starting at or before u
+            const uint32_t r = u - s_pre[ent];
+            const uint32_t nblk = s_nch[ent];
+            const uint32_t j = r / nblk, k = r - j * nblk;
+            g = s_g[ent];
+            const uint32_t c = nth_bit(s_m[ent], j);
+            bit = 1ull << c;
+            const uint32_t fl = s_fl[ent];
+            const uint32_t beta = (fl >> kBetaShift) & 31u;
+            const uint32_t B = 1u << beta;
+            e0 = s_beg[ent] + (k << beta);
+            len = min(s_deg[ent] - (k << beta), B);
+            T = s_T[ent];
+            uskip = len == B ? s_uf[ent] : s_ul[ent];
+            q = s_q[ent];
+            rq = s_rq[ent];
+            uniform = (fl & kRowUniform) != 0;
+            // block k's first element: P^(2 B k) after the row's first
+            Affine E{s_ja[ent], s_jb[ent]};
+            for (uint32_t kk = k, b2 = beta + 1; kk; kk >>= 1, ++b2)
+              if (kk & 1) E = aff_then(E, s_p2[b2]);
+            x = E.a * P.z[64ull * g + c] + E.b;
             pos = 0;
-            ++k;
-            ++st_draws;
-            // a block without candidates costs one compare
-            if ((x >> 32) + 1 <= (len == B ? uf : ul)) len = 0;
-            continue;
-          }
-          const uint64_t gap = ic_gap_fast(x, q, rq);
-          if (gap >= len - pos) {
-            pos = len;
-            continue;
-          }
-          pos += static_cast<uint32_t>(gap);
-          const uint32_t e = e0 + pos;
-          const uint64_t a = Pst.a * x + Pst.b;        // acceptance element
-          ++st_cand;
-          if (!uniform) {
-            ++st_draws;
-            st_bytes += 4;
-          }
-          live = uniform || ic_accept(a, T, P.val[e]);
-          ++pos;
-          x = Pst.a * a + Pst.b;                       // next gap element
-          if (pos < len) ++st_draws;
-          if (live) {
-            src = P.ids[e];
-            ++st_live;
-            st_bytes += 4;
-            break;
+            has = true;
+            started = false;
           }
         }
-        if (__ballot(live) == 0) break;
-        grid_discover(P, live, g, src, bit);
+        nxt = min(nxt + static_cast<uint32_t>(__popcll(idle)), hi);
       }
+      if (__ballot(has) == 0) {
+        if (nxt >= total) break;
+        continue;
+      }
+      bool live = false;
+      uint32_t src = 0;
+      if (has) {
+        if (!started) {
+          started = true;
+          ++st_draws;
+          // a block without candidates costs one compare
+          if ((x >> 32) + 1 <= uskip) has = false;
+        }
+        if (has) {
+          const uint64_t gap = ic_gap_fast(x, q, rq, &S.tab);
+          if (gap >= len - pos) {
+            has = false;
+          } else {
+            pos += static_cast<uint32_t>(gap);
+            const uint32_t e = e0 + pos;
+            const uint64_t a = Pst.a * x + Pst.b;        // acceptance element
+            ++st_cand;
+            if (!uniform) {
+              ++st_draws;
+              st_bytes += 4;
+            }
+            live = uniform || ic_accept(a, T, P.val[e]);
+            ++pos;
+            x = Pst.a * a + Pst.b;                       // next gap element
+            if (pos < len) ++st_draws;
+            else has = false;
+            if (live) {
+              src = P.ids[e];
+              ++st_live;
+              st_bytes += 4;
+            }
+          }
+        }
+      }
+      grid_discover(P, live, g, src, bit);
     }
   }
-  report_stats(P.ctl, st_bytes, st_cand, st_draws, st_exp, st_live);
+  // Rows pending for many colours: wave task t is block k of entry ent.
+  // Lane c walks colour c's candidates of the block; the wave takes them a
+  // window of 64 positions at a time (from the smallest candidate left),
+  // one ballot per live position gives the colours that position is live
+  // for, and each position is one discovery with all of them.
+  for (uint32_t t = wid; t < dtotal; t += NW) {
+    uint32_t lo = 0, hi = cl;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_dpre[mid] <= t) lo = mid + 1;
+      else hi = mid;
+    }
+    const uint32_t ent = uniform_u32(lo - 1);   // the last entry starting at or before t
+    const uint32_t k = t - s_dpre[ent];
+    const uint32_t g = s_g[ent];
+    const unsigned long long M = s_m[ent];
+    const uint32_t fl = s_fl[ent];
+    const uint32_t beta = (fl >> kBetaShift) & 31u;
+    const uint32_t B = 1u << beta;
+    const uint32_t deg = s_deg[ent];
+    const uint32_t T = s_T[ent];
+    const uint64_t q = s_q[ent], rq = s_rq[ent];
+    const bool uniform = (fl & kRowUniform) != 0;
+    const uint32_t e0 = s_beg[ent] + (k << beta);
+    const uint32_t len = min(deg - (k << beta), B);
+    Affine E{s_ja[ent], s_jb[ent]};
+    for (uint32_t kk = k, bit = beta + 1; kk; kk >>= 1, ++bit)
+      if (kk & 1) E = aff_then(E, s_p2[bit]);
+    const Affine Pst = s_p2[0];
+    const bool act = (M >> lane) & 1ull;
+    // lane state: next search position, stream element, current candidate
+    // cp (len: none left) and whether it is live
+    uint32_t pos = 0, cp = len;
+    bool clive = false;
+    uint64_t x = 0;
+    auto advance = [&]() {
+      const uint64_t gap = ic_gap_fast(x, q, rq, &S.tab);
+      if (gap >= len - pos) {
+        cp = len;
+        return;
+      }
+      pos += static_cast<uint32_t>(gap);
+      cp = pos;
+      const uint64_t a = Pst.a * x + Pst.b;      // acceptance element
+      ++st_cand;
+      if (!uniform) {
+        ++st_draws;
+        st_bytes += 4;
+      }
+      clive = uniform || ic_accept(a, T, P.val[e0 + cp]);
+      if (clive) ++st_live;
+      ++pos;
+      x = Pst.a * a + Pst.b;                     // next gap element
+      if (pos < len) ++st_draws;
+    };
+    if (act) {
+      x = E.a * P.z[64ull * g + lane] + E.b;
+      ++st_draws;
+      // a block without candidates costs one compare
+      if ((x >> 32) + 1 > (len == B ? s_uf[ent] : s_ul[ent])) advance();
+    }
+    for (;;) {
+      const uint32_t w0 = uniform_u32(wave_min_u32(cp));
+      if (w0 >= len) break;
+      unsigned long long L = 0;
+      while (cp < w0 + 64u && cp < len) {
+        if (clive) L |= 1ull << (cp - w0);
+        advance();
+      }
+      const unsigned long long U = uniform_u64(wave_or_u64(L));
+      const uint32_t npos = static_cast<uint32_t>(__popcll(U));
+      uint32_t mypos = 0;
+      unsigned long long mybits = 0;
+      for (uint32_t j = 0; j < npos; ++j) {
+        const uint32_t p = nth_bit(U, j);
+        const unsigned long long m = __ballot((L >> p) & 1ull);
+        if (lane == j) {
+          mypos = p;
+          mybits = m;
+        }
+      }
+      const bool lv = lane < npos;
+      const uint32_t u = lv ? P.ids[e0 + w0 + mypos] : 0u;
+      if (lv) st_bytes += 4;
+      grid_discover(P, lv, g, u, mybits);
+    }
+  }
+}
+
+// BFS level L: the frontier is buf[L % 2][0, n) with n = ring[L % kRing]
+// (its first cap entries when it overflowed, which raises the overflow
+// flag); every workgroup with a chunk to take pulls chunks of BT items from
+// the level's counter.  Levels are enqueued ahead of the host: a level whose
+// frontier is empty costs its launch only.
+template <int BT>
+__global__ void __launch_bounds__(BT) grid_level_kernel(GridParams P, uint32_t L) {
+  __shared__ ExpandLds<BT> S;
+  __shared__ unsigned long long s_chunk;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t raw = __hip_atomic_load(&P.ring[L % kRing], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t ncur = raw < P.cap ? raw : P.cap;
+  if (blockIdx.x == 0 && tid == 0 && ncur) {
+    P.gctl[kGLevels] += 1;
+    if (raw > P.cap) P.gctl[kGOver] = 1;
+  }
+  if (static_cast<uint64_t>(blockIdx.x) * BT >= ncur) return;   // no chunk for this workgroup
+  GridParams Q = P;
+  Q.cur = P.buf[L & 1u];
+  Q.ncur = ncur;
+  Q.next = P.buf[(L + 1) & 1u];
+  Q.next_cnt = &P.ring[(L + 1) % kRing];
+  unsigned long long *head = &P.ring[kRing + L % kRing];
+  if (tid == 0) fill_step_powers(P.step, S.p2);
+  ic_load_tables(&S.tab);
+  ExpandStats st;
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) s_chunk = atomicAdd(head, static_cast<unsigned long long>(BT));
+    __syncthreads();
+    const uint64_t c0 = s_chunk;
+    if (c0 >= ncur) break;
+    expand_chunk<BT>(Q, Q.cur, c0, static_cast<uint32_t>(min(static_cast<uint64_t>(BT), ncur - c0)),
+                     S, st);
+  }
+  report_stats(P.ctl, st.bytes, st.cand, st.draws, st.exp, st.live);
+}
+
+// Zeroes the frontier lengths of levels a .. a + k - 1 and the chunk
+// counters of levels a - 1 .. a + k - 2 (ring indices mod kRing), or, with
+// `all`, the whole ring and the control words.
+__global__ void grid_ring_kernel(unsigned long long *ring, unsigned long long *gctl, uint32_t a,
+                                 uint32_t k, uint32_t all) {
+  const uint32_t i = threadIdx.x;
+  if (all) {
+    if (i < 2 * kRing) ring[i] = 0;
+    if (i < kGWords) gctl[i] = 0;
+    return;
+  }
+  if (i < k) {
+    ring[(a + i) % kRing] = 0;
+    ring[kRing + (a + i - 1) % kRing] = 0;
+  }
 }
 
 // Frontier from the pend words (after an overflow): every (g, v) of groups
@@ -306,7 +500,7 @@ __global__ void grid_rebuild_kernel(GridParams P, uint32_t groups) {
     const uint32_t leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
     unsigned long long base = 0;
     if (lane_id() == leader)
-      base = atomicAdd(&P.gctl[kGNext], static_cast<unsigned long long>(__popcll(m)));
+      base = atomicAdd(P.next_cnt, static_cast<unsigned long long>(__popcll(m)));
     base = __shfl(base, static_cast<int>(leader), 64);
     if (push) {
       const unsigned long long at = base + __popcll(m & lanes_below());
@@ -316,10 +510,6 @@ __global__ void grid_rebuild_kernel(GridParams P, uint32_t groups) {
   }
 }
 
-__global__ void grid_reset_kernel(unsigned long long *gctl) {
-  gctl[kGHead] = 0;
-  gctl[kGNext] = 0;
-}
 
 // Union size of every group: the set bits of its bitmap.
 __global__ void __launch_bounds__(256) grid_union_kernel(const uint32_t *bm0, uint32_t nw0,
@@ -563,6 +753,7 @@ static void grid_ensure(rip_graph *g, uint32_t groups, uint64_t fcap, hipStream_
     s.fcap = fcap;
   }
   if (!s.d_gctl) {
+    s.d_ring = dev_alloc<unsigned long long>(2 * kRing);
     s.d_gctl = dev_alloc<unsigned long long>(kGWords);
     s.d_union = dev_alloc<unsigned long long>(256);
     RIP_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_ctl), sizeof(unsigned long long) * 256,
@@ -591,6 +782,7 @@ void grid_scratch_free(GridScratch &s) {
   dev_free(s.d_fb);
   dev_free(s.d_gctl);
   dev_free(s.d_union);
+  dev_free(s.d_ring);
   if (s.h_ctl) (void)hipHostFree(s.h_ctl);
   s.h_ctl = nullptr;
   s.groups = 0;
@@ -633,7 +825,10 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
   P.cap = lcap;
   P.gctl = s.d_gctl;
   P.ctl = p->d_ctl;
-  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(&grid_expand_kernel<kExpBT>),
+  P.ring = s.d_ring;
+  P.buf[0] = s.d_fa;
+  P.buf[1] = s.d_fb;
+  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(&grid_level_kernel<kExpBT>),
                                    kExpBT);
   const uint32_t exp_grid = static_cast<uint32_t>(per_cu * cu_count());
   uint32_t launches = 0;
@@ -641,44 +836,56 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
     const uint32_t Gb = static_cast<uint32_t>(std::min<uint64_t>(G, (count - b0 + 63) / 64));
     hipLaunchKernelGGL(grid_samples_kernel, dim3((64 * Gb + 255) / 256), dim3(256), 0, st,
                        p->seed, first, ids, b0, count, Gb, s.d_z, s.d_sid, s.d_nb);
-    hipLaunchKernelGGL(grid_reset_kernel, dim3(1), dim3(1), 0, st, s.d_gctl);
+    hipLaunchKernelGGL(grid_ring_kernel, dim3(1), dim3(2 * kRing), 0, st, s.d_ring, s.d_gctl, 0u,
+                       0u, 1u);
+    // level 0: the roots, in buf[0]
     P.next = s.d_fa;
+    P.next_cnt = &s.d_ring[0];
     hipLaunchKernelGGL(grid_seed_kernel, dim3(Gb), dim3(64), 0, st, P,
                        static_cast<const uint32_t *>(s.d_nb));
     launches += 3;
-    unsigned long long *cur = s.d_fa, *nxt = s.d_fb;
-    uint64_t pending = grid_read(s, s.d_gctl + kGNext, 1, st)[0];
-    bool dirty = false;  // some vertices with pend bits are in no list
+    uint32_t L = 0;            // the level whose frontier is ring[L % kRing]
+    bool dirty = false;        // some vertices with pend bits are in no list
+    const uint32_t levels0 = p->grid_levels;
     for (;;) {
-      if (pending > lcap) dirty = true;
-      uint64_t ncur = std::min<uint64_t>(pending, lcap);
-      if (ncur == 0) {
-        if (!dirty) break;
-        // the vertices the overflowing lists could not hold
-        P.next = cur;
-        hipLaunchKernelGGL(grid_reset_kernel, dim3(1), dim3(1), 0, st, s.d_gctl);
-        hipLaunchKernelGGL(grid_rebuild_kernel, dim3(2048), dim3(256), 0, st, P, Gb);
-        launches += 2;
-        pending = grid_read(s, s.d_gctl + kGNext, 1, st)[0];
-        dirty = pending > lcap;
-        ncur = std::min<uint64_t>(pending, lcap);
-        if (ncur == 0) break;
-      }
-      P.cur = cur;
-      P.ncur = ncur;
-      P.next = nxt;
-      hipLaunchKernelGGL(grid_reset_kernel, dim3(1), dim3(1), 0, st, s.d_gctl);
+      // kLevelsPerSync levels, then one look at the frontier length
+      hipLaunchKernelGGL(grid_ring_kernel, dim3(1), dim3(kLevelsPerSync), 0, st, s.d_ring,
+                         s.d_gctl, L + 1, kLevelsPerSync, 0u);
       RIP_HIP(hipEventRecord(p->ev3, st));
-      hipLaunchKernelGGL((grid_expand_kernel<kExpBT>), dim3(exp_grid), dim3(kExpBT), 0, st, P);
+      for (uint32_t i = 0; i < kLevelsPerSync; ++i)
+        hipLaunchKernelGGL((grid_level_kernel<kExpBT>), dim3(exp_grid), dim3(kExpBT), 0, st, P,
+                           L + i);
       RIP_HIP(hipGetLastError());
       RIP_HIP(hipEventRecord(p->ev4, st));
-      launches += 2;
-      pending = grid_read(s, s.d_gctl + kGNext, 1, st)[0];
+      launches += 1 + kLevelsPerSync;
+      L += kLevelsPerSync;
+      const unsigned long long *h = grid_read(s, s.d_gctl, kGWords, st);
+      const bool over = h[kGOver] != 0;
+      const uint32_t lv = static_cast<uint32_t>(h[kGLevels]);
+      uint64_t pending = 0;
+      RIP_HIP(hipMemcpyAsync(s.h_ctl + kGWords, s.d_ring + L % kRing, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, st));
+      RIP_HIP(hipStreamSynchronize(st));
+      pending = s.h_ctl[kGWords];
       float ms = 0.0f;
       RIP_HIP(hipEventElapsedTime(&ms, p->ev3, p->ev4));
       p->grid_ms += ms;
-      ++p->grid_levels;
-      std::swap(cur, nxt);
+      p->grid_levels = levels0 + lv;
+      if (over) dirty = true;
+      if (pending > lcap) dirty = true;
+      if (pending != 0) continue;
+      if (!dirty) break;
+      // the vertices the overflowing lists could not hold, as level L
+      P.next = P.buf[L & 1u];
+      P.next_cnt = &s.d_ring[L % kRing];
+      hipLaunchKernelGGL(grid_rebuild_kernel, dim3(2048), dim3(256), 0, st, P, Gb);
+      launches += 1;
+      RIP_HIP(hipMemcpyAsync(s.h_ctl + kGWords, s.d_ring + L % kRing, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, st));
+      RIP_HIP(hipStreamSynchronize(st));
+      pending = s.h_ctl[kGWords];
+      dirty = pending > lcap;
+      if (pending == 0) break;
     }
     // output: groups whose unions fit the staging together share a launch
     hipLaunchKernelGGL(grid_union_kernel, dim3(Gb), dim3(256), 0, st, s.d_bm0, P.nw0, s.d_union);

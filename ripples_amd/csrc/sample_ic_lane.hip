@@ -107,7 +107,9 @@ enum : uint32_t { kIdle = 0, kPop = 1, kRow = 2, kDone = 3 };
 template <int BT>
 __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
   __shared__ Affine s_p2[64];
+  __shared__ IcLogTables s_tab;
   if (threadIdx.x == 0) fill_step_powers(P.step, s_p2);
+  ic_load_tables(&s_tab);
   __syncthreads();
   const uint32_t lane = lane_id();
   const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (BT / 64) + (threadIdx.x >> 6);
@@ -210,7 +212,7 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
           state = kPop;
         }
       } else {
-        const uint64_t gap = ic_gap_fast(x, q, rq);
+        const uint64_t gap = ic_gap_fast(x, q, rq, &s_tab);
         if (gap >= len - pos) {
           pos = len;
         } else {

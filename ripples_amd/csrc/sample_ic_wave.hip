@@ -85,7 +85,8 @@ __device__ inline bool table_insert(unsigned long long *table, uint32_t mask,
 template <typename Visit>
 __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
                                   const uint32_t *val, uint32_t v, uint64_t z,
-                                  const Affine *p2, uint32_t *list, uint32_t &tail, uint32_t cap,
+                                  const Affine *p2, const IcLogTables *tab, uint32_t *list,
+                                  uint32_t &tail, uint32_t cap,
                                   unsigned long long &st_cand, unsigned long long &st_draws,
                                   unsigned long long &st_bytes, unsigned long long &st_live,
                                   Visit &&visit) {
@@ -131,7 +132,7 @@ __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
       if (pos < len) {
         if (!first) ++st_draws;
         first = false;
-        const uint64_t gap = ic_gap_fast(x, q, rq);
+        const uint64_t gap = ic_gap_fast(x, q, rq, tab);
         if (gap >= len - pos) {
           pos = len;
         } else {
@@ -183,7 +184,9 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
   // of a chunk that cannot take the next set stays a gap in the pool's data
   unsigned long long out_cur = 0, out_end = 0;
   __shared__ Affine s_p2[64];                 // P^(2^j): jumps between gap blocks
+  __shared__ IcLogTables s_tab;
   if (threadIdx.x == 0) fill_step_powers(P.step, s_p2);
+  ic_load_tables(&s_tab);
   __syncthreads();
   unsigned long long st_cand = 0, st_draws = 0, st_exp = 0, st_bytes = 0, st_live = 0;
 
@@ -208,7 +211,7 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       const uint32_t v = uniform_u32(list[head++]);
       ++st_exp;
-      if (!expand_row(P.vrec, P.ids, P.val, v, z, s_p2, list, tail, P.cap, st_cand, st_draws,
+      if (!expand_row(P.vrec, P.ids, P.val, v, z, s_p2, &s_tab, list, tail, P.cap, st_cand, st_draws,
                       st_bytes, st_live,
                       [&](uint32_t src) { return table_insert(table, tmask, tag, src); }))
         big = true;
@@ -302,7 +305,9 @@ __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
   uint32_t *sum = P.sum + wave * P.swords;
   uint32_t *list = P.list + wave * P.n;
   __shared__ Affine s_p2[64];
+  __shared__ IcLogTables s_tab;
   if (threadIdx.x == 0) fill_step_powers(P.step, s_p2);
+  ic_load_tables(&s_tab);
   __syncthreads();
   unsigned long long st_cand = 0, st_draws = 0, st_exp = 0, st_bytes = 0, st_live = 0;
   auto visit = [&](uint32_t u) {
@@ -331,7 +336,7 @@ __global__ void __launch_bounds__(BT) ic_big_kernel(BigParams P) {
       // every vertex enters the list once, so it never passes n entries; the
       // cap test of expand_row counts live edges, not new vertices, so it
       // gets no bound here (with cap n it could stop a row near the end)
-      expand_row(P.vrec, P.ids, P.val, v, z, s_p2, list, tail, 0xFFFFFFFFu, st_cand, st_draws,
+      expand_row(P.vrec, P.ids, P.val, v, z, s_p2, &s_tab, list, tail, 0xFFFFFFFFu, st_cand, st_draws,
                  st_bytes, st_live, visit);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");

@@ -28,6 +28,11 @@ int main(int argc, char **argv) {
   const unsigned seed = argc > 1 ? std::atoi(argv[1]) : 1;
   const int rows = argc > 2 ? std::atoi(argv[2]) : 1000;
   std::mt19937_64 rng(seed);
+  IcLogTables tab;
+  for (int i = 0; i < 256; ++i) {
+    tab.log2c[i] = kLog2C[i];
+    tab.recipc[i] = kRecipC[i];
+  }
   for (int r = 0; r < rows; ++r) {
     uint32_t T;
     switch (r % 6) {
@@ -49,7 +54,7 @@ int main(int argc, char **argv) {
         default: U = 1 + (rng() % (1ull << 32)); break;
       }
       const uint64_t x = x_of(U, rng());
-      const uint64_t g0 = ic_gap(x, q), g1 = ic_gap_fast(x, q, rq);
+      const uint64_t g0 = ic_gap(x, q), g1 = ic_gap_fast(x, q, rq, &tab);
       if (g0 != g1) fail("gap", T, g0, g1);
     }
     const uint32_t beta = ic_block_log2(T);
