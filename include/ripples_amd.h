@@ -160,6 +160,18 @@ int rip_pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune);
  * (streaming_find_most_influential.h:139-244). */
 int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
                          const uint32_t *vertices);
+/* Autotuner: draws probe_sets samples with each candidate setting (IC: the
+ * lane list capacity 64/256/1024 x the lane kernel's workgroup size
+ * 256/512/1024; LT: the walk's output chunk) and keeps the fastest, which
+ * the graph then uses for every sampling call given no tune (rip_pool_sample,
+ * rip_imm, the server).  The choice is remembered per (model, degree skew =
+ * round(log2(max in-degree / mean)), round(log2 n)); a later call on a graph
+ * of the same key takes it without probing unless force is set.  *best_ms:
+ * the winner's time per probe batch (0 when taken from the cache); *tried:
+ * settings timed.  No counterpart call in the reference, whose GPU batch
+ * size comes from a similar probe (streaming_rrr_generator.h:1150-1241). */
+int rip_autotune(rip_graph *g, uint64_t probe_sets, int force, rip_tune *best,
+                 double *best_ms, uint32_t *tried);
 /* Forget all sets (keeps the allocations). */
 int rip_pool_reset(rip_pool *p);
 int rip_pool_size(const rip_pool *p, uint64_t *num_sets,

@@ -136,6 +136,8 @@ def lib():
     L.rip_pool_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double),
                                           C.POINTER(C.c_uint32)]
     L.rip_pool_last_call_ms.argtypes = [vp, C.POINTER(C.c_double)]
+    L.rip_autotune.argtypes = [vp, C.c_uint64, C.c_int, C.POINTER(Tune), C.POINTER(C.c_double),
+                               C.POINTER(C.c_uint32)]
     L.rip_pool_last_grid.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), _u64p,
                                      C.c_uint32]
     L.rip_pool_columns.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
@@ -327,6 +329,16 @@ class Graph:
         _check(lib().rip_spread(self._h, seeds if len(seeds) else np.zeros(1, np.uint32),
                                 len(seeds), seed, first, replicas, act, lev))
         return act[:replicas], lev[:replicas]
+
+    def autotune(self, probe_sets=16384, force=False):
+        """rip_autotune: the fastest sampler setting on a probe batch; the
+        graph uses it from then on.  Returns (setting dict, ms per probe
+        batch, settings tried)."""
+        t = Tune()
+        ms, tried = C.c_double(0), C.c_uint32(0)
+        _check(lib().rip_autotune(self._h, probe_sets, 1 if force else 0, C.byref(t),
+                                  C.byref(ms), C.byref(tried)))
+        return {f: getattr(t, f) for f, _ in Tune._fields_ if getattr(t, f)}, ms.value, tried.value
 
     def pool(self, seed=0):
         return Pool(self, seed)

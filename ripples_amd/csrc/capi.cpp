@@ -133,6 +133,14 @@ int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
   });
 }
 
+int rip_autotune(rip_graph *g, uint64_t probe_sets, int force, rip_tune *best,
+                 double *best_ms, uint32_t *tried) {
+  return guarded([&] {
+    RIP_REQUIRE(g && best, RIP_EINVAL, "graph or best is NULL");
+    rip::autotune(g, probe_sets, force != 0, best, best_ms, tried);
+  });
+}
+
 int rip_pool_reset(rip_pool *p) {
   return guarded([&] {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");

@@ -77,6 +77,8 @@ struct GridScratch {
   unsigned long long *d_fa = nullptr, *d_fb = nullptr;
   unsigned long long *d_gctl = nullptr, *d_union = nullptr;
   unsigned long long *d_ring = nullptr;  // per-level frontier lengths and chunk counters
+  unsigned long long *d_out = nullptr;   // output tables of a sub-batch
+  uint64_t out_words = 0;
   unsigned long long *h_ctl = nullptr;
 };
 
@@ -177,6 +179,10 @@ struct rip_graph {
   uint32_t hot_n = 0;
   uint32_t *d_hot_index = nullptr;   // n: index among the hot vertices, or kNotHot
   uint32_t *d_hot_vertex = nullptr;  // hot_n: the hot vertices, ascending
+  // Sampler settings chosen by the autotuner (tune.cpp), used by sampling
+  // calls that pass none.
+  rip_tune tuned{};
+  bool tuned_set = false;
   // Sampler scratch shared by every pool of this graph; pool_sample holds
   // sampler_mu while a launch uses it.
   std::mutex sampler_mu;
@@ -371,6 +377,12 @@ uint64_t coverage_run_noalloc(rip_pool *p, uint64_t num_sets, const uint32_t *se
 // scan.hip: exclusive scan of uint32 values into uint64 prefixes (n + 1 out)
 void exclusive_scan_u32(const uint32_t *d_in, unsigned long long *d_out,
                         uint64_t n, hipStream_t stream);
+
+// tune.cpp: the candidate settings that draw `probe` samples fastest on g
+// (or, unless force, those found before for a graph of the same model and
+// degree skew); g then uses them when a sampling call passes none.
+void autotune(rip_graph *g, uint64_t probe, bool force, rip_tune *best, double *best_ms,
+              uint32_t *tried);
 
 // imm.cpp
 void imm_run(rip_graph *g, uint32_t k, double eps, uint64_t seed,

@@ -449,6 +449,7 @@ void pool_read_ctl(rip_pool *p, bool main) {
 
 void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
   if (count == 0) return;
+  if (!tune && p->g->tuned_set) tune = &p->g->tuned;   // the autotuner's choice
   RIP_HIP(hipSetDevice(p->g->device));
   RIP_REQUIRE(p->num_sets + count < (1ull << 32), -1,
               "pool: at most 2^32-1 samples per stream");

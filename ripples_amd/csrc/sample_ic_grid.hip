@@ -44,10 +44,11 @@ namespace rip {
 namespace {
 
 constexpr int kExpBT = 256;          // expand kernel
-constexpr int kOutBT = 1024;         // output kernel, one workgroup per group
 // rows pending for this many colours or more are expanded a block per wave,
 // lane c drawing colour c, so a live edge is one discovery for all colours
 constexpr uint32_t kDenseColours = 8;
+constexpr uint32_t kDenseSpan = 128;   // largest block of the dense path (edges)
+constexpr uint32_t kDenseSlots = 16;   // dense tasks per round of a workgroup
 // level counters kept on the device (a ring), and levels enqueued per host
 // synchronisation: a BFS of hundreds of narrow levels costs a launch per
 // level, not a round trip
@@ -139,11 +140,6 @@ __device__ inline void grid_discover(const GridParams &P, bool live, uint32_t g,
   }
 }
 
-__device__ inline uint32_t wave_min_u32(uint32_t v) {
-  for (int d = 32; d > 0; d >>= 1) v = min(v, static_cast<uint32_t>(__shfl_xor(v, d, 64)));
-  return v;
-}
-
 __device__ inline unsigned long long wave_or_u64(unsigned long long v) {
   for (int d = 32; d > 0; d >>= 1) v |= __shfl_xor(v, d, 64);
   return v;
@@ -169,6 +165,11 @@ struct ExpandLds {
   uint32_t g[BT], beg[BT], deg[BT], T[BT], fl[BT], uf[BT], ul[BT], pre[BT], nch[BT], dpre[BT];
   unsigned long long q[BT], rq[BT], ja[BT], jb[BT], m[BT];
   uint32_t ucur;   // next unit of the chunk not claimed by a wave
+  // the dense round's tasks: entry, block, first element, prefix of colours,
+  // and per task and position the colours live there
+  uint32_t tent[kDenseSlots], tk[kDenseSlots], tpre[kDenseSlots];
+  Affine tE[kDenseSlots];
+  unsigned long long pos[kDenseSlots * kDenseSpan];
 };
 
 struct ExpandStats {
@@ -182,7 +183,6 @@ struct ExpandStats {
 template <int BT>
 __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur, uint64_t c0,
                              uint32_t cl, ExpandLds<BT> &S, ExpandStats &st) {
-  constexpr int NW = BT / 64;
   const uint32_t tid = threadIdx.x;
   __syncthreads();   // the previous chunk's waves are done with the tables
   Affine *s_p2 = S.p2;
@@ -217,8 +217,8 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
       if (r.thr) {
         const uint32_t beta = (r.flags >> kBetaShift) & 31u;
         const uint32_t nblk = ((r.end - r.begin - 1) >> beta) + 1;
-        if (__popcll(M) >= kDenseColours) {
-          ntask = nblk;   // one wave per block, a lane per colour
+        if (__popcll(M) >= kDenseColours && (1u << beta) <= kDenseSpan) {
+          ntask = nblk;   // a round task per block, a lane per (block, colour)
         } else {
           nch = nblk;   // a unit per (colour, block)
           units = static_cast<uint32_t>(__popcll(M)) * nch;
@@ -242,7 +242,7 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
   // unit makes one draw (the block's first element and its first gap, or the
   // next gap) and tests its candidate, so the lanes stay busy whatever the
   // lengths of their units; a live edge is discovered in the same step.
-  const uint32_t lane = lane_id(), wid = tid >> 6;
+  const uint32_t lane = lane_id();
   {
     const Affine Pst = s_p2[0];
     uint32_t nxt = 0, hi = 0;   // the wave's claimed units [nxt, hi) (wave-uniform)
@@ -341,92 +341,107 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
       grid_discover(P, live, g, src, bit);
     }
   }
-  // Rows pending for many colours: wave task t is block k of entry ent.
-  // Lane c walks colour c's candidates of the block; the wave takes them a
-  // window of 64 positions at a time (from the smallest candidate left),
-  // one ballot per live position gives the colours that position is live
-  // for, and each position is one discovery with all of them.
-  for (uint32_t t = wid; t < dtotal; t += NW) {
-    uint32_t lo = 0, hi = cl;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (s_dpre[mid] <= t) lo = mid + 1;
-      else hi = mid;
+  // Rows pending for many colours (blocks of at most kDenseSpan edges): task
+  // t is block k of entry ent, kDenseSlots tasks at a time.  Phase 1: a lane
+  // per (task, colour) walks the colour's candidates of the block and ORs the
+  // colour's bit into the LDS mask of each live position; phase 2: a lane per
+  // position makes one discovery with all the colours live there.  The
+  // atomics of a phase-2 step are independent of each other (many in flight
+  // per wave), and colours reaching one edge together cost one.
+  for (uint32_t r0 = 0; r0 < dtotal; r0 += kDenseSlots) {
+    const uint32_t nt = min(kDenseSlots, dtotal - r0);
+    // phase 0: the round's tasks, their colours and first elements
+    uint32_t tu = 0;
+    if (tid < nt) {
+      const uint32_t t = r0 + tid;
+      uint32_t lo = 0, hi = cl;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_dpre[mid] <= t) lo = mid + 1;
+        else hi = mid;
+      }
+      const uint32_t ent = lo - 1;   // the last entry starting at or before t
+      const uint32_t k = t - s_dpre[ent];
+      const uint32_t beta = (s_fl[ent] >> kBetaShift) & 31u;
+      Affine E{s_ja[ent], s_jb[ent]};
+      for (uint32_t kk = k, b2 = beta + 1; kk; kk >>= 1, ++b2)
+        if (kk & 1) E = aff_then(E, s_p2[b2]);
+      S.tent[tid] = ent;
+      S.tk[tid] = k;
+      S.tE[tid] = E;
+      tu = static_cast<uint32_t>(__popcll(s_m[ent]));
     }
-    const uint32_t ent = uniform_u32(lo - 1);   // the last entry starting at or before t
-    const uint32_t k = t - s_dpre[ent];
-    const uint32_t g = s_g[ent];
-    const unsigned long long M = s_m[ent];
-    const uint32_t fl = s_fl[ent];
-    const uint32_t beta = (fl >> kBetaShift) & 31u;
-    const uint32_t B = 1u << beta;
-    const uint32_t deg = s_deg[ent];
-    const uint32_t T = s_T[ent];
-    const uint64_t q = s_q[ent], rq = s_rq[ent];
-    const bool uniform = (fl & kRowUniform) != 0;
-    const uint32_t e0 = s_beg[ent] + (k << beta);
-    const uint32_t len = min(deg - (k << beta), B);
-    Affine E{s_ja[ent], s_jb[ent]};
-    for (uint32_t kk = k, bit = beta + 1; kk; kk >>= 1, ++bit)
-      if (kk & 1) E = aff_then(E, s_p2[bit]);
-    const Affine Pst = s_p2[0];
-    const bool act = (M >> lane) & 1ull;
-    // lane state: next search position, stream element, current candidate
-    // cp (len: none left) and whether it is live
-    uint32_t pos = 0, cp = len;
-    bool clive = false;
-    uint64_t x = 0;
-    auto advance = [&]() {
-      const uint64_t gap = ic_gap_fast(x, q, rq, &S.tab);
-      if (gap >= len - pos) {
-        cp = len;
-        return;
+    uint32_t units;
+    const uint32_t tpre = block_exclusive_scan<BT>(tu, s_tmp, &units);
+    if (tid < nt) S.tpre[tid] = tpre;
+    __syncthreads();
+    // phase 1
+    for (uint32_t u = tid; u < units; u += BT) {
+      uint32_t lo = 0, hi = nt;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (S.tpre[mid] <= u) lo = mid + 1;
+        else hi = mid;
       }
-      pos += static_cast<uint32_t>(gap);
-      cp = pos;
-      const uint64_t a = Pst.a * x + Pst.b;      // acceptance element
-      ++st_cand;
-      if (!uniform) {
-        ++st_draws;
-        st_bytes += 4;
-      }
-      clive = uniform || ic_accept(a, T, P.val[e0 + cp]);
-      if (clive) ++st_live;
-      ++pos;
-      x = Pst.a * a + Pst.b;                     // next gap element
-      if (pos < len) ++st_draws;
-    };
-    if (act) {
-      x = E.a * P.z[64ull * g + lane] + E.b;
+      const uint32_t t = lo - 1;
+      const uint32_t ent = S.tent[t], k = S.tk[t];
+      const uint32_t c = nth_bit(s_m[ent], u - S.tpre[t]);
+      const uint32_t g = s_g[ent];
+      const uint32_t fl = s_fl[ent];
+      const uint32_t beta = (fl >> kBetaShift) & 31u;
+      const uint32_t B = 1u << beta;
+      const uint32_t len = min(s_deg[ent] - (k << beta), B);
+      const uint32_t e0 = s_beg[ent] + (k << beta);
+      const uint32_t T = s_T[ent];
+      const uint64_t q = s_q[ent], rq = s_rq[ent];
+      const bool uniform = (fl & kRowUniform) != 0;
+      const Affine Pst = s_p2[0];
+      const Affine E = S.tE[t];
+      uint64_t x = E.a * P.z[64ull * g + c] + E.b;
       ++st_draws;
       // a block without candidates costs one compare
-      if ((x >> 32) + 1 > (len == B ? s_uf[ent] : s_ul[ent])) advance();
-    }
-    for (;;) {
-      const uint32_t w0 = uniform_u32(wave_min_u32(cp));
-      if (w0 >= len) break;
-      unsigned long long L = 0;
-      while (cp < w0 + 64u && cp < len) {
-        if (clive) L |= 1ull << (cp - w0);
-        advance();
+      if ((x >> 32) + 1 <= (len == B ? s_uf[ent] : s_ul[ent])) continue;
+      unsigned long long *pm = S.pos + t * kDenseSpan;
+      for (uint32_t pos = 0;;) {
+        const uint64_t gap = ic_gap_fast(x, q, rq, &S.tab);
+        if (gap >= len - pos) break;
+        pos += static_cast<uint32_t>(gap);
+        const uint64_t a = Pst.a * x + Pst.b;        // acceptance element
+        ++st_cand;
+        if (!uniform) {
+          ++st_draws;
+          st_bytes += 4;
+        }
+        if (uniform || ic_accept(a, T, P.val[e0 + pos])) {
+          atomicOr(&pm[pos], 1ull << c);
+          ++st_live;
+        }
+        ++pos;
+        if (pos >= len) break;
+        x = Pst.a * a + Pst.b;                       // next gap element
+        ++st_draws;
       }
-      const unsigned long long U = uniform_u64(wave_or_u64(L));
-      const uint32_t npos = static_cast<uint32_t>(__popcll(U));
-      uint32_t mypos = 0;
-      unsigned long long mybits = 0;
-      for (uint32_t j = 0; j < npos; ++j) {
-        const uint32_t p = nth_bit(U, j);
-        const unsigned long long m = __ballot((L >> p) & 1ull);
-        if (lane == j) {
-          mypos = p;
-          mybits = m;
+    }
+    __syncthreads();
+    // phase 2
+    for (uint32_t p0 = 0; p0 < kDenseSlots * kDenseSpan; p0 += BT) {
+      const uint32_t pi = p0 + tid;
+      const uint32_t t = pi / kDenseSpan, pos = pi % kDenseSpan;
+      unsigned long long m = 0;
+      uint32_t g = 0, u = 0;
+      if (t < nt) {
+        m = S.pos[pi];
+        if (m) {
+          S.pos[pi] = 0;
+          const uint32_t ent = S.tent[t];
+          g = s_g[ent];
+          u = P.ids[s_beg[ent] + (S.tk[t] << ((s_fl[ent] >> kBetaShift) & 31u)) + pos];
+          st_bytes += 4;
         }
       }
-      const bool lv = lane < npos;
-      const uint32_t u = lv ? P.ids[e0 + w0 + mypos] : 0u;
-      if (lv) st_bytes += 4;
-      grid_discover(P, lv, g, u, mybits);
+      grid_discover(P, m != 0, g, u, m);
     }
+    __syncthreads();
   }
 }
 
@@ -456,6 +471,7 @@ __global__ void __launch_bounds__(BT) grid_level_kernel(GridParams P, uint32_t L
   unsigned long long *head = &P.ring[kRing + L % kRing];
   if (tid == 0) fill_step_powers(P.step, S.p2);
   ic_load_tables(&S.tab);
+  for (uint32_t i = tid; i < kDenseSlots * kDenseSpan; i += BT) S.pos[i] = 0;  // phase 2 keeps it zero
   ExpandStats st;
   for (;;) {
     __syncthreads();
@@ -525,14 +541,40 @@ __global__ void __launch_bounds__(256) grid_union_kernel(const uint32_t *bm0, ui
   if (threadIdx.x == 0) out[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
+
+// ------------------------------------------------------------------ output
+// After a batch's last level the groups' sets are written in four steps,
+// each spread over many workgroups per group:
+//   1. union lists: the group's bitmap is cut into kRanges ranges of summary
+//      words; one kernel counts each range's members, the next writes them in
+//      vertex order into the group's staging (list + colour masks), clearing
+//      bitmaps and vis on the way;
+//   2. per chunk of kChunk staging entries (one wave each): members of each
+//      colour, all and hot ones;
+//   3. per group: the column decision, each colour's offsets over the chunks,
+//      one reservation of pool space, the sets' offsets and lengths;
+//   4. per chunk: hot members into the group's column (when it has one), the
+//      rest scattered to their colours' sorted lists.
+constexpr uint32_t kRanges = 64;
+constexpr uint32_t kChunk = 2048;   // staging entries per wave of steps 2 and 4
+
 struct OutParams {
   uint32_t n, nw0, nw1;
   unsigned long long *vis;
   uint32_t *bm0, *bm1;
-  uint32_t g0;                          // first group of this launch
-  const unsigned long long *soff;       // per group of the launch: staging offset
+  uint32_t g0;                          // first group of this sub-batch
+  uint32_t groups;                      // groups of the sub-batch
+  const unsigned long long *soff;       // per group of the sub-batch: staging offset
   uint32_t *slist;                      // staging: union lists
   unsigned long long *smask;            // staging: union masks
+  unsigned long long *rc;               // groups x kRanges: range member counts
+  const unsigned long long *cbase;      // groups + 1: first chunk of each group
+  const unsigned long long *usz;        // groups: union sizes
+  uint32_t *cc;                         // chunks x 64: members per colour (then offsets)
+  uint32_t *cch;                        // chunks x 64: hot members per colour
+  unsigned long long *dbase;            // groups x 64: pool offset of each colour's list
+  uint32_t *gcol;                       // groups: column of the group, or kNoGroup
+  uint32_t *gfail;                      // groups: pool space ran out (nothing is written)
   const uint32_t *sid;                  // groups x 64 sample ids (kNoGroup: none)
   const uint32_t *nb;                   // colours per group
   unsigned long long *set_off;
@@ -549,37 +591,74 @@ struct OutParams {
   uint32_t group_base;
 };
 
+// Summary words [lo, hi) of range r of a group with nw1 of them.
+__device__ inline void range_of(uint32_t nw1, uint32_t r, uint32_t *lo, uint32_t *hi) {
+  *lo = static_cast<uint32_t>(static_cast<uint64_t>(nw1) * r / kRanges);
+  *hi = static_cast<uint32_t>(static_cast<uint64_t>(nw1) * (r + 1) / kRanges);
+}
+
+// Step 1a: members of each (group, range).
 template <int BT>
-__global__ void __launch_bounds__(BT) grid_output_kernel(OutParams P) {
-  constexpr int NW = BT / 64;
-  __shared__ uint32_t s_tmp[NW];
-  __shared__ unsigned long long s_hot[NW];
-  __shared__ uint32_t s_cnt[64], s_cofs[64];
-  __shared__ unsigned long long s_base;
-  __shared__ uint32_t s_fail, s_col;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
-  const uint32_t g = P.g0 + blockIdx.x;
-  const uint32_t nb = P.nb[g];
+__global__ void __launch_bounds__(BT) out_range_count_kernel(OutParams P) {
+  __shared__ unsigned long long s_w[BT / 64];
+  const uint32_t i = blockIdx.x / kRanges, r = blockIdx.x % kRanges;
+  const uint32_t g = P.g0 + i;
+  const uint32_t *bm0 = P.bm0 + static_cast<uint64_t>(g) * P.nw0;
+  const uint32_t *bm1 = P.bm1 + static_cast<uint64_t>(g) * P.nw1;
+  uint32_t lo, hi;
+  range_of(P.nw1, r, &lo, &hi);
+  unsigned long long cnt = 0;
+  for (uint32_t w1i = lo + threadIdx.x; w1i < hi; w1i += BT) {
+    const uint32_t w1 = __hip_atomic_load(&bm1[w1i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t b = w1; b; b &= b - 1)
+      cnt += __popc(__hip_atomic_load(&bm0[w1i * 32 + (__ffs(b) - 1)], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT));
+  }
+  for (int d = 32; d > 0; d >>= 1) cnt += __shfl_down(cnt, d, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < BT / 64; ++w) t += s_w[w];
+    P.rc[blockIdx.x] = t;
+  }
+}
+
+// Step 1b: the members of each (group, range) into the staging in vertex
+// order; bitmaps and vis read and cleared with atomics (they were written by
+// atomics, which complete in L2).
+template <int BT>
+__global__ void __launch_bounds__(BT) out_range_write_kernel(OutParams P) {
+  __shared__ uint32_t s_tmp[BT / 64];
+  __shared__ unsigned long long s_pre;
+  const uint32_t i = blockIdx.x / kRanges, r = blockIdx.x % kRanges;
+  const uint32_t g = P.g0 + i;
+  const uint32_t tid = threadIdx.x;
   uint32_t *bm0 = P.bm0 + static_cast<uint64_t>(g) * P.nw0;
   uint32_t *bm1 = P.bm1 + static_cast<uint64_t>(g) * P.nw1;
   unsigned long long *vis = P.vis + static_cast<uint64_t>(g) * P.n;
-  uint32_t *list = P.slist + P.soff[blockIdx.x];
-  unsigned long long *masks = P.smask + P.soff[blockIdx.x];
-  unsigned long long st_bytes = 0;
-  if (tid < 64) s_cnt[tid] = 0;
-  // the union in vertex order; bitmaps and vis read and cleared with atomics
-  uint32_t running = 0;
-  for (uint32_t t0 = 0; t0 < P.nw1; t0 += BT) {
+  if (tid == 0) {
+    unsigned long long t = 0;
+    for (uint32_t q = 0; q < r; ++q) t += P.rc[i * kRanges + q];
+    s_pre = t;
+  }
+  __syncthreads();
+  uint32_t *list = P.slist + P.soff[i];
+  unsigned long long *masks = P.smask + P.soff[i];
+  uint32_t lo, hi;
+  range_of(P.nw1, r, &lo, &hi);
+  uint64_t running = s_pre;
+  for (uint32_t t0 = lo; t0 < hi; t0 += BT) {
     const uint32_t w1i = t0 + tid;
-    const uint32_t w1 = w1i < P.nw1 ? __hip_atomic_load(&bm1[w1i], __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT)
-                                    : 0u;
+    const uint32_t w1 = w1i < hi ? __hip_atomic_load(&bm1[w1i], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)
+                                 : 0u;
     uint32_t cnt = 0;
     for (uint32_t b = w1; b; b &= b - 1)
       cnt += __popc(__hip_atomic_load(&bm0[w1i * 32 + (__ffs(b) - 1)], __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT));
     uint32_t tot;
-    uint32_t p = running + block_exclusive_scan<BT>(cnt, s_tmp, &tot);
+    uint64_t p = running + block_exclusive_scan<BT>(cnt, s_tmp, &tot);
     for (uint32_t b = w1; b; b &= b - 1) {
       const uint32_t w0i = w1i * 32 + (__ffs(b) - 1);
       const uint32_t w0 = atomicExch(&bm0[w0i], 0u);
@@ -593,86 +672,160 @@ __global__ void __launch_bounds__(BT) grid_output_kernel(OutParams P) {
     if (w1) atomicExch(&bm1[w1i], 0u);
     running += tot;
   }
-  __syncthreads();
-  const uint32_t U = running;
-  if (P.colptr) {
-    // a column costs 8 bytes per hot vertex, lists 4 per member: the group's
-    // hot members decide
-    unsigned long long hot = 0;
-    for (uint32_t i = tid; i < U; i += BT)
-      if (P.hot_index[list[i]] != kNotHot) hot += __popcll(masks[i]);
-    unsigned long long tot_hot;
-    (void)block_exclusive_scan_u64<BT>(hot, s_hot, &tot_hot);
-    if (tid == 0) {
-      s_col = kNoGroup;
-      if (tot_hot > 2ull * P.hot_n)
-        s_col = P.group_base + static_cast<uint32_t>(atomicAdd(&P.ctl[kCtlGroupCounter2], 1ull));
-    }
-    __syncthreads();
-    const uint32_t colg = s_col;
-    if (colg != kNoGroup) {
-      unsigned long long *col = P.colptr[colg];
-      for (uint32_t h = tid; h < P.hot_n; h += BT) col[h] = 0ull;
-      __syncthreads();
-      for (uint32_t i = tid; i < U; i += BT) {
-        const uint32_t h = P.hot_index[list[i]];
-        if (h != kNotHot) {
-          col[h] = masks[i];
-          masks[i] = 0ull;
-        }
-      }
-      st_bytes += 8ull * ((P.hot_n + BT - 1 - tid) / BT);
-      if (tid == 0) atomicAdd(&P.ctl[kCtlElems], tot_hot);
-      if (tid < 64) {
-        const uint32_t sid = tid < nb ? P.sid[64ull * g + tid] : kNoGroup;
-        if (tid < nb) P.gref[sid] = (colg << 6) | tid;
-        P.gsample[64ull * colg + tid] = sid;
-      }
-    }
-    __syncthreads();
+}
+
+// The group of chunk c (cbase is ascending, groups + 1 entries).
+__device__ inline uint32_t group_of_chunk(const unsigned long long *cbase, uint32_t groups,
+                                          uint64_t c) {
+  uint32_t lo = 0, hi = groups;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cbase[mid + 1] <= c) lo = mid + 1;
+    else hi = mid;
   }
-  for (uint32_t i = tid; i < U; i += BT)
-    for (unsigned long long mk = masks[i]; mk; mk &= mk - 1) atomicAdd(&s_cnt[__ffsll(mk) - 1], 1u);
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t acc = 0;
-    for (uint32_t c = 0; c < nb; ++c) {
-      s_cofs[c] = acc;
-      acc += s_cnt[c];
+  return lo;
+}
+
+// Step 2: per chunk (one wave), members of each colour, all and hot.
+__global__ void __launch_bounds__(64) out_chunk_count_kernel(OutParams P) {
+  const uint64_t c = blockIdx.x;
+  const uint32_t i = group_of_chunk(P.cbase, P.groups, c);
+  const uint32_t lane = lane_id();
+  const uint64_t e0 = (c - P.cbase[i]) * kChunk;
+  const uint64_t e1 = min(e0 + kChunk, P.usz[i]);
+  const uint32_t *list = P.slist + P.soff[i];
+  const unsigned long long *masks = P.smask + P.soff[i];
+  uint32_t all = 0, hot = 0;   // lane j: colour j's counts
+  for (uint64_t t = e0; t < e1; t += 64) {
+    const uint64_t e = t + lane;
+    const unsigned long long m = e < e1 ? masks[e] : 0ull;
+    const bool h = P.colptr && e < e1 && P.hot_index[list[e]] != kNotHot;
+    for (unsigned long long u = uniform_u64(wave_or_u64(m)); u; u &= u - 1) {
+      const uint32_t col = static_cast<uint32_t>(__ffsll(u) - 1);
+      const uint64_t b = __ballot((m >> col) & 1ull);
+      const uint64_t bh = __ballot(h && ((m >> col) & 1ull));
+      if (lane == col) {
+        all += static_cast<uint32_t>(__popcll(b));
+        hot += static_cast<uint32_t>(__popcll(bh));
+      }
     }
-    const unsigned long long base =
-        atomicAdd(&P.ctl[kCtlDataBump], static_cast<unsigned long long>(acc));
+  }
+  P.cc[c * 64 + lane] = all;
+  P.cch[c * 64 + lane] = hot;
+}
+
+// Step 3: per group (64 threads, thread j = colour j): the column decision,
+// colour j's list offsets over the group's chunks, one pool reservation.
+__global__ void __launch_bounds__(64) out_group_kernel(OutParams P) {
+  __shared__ unsigned long long s_hot, s_base;
+  __shared__ uint32_t s_col, s_fail;
+  __shared__ unsigned long long s_tot[64];
+  const uint32_t i = blockIdx.x, g = P.g0 + i, j = threadIdx.x;
+  const uint32_t nb = P.nb[g];
+  const uint64_t c0 = P.cbase[i], c1 = P.cbase[i + 1];
+  unsigned long long hot = 0;
+  for (uint64_t c = c0; c < c1; ++c) hot += P.cch[c * 64 + j];
+  for (int d = 32; d > 0; d >>= 1) hot += __shfl_xor(hot, d, 64);
+  if (j == 0) {
+    s_hot = hot;
+    s_col = kNoGroup;
+    // a column costs 8 bytes per hot vertex, lists 4 per member
+    if (P.colptr && hot > 2ull * P.hot_n)
+      s_col = P.group_base + static_cast<uint32_t>(atomicAdd(&P.ctl[kCtlGroupCounter2], 1ull));
+  }
+  __syncthreads();
+  const bool col = s_col != kNoGroup;
+  // colour j's offsets in its list, chunk by chunk
+  unsigned long long run = 0;
+  for (uint64_t c = c0; c < c1; ++c) {
+    const uint32_t k = P.cc[c * 64 + j] - (col ? P.cch[c * 64 + j] : 0u);
+    P.cc[c * 64 + j] = static_cast<uint32_t>(run);
+    run += k;
+  }
+  s_tot[j] = run;
+  __syncthreads();
+  if (j == 0) {
+    unsigned long long acc = 0;
+    for (uint32_t q = 0; q < nb; ++q) acc += s_tot[q];
+    const unsigned long long base = atomicAdd(&P.ctl[kCtlDataBump], acc);
     s_base = base;
     s_fail = 0;
     if (base + acc > P.cap_data) {
       s_fail = 1;
       atomicOr(&P.ctl[kCtlOverflow], 1ull);
     }
+    if (col) atomicAdd(&P.ctl[kCtlElems], s_hot);
+    P.gcol[i] = s_col;
+    P.gfail[i] = s_fail;
   }
   __syncthreads();
-  if (!s_fail) {
-    const unsigned long long base = s_base;
-    for (uint32_t c = wid; c < nb; c += NW) {
-      uint32_t run = 0;
-      const unsigned long long dst0 = base + s_cofs[c];
-      for (uint32_t i0 = 0; i0 < U; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        const bool bit = i < U && ((masks[i] >> c) & 1ull);
-        const uint64_t bm = __ballot(bit);
-        if (bit) P.data[dst0 + run + __popcll(bm & lanes_below())] = list[i];
-        run += __popcll(bm);
+  unsigned long long mine = s_base;
+  for (uint32_t q = 0; q < j; ++q) mine += s_tot[q];
+  P.dbase[i * 64ull + j] = mine;
+  if (j < nb) {
+    const uint32_t sid = P.sid[64ull * g + j];
+    if (s_fail) {
+      P.set_len[sid] = kLenFail;
+    } else {
+      P.set_off[sid] = mine;
+      P.set_len[sid] = static_cast<uint32_t>(s_tot[j]);
+    }
+    if (col) P.gref[sid] = (s_col << 6) | j;
+  }
+  if (col) P.gsample[64ull * s_col + j] = j < nb ? P.sid[64ull * g + j] : kNoGroup;
+  unsigned long long bytes = s_fail ? 0ull : 4ull * s_tot[j];
+  report_stats(P.ctl, bytes, 0, 0, 0, 0);
+}
+
+// Step 3b: the columns of the sub-batch's column groups start at zero.
+__global__ void out_column_clear_kernel(OutParams P) {
+  const uint64_t total = static_cast<uint64_t>(P.groups) * P.hot_n;
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t i = static_cast<uint32_t>(t / P.hot_n);
+    const uint32_t cg = P.gcol[i];
+    if (cg != kNoGroup && !P.gfail[i]) P.colptr[cg][t - static_cast<uint64_t>(i) * P.hot_n] = 0ull;
+  }
+}
+
+// Step 4: per chunk (one wave), hot members into the column (column
+// groups), the rest to their colours' lists in vertex order.
+__global__ void __launch_bounds__(64) out_scatter_kernel(OutParams P) {
+  const uint64_t c = blockIdx.x;
+  const uint32_t i = group_of_chunk(P.cbase, P.groups, c);
+  if (P.gfail[i]) return;   // the call is redone with more pool space
+  const uint32_t lane = lane_id();
+  const uint32_t cg = P.gcol[i];
+  const unsigned long long base = P.dbase[i * 64ull + lane];   // lane j: colour j
+  const uint64_t e0 = (c - P.cbase[i]) * kChunk;
+  const uint64_t e1 = min(e0 + kChunk, P.usz[i]);
+  const uint32_t *list = P.slist + P.soff[i];
+  const unsigned long long *masks = P.smask + P.soff[i];
+  unsigned long long *colv = cg != kNoGroup ? P.colptr[cg] : nullptr;
+  unsigned long long pos = base + P.cc[c * 64 + lane];   // lane j: next slot of colour j
+  for (uint64_t t = e0; t < e1; t += 64) {
+    const uint64_t e = t + lane;
+    unsigned long long m = 0;
+    uint32_t v = 0;
+    if (e < e1) {
+      m = masks[e];
+      v = list[e];
+      if (colv) {
+        const uint32_t h = P.hot_index[v];
+        if (h != kNotHot) {
+          colv[h] = m;
+          m = 0;
+        }
       }
     }
-    if (tid < nb) {
-      const uint32_t s = P.sid[64ull * g + tid];
-      P.set_off[s] = base + s_cofs[tid];
-      P.set_len[s] = s_cnt[tid];
-      st_bytes += 4ull * s_cnt[tid];
+    for (unsigned long long u = uniform_u64(wave_or_u64(m)); u; u &= u - 1) {
+      const uint32_t col = static_cast<uint32_t>(__ffsll(u) - 1);
+      const uint64_t b = __ballot((m >> col) & 1ull);
+      const unsigned long long at = __shfl(pos, static_cast<int>(col), 64);
+      if ((m >> col) & 1ull) P.data[at + __popcll(b & lanes_below())] = v;
+      if (lane == col) pos += __popcll(b);
     }
-  } else if (tid < nb) {
-    P.set_len[P.sid[64ull * g + tid]] = kLenFail;
   }
-  report_stats(P.ctl, st_bytes, 0, 0, 0, 0);
 }
 
 // Element 0 and sample id of every colour of the batch's groups.
@@ -707,13 +860,13 @@ static void grid_plan(const rip_graph *g, uint64_t count, uint32_t *groups, uint
   const uint64_t held = s.n == g->n ? s.groups * group_bytes(g->n) + 16ull * s.fcap : 0;
   // the scratch stays within an eighth of the memory it may use, 32 GiB at most
   const uint64_t budget = std::min<uint64_t>((free_b + held) / 8, 32ull << 30);
-  const uint64_t per = group_bytes(g->n) + 16ull * std::min<uint64_t>(g->n, 1u << 21);
+  const uint64_t per = group_bytes(g->n) + 16ull * std::min<uint64_t>(g->n, 1u << 22);
   uint64_t G = std::max<uint64_t>(1, budget / per);
   G = std::min<uint64_t>({G, 256, (count + 63) / 64});
   *groups = static_cast<uint32_t>(G);
-  // room for every group's frontier up to 2^21 vertices, and for one whole
+  // room for every group's frontier up to 2^22 vertices, and for one whole
   // union (the output staging reuses the frontier buffers)
-  *fcap = std::max<uint64_t>(g->n, G * std::min<uint64_t>(g->n, 1u << 21));
+  *fcap = std::max<uint64_t>(g->n, G * std::min<uint64_t>(g->n, 1u << 22));
 }
 
 static void grid_ensure(rip_graph *g, uint32_t groups, uint64_t fcap, hipStream_t st) {
@@ -761,6 +914,15 @@ static void grid_ensure(rip_graph *g, uint32_t groups, uint64_t fcap, hipStream_
   }
 }
 
+// Room for `words` 64-bit words of output tables.
+static void grid_out_arena(GridScratch &s, uint64_t words) {
+  if (s.out_words >= words) return;
+  dev_free(s.d_out);
+  s.out_words = 0;
+  s.d_out = dev_alloc<unsigned long long>(words);
+  s.out_words = words;
+}
+
 // Reads `count` words of device memory into the scratch's pinned buffer.
 static const unsigned long long *grid_read(GridScratch &s, const unsigned long long *src,
                                            uint32_t count, hipStream_t st) {
@@ -783,6 +945,8 @@ void grid_scratch_free(GridScratch &s) {
   dev_free(s.d_gctl);
   dev_free(s.d_union);
   dev_free(s.d_ring);
+  dev_free(s.d_out);
+  s.out_words = 0;
   if (s.h_ctl) (void)hipHostFree(s.h_ctl);
   s.h_ctl = nullptr;
   s.groups = 0;
@@ -915,24 +1079,62 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
     O.hot_n = columns ? gr->hot_n : 0;
     O.group_base = p->groups;
     for (uint32_t a = 0; a < Gb;) {
-      // staging offsets of groups a, a + 1, ... while they fit (one always does)
-      std::vector<unsigned long long> off;
+      // groups a, a + 1, ... while their unions fit the staging together (one
+      // always does); their staging offsets and chunks
+      std::vector<unsigned long long> off, cb(1, 0);
       unsigned long long acc = 0;
       uint32_t e = a;
       while (e < Gb && (e == a || acc + usz[e] <= s.fcap)) {
         off.push_back(acc);
         acc += usz[e];
+        cb.push_back(cb.back() + (usz[e] + kChunk - 1) / kChunk);
         ++e;
       }
-      // the offsets travel in the union buffer (its counts are on the host)
-      RIP_HIP(hipMemcpyAsync(s.d_union, off.data(), sizeof(unsigned long long) * off.size(),
+      const uint32_t m = e - a;
+      const uint64_t chunks = cb.back();
+      // device tables of the sub-batch, carved from the output arena
+      const uint64_t words = 4ull * m + kRanges * static_cast<uint64_t>(m) + 64ull * m +
+                             64ull * chunks + 2;   // u64 words
+      grid_out_arena(s, words);
+      unsigned long long *w = s.d_out;
+      unsigned long long *d_soff = w;          w += m;
+      unsigned long long *d_cbase = w;         w += m + 1;
+      unsigned long long *d_usz = w;           w += m;
+      unsigned long long *d_rc = w;            w += static_cast<uint64_t>(kRanges) * m;
+      unsigned long long *d_dbase = w;         w += 64ull * m;
+      uint32_t *d_cc = reinterpret_cast<uint32_t *>(w);
+      uint32_t *d_cch = d_cc + 64ull * chunks; w += 64ull * chunks;
+      uint32_t *d_gcol = reinterpret_cast<uint32_t *>(w);
+      uint32_t *d_gfail = d_gcol + m;          w += m;
+      std::vector<unsigned long long> up(3ull * m + 1);
+      std::copy(off.begin(), off.end(), up.begin());
+      std::copy(cb.begin(), cb.end(), up.begin() + m);
+      std::copy(usz.begin() + a, usz.begin() + e, up.begin() + 2 * m + 1);
+      RIP_HIP(hipMemcpyAsync(s.d_out, up.data(), sizeof(unsigned long long) * up.size(),
                              hipMemcpyHostToDevice, st));
       O.g0 = a;
-      O.soff = s.d_union;
-      hipLaunchKernelGGL((grid_output_kernel<kOutBT>), dim3(e - a), dim3(kOutBT), 0, st, O);
+      O.groups = m;
+      O.soff = d_soff;
+      O.cbase = d_cbase;
+      O.usz = d_usz;
+      O.rc = d_rc;
+      O.dbase = d_dbase;
+      O.cc = d_cc;
+      O.cch = d_cch;
+      O.gcol = d_gcol;
+      O.gfail = d_gfail;
+      hipLaunchKernelGGL((out_range_count_kernel<256>), dim3(m * kRanges), dim3(256), 0, st, O);
+      hipLaunchKernelGGL((out_range_write_kernel<256>), dim3(m * kRanges), dim3(256), 0, st, O);
+      hipLaunchKernelGGL(out_chunk_count_kernel, dim3(static_cast<uint32_t>(chunks)), dim3(64), 0,
+                         st, O);
+      hipLaunchKernelGGL(out_group_kernel, dim3(m), dim3(64), 0, st, O);
+      if (columns)
+        hipLaunchKernelGGL(out_column_clear_kernel, dim3(4096), dim3(256), 0, st, O);
+      hipLaunchKernelGGL(out_scatter_kernel, dim3(static_cast<uint32_t>(chunks)), dim3(64), 0, st,
+                         O);
       RIP_HIP(hipGetLastError());
-      RIP_HIP(hipStreamSynchronize(st));   // `off` and the staging are reused
-      launches += 1;
+      RIP_HIP(hipStreamSynchronize(st));   // `up` and the staging are reused
+      launches += 6;
       a = e;
     }
     launches += 1;

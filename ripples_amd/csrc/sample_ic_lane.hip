@@ -337,21 +337,37 @@ uint32_t ic_lane_cap(const rip_tune *tune) {
   return c;
 }
 
+// Threads per workgroup of the lane kernel (tune->block; 256 by default).
+static uint32_t ic_lane_block(const rip_tune *tune) {
+  const uint32_t b = tune && tune->block ? tune->block : kLaneBT;
+  RIP_REQUIRE(b == 256 || b == 512 || b == 1024, -1, "tune: block must be 256, 512 or 1024");
+  return b;
+}
+
+static const void *lane_kernel_ptr(uint32_t bt) {
+  switch (bt) {
+    case 512: return reinterpret_cast<const void *>(&ic_lane_kernel<512>);
+    case 1024: return reinterpret_cast<const void *>(&ic_lane_kernel<1024>);
+    default: return reinterpret_cast<const void *>(&ic_lane_kernel<256>);
+  }
+}
+
 // Lanes of a lane-sampler launch of `count` samples: every resident lane,
-// at most one per sample, within 1/16 of the device memory.
-static uint64_t lane_lanes(uint64_t count, uint32_t cap) {
-  const int per_cu = blocks_per_cu(reinterpret_cast<const void *>(&ic_lane_kernel<kLaneBT>), kLaneBT);
+// at most one per sample, within 1/16 of the device memory; a multiple of
+// the block size bt.
+static uint64_t lane_lanes(uint64_t count, uint32_t cap, uint32_t bt) {
+  const int per_cu = blocks_per_cu(lane_kernel_ptr(bt), static_cast<int>(bt));
   size_t free_b = 0, total_b = 0;
   RIP_HIP(hipMemGetInfo(&free_b, &total_b));
   const uint64_t per_lane = 20ull * cap + 4;       // table 16 cap + list 4 cap + epoch
-  uint64_t lanes = static_cast<uint64_t>(per_cu) * cu_count() * kLaneBT;
-  lanes = std::min<uint64_t>(lanes, std::max<uint64_t>(kLaneBT, (total_b / 16) / per_lane));
-  lanes = std::min<uint64_t>(lanes, (count + kLaneBT - 1) / kLaneBT * kLaneBT);
-  return std::max<uint64_t>(kLaneBT, lanes / kLaneBT * kLaneBT);
+  uint64_t lanes = static_cast<uint64_t>(per_cu) * cu_count() * bt;
+  lanes = std::min<uint64_t>(lanes, std::max<uint64_t>(bt, (total_b / 16) / per_lane));
+  lanes = std::min<uint64_t>(lanes, (count + bt - 1) / bt * bt);
+  return std::max<uint64_t>(bt, lanes / bt * bt);
 }
 
 uint64_t ic_lane_slack(const rip_pool *p, uint64_t count, const rip_tune *tune) {
-  const uint64_t waves = lane_lanes(count, ic_lane_cap(tune)) / 64;
+  const uint64_t waves = lane_lanes(count, ic_lane_cap(tune), ic_lane_block(tune)) / 64;
   const uint32_t chunk = tune && tune->out_chunk ? tune->out_chunk : 0;
   return waves * (chunk ? chunk : 4096u);
 }
@@ -362,7 +378,8 @@ uint64_t ic_lane_launch(rip_pool *p, uint64_t first, uint64_t count, const rip_t
   rip_graph *g = p->g;
   WaveScratch &w = g->wave;
   const uint32_t cap = ic_lane_cap(tune);
-  const uint64_t lanes = lane_lanes(count, cap);
+  const uint32_t bt = ic_lane_block(tune);
+  const uint64_t lanes = lane_lanes(count, cap, bt);
   if (w.lane_lanes < lanes || w.lane_cap != cap) {
     dev_free(w.d_ltable);
     dev_free(w.d_llist);
@@ -413,8 +430,12 @@ uint64_t ic_lane_launch(rip_pool *p, uint64_t first, uint64_t count, const rip_t
                                                       4096);
   hipLaunchKernelGGL(reset_lane_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
   RIP_HIP(hipEventRecord(p->ev0, p->stream));
-  hipLaunchKernelGGL((ic_lane_kernel<kLaneBT>), dim3(static_cast<uint32_t>(lanes / kLaneBT)),
-                     dim3(kLaneBT), 0, p->stream, P);
+  const dim3 grid(static_cast<uint32_t>(lanes / bt));
+  switch (bt) {
+    case 512: hipLaunchKernelGGL((ic_lane_kernel<512>), grid, dim3(512), 0, p->stream, P); break;
+    case 1024: hipLaunchKernelGGL((ic_lane_kernel<1024>), grid, dim3(1024), 0, p->stream, P); break;
+    default: hipLaunchKernelGGL((ic_lane_kernel<256>), grid, dim3(256), 0, p->stream, P); break;
+  }
   RIP_HIP(hipGetLastError());
   RIP_HIP(hipEventRecord(p->ev1, p->stream));
   switch (cap) {
