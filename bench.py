@@ -8,11 +8,16 @@ self loops, R-MAT seed = scale; weights from lcg64(seed).split(2,0)):
   rmat20_lt  IMM LT, scale 20, in-weights U(0,1] normalised to 0.9, k=100,
              eps=0.13
   rmat16_ic  IMM IC, scale 16, w ~ U[0,0.1], k=50, eps=0.5
-The headline line is the largest workload that runs on one GPU.  A step is
-one sampling pass of `batch` RRR sets (batch = the theta its IMM run ends
-with) into a reset HBM pool; the graph is resident in HBM before the timed
-region starts.  IMM wall time is one full rip_imm call (theta estimation +
-final sampling + selection) after a warm-up call.
+The headline line is the largest workload, rmat24_ic.  A step is one
+sampling pass of `batch` RRR sets into a reset HBM pool (batch = 100,000 sets
+for rmat24_ic, whose theta of about 4.4M sets is reported by its IMM run;
+the theta its IMM run ends with for the others); the graph is resident in
+HBM before the timed region starts.  IMM wall time is one full rip_imm call
+(theta estimation + final sampling + selection), after a warm-up call on the
+smaller workloads and after a warm-up sampling call of 20,000 sets on
+rmat24_ic.  graph_create_s (the device layout, including the LT breakpoints,
+whose cost grows with the sum of squared in-degrees, reported as
+sum_indeg_squared) is reported beside it.
 
 Roofline: the dominant kernel's algorithmic bytes per launch (the bytes its
 own algorithm reads and writes, counted on the device: IC gap samplers 64 per
@@ -52,6 +57,7 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     "rmat24_ic": dict(scale=24, model=R.IC, scheme=R.W_UNIFORM, wscale=0.02, k=100, eps=0.1,
+                      batch=100000, imm_warmup=False,
                       desc="IMM IC, R-MAT scale 24, w ~ U[0,0.02], k=100, eps=0.1"),
     "rmat22_wc": dict(scale=22, model=R.IC, scheme=R.W_CASCADE, wscale=1.0, k=200, eps=0.1,
                       desc="IMM IC, R-MAT scale 22, weighted cascade 1/indeg, k=200, eps=0.1"),
@@ -182,13 +188,18 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
     graph_create_s = time.time() - t
     log("%s: rip_graph_create %.2fs (device layout%s)" % (
         name, graph_create_s, ", LT breakpoints and index" if wl["model"] == R.LT else ""))
-    seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)   # warm-up
+    if wl.get("imm_warmup", True):
+        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)   # warm-up
+    else:
+        warm = g.pool(seed + 1)
+        warm.generate_rrr_sets(20000, tune=tune)
+        warm.close()
     t = time.time()
     seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)
     imm_wall = time.time() - t
     log("%s IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
         name, rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
-    batch = batch or int(rec["Theta"])
+    batch = batch or wl.get("batch") or int(rec["Theta"])
     pool = g.pool(seed)
     wall, kms, calls, kstats, grids = sampling_steps(pool, batch, steps, warmup, tune)
     stats = pool.last_stats()
@@ -211,16 +222,21 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
         if grid_ms > kernel_ms:
             # the samples that outgrew a lane dominate: the grid sampler's
             # level kernels are the dominant kernel, timed over all levels
-            roof["kernel"] = "grid_expand_kernel + grid_tail_kernel (all BFS levels of the call)"
+            roof["kernel"] = "grid_level_kernel (all BFS levels of the call)"
             per_launch = grid_bytes
             kernel_ms = grid_ms
             roof["kernel_ms"] = grid_ms
         roof["algo_bytes_formula"] = ("64 per row expanded (vertex record) + 4 per candidate "
                                       "edge of a row with mixed thresholds + 4 per live edge "
                                       "(in-neighbour) + 4 per output id, counted on the device")
-        off, ver = pool.export()
-        ref = ic_reference_bytes(index, off, ver)
-        roof["reference_per_edge_bytes"] = ref
+        if n_elem <= (1 << 28):
+            off, ver = pool.export()
+            roof["reference_per_edge_bytes"] = ic_reference_bytes(index, off, ver)
+        else:
+            # a pool of billions of elements is not copied to the host: the
+            # figure is taken over its first 4096 sets
+            off, ver = pool.export(0, 4096)
+            roof["reference_per_edge_bytes_first_4096_sets"] = ic_reference_bytes(index, off, ver)
         roof["reference_per_edge_note"] = (
             "SURVEY 8d: sum over the union of the launch's sets of (8 + 8 indeg) + 4 per "
             "output id, the bytes of one coin per in-edge; the gap sampler draws only the "
@@ -256,7 +272,8 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
                 "generate_ms": rec["GenerateRRRSets"], "select_ms": rec["FindMostInfluentialSet"],
                 "rrr_elements": rec["RRRElements"]},
         "graph_create_s": graph_create_s,
-        "pool": {"sets": n_sets, "elements": n_elem},
+        "sum_indeg_squared": int(np.sum(np.diff(index.astype(np.int64)) ** 2)),
+        "pool": {"sets": n_sets, "elements": n_elem, "device_bytes": pool.bytes()},
         "kernel_counters": ks, "call_counters": stats,
         "roofline": roof,
     }
@@ -457,17 +474,17 @@ def copy_probe(gib):
 
 
 SECONDARY_KEYS = ("value", "ms_per_step", "config", "imm_wall_s", "imm", "graph_create_s",
-                  "pool", "kernel_counters", "roofline", "cpu_baseline", "cpu_imm", "spread",
+                  "sum_indeg_squared", "pool", "kernel_counters", "roofline", "cpu_baseline", "cpu_imm", "spread",
                   "plain_wave", "colour_groups")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat22_wc", choices=sorted(WORKLOADS))
-    ap.add_argument("--secondary", default="rmat20_lt,rmat16_ic",
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat24_ic", choices=sorted(WORKLOADS))
+    ap.add_argument("--secondary", default="rmat22_wc,rmat20_lt,rmat16_ic",
                     help="workloads reported beside the first, comma-separated ('' for none)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--batch", type=int, default=0, help="sets per step (0 = theta)")
@@ -476,7 +493,7 @@ def main():
                     help="CPU-baseline budget per workload as estimated from a small probe")
     ap.add_argument("--tune", default="", help='sampler tuning, e.g. "sampler=1,queue_cap=256"')
     ap.add_argument("--fused-steps", type=int, default=2,
-                    help="steps of the plain and colour-group samplers on the headline batch")
+                    help="steps of the plain and colour-group samplers on the rmat22_wc batch")
     ap.add_argument("--serve-scales", default="18,20,22",
                     help="R-MAT scales of the serving workload ('' skips it)")
     ap.add_argument("--serve-queries", type=int, default=48)
@@ -506,8 +523,11 @@ def main():
         tune = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in a.tune.split(",")}
     host = host_info()
     cpu_s = 0.0 if a.no_cpu else a.cpu_seconds
+    big = WORKLOADS[a.workload]["scale"] >= 24
     res = measure(a.workload, a.seed, a.steps, a.warmup, a.batch, tune, host, cpu_s,
-                  WORKLOADS[a.workload]["scale"] <= 20, a.fused_steps, (4096, 64))
+                  WORKLOADS[a.workload]["scale"] <= 20,
+                  a.fused_steps if a.workload == "rmat22_wc" else 0,
+                  (1024, 32) if big else (4096, 64))
     out = {
         "metric": "RRR sets/s + IMM wall s at fixed k,eps; p50/p99 latency of served IMM queries",
         "value": res["value"],
@@ -533,7 +553,8 @@ def main():
     others = [x for x in a.secondary.split(",") if x and x != a.workload]
     for name in others:
         sec = measure(name, a.seed, max(1, a.steps // 2), 1, 0, None, host, cpu_s,
-                      WORKLOADS[name]["scale"] <= 20, 0, None)
+                      WORKLOADS[name]["scale"] <= 20,
+                      a.fused_steps if name == "rmat22_wc" else 0, None)
         out["secondary_" + name] = {k: sec[k] for k in SECONDARY_KEYS if k in sec}
     if a.serve_scales:
         out["serving"] = serve_bench([int(x) for x in a.serve_scales.split(",")],

@@ -160,7 +160,8 @@ int rip_pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune);
  * (streaming_find_most_influential.h:139-244). */
 int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
                          const uint32_t *vertices);
-/* Autotuner: draws probe_sets samples with each candidate setting (IC: the
+/* Autotuner: draws probe_sets samples (0: 16384 times the factor, 1 to 16,
+ * that brings one candidate's draw to about 20 ms) with each candidate setting (IC: the
  * lane list capacity 64/256/1024 x the lane kernel's workgroup size
  * 256/512/1024; LT: the walk's output chunk) and keeps the fastest, which
  * the graph then uses for every sampling call given no tune (rip_pool_sample,
@@ -176,6 +177,11 @@ int rip_autotune(rip_graph *g, uint64_t probe_sets, int force, rip_tune *best,
 int rip_pool_reset(rip_pool *p);
 int rip_pool_size(const rip_pool *p, uint64_t *num_sets,
                   uint64_t *num_elements);
+/* Device bytes the pool holds (sorted lists, per-set offsets and lengths,
+ * the column store and its tables): what the serving budget counts.  The
+ * reference's RRRsets are host vectors (rrr_sets.h:81-105); no counterpart
+ * call. */
+int rip_pool_device_bytes(const rip_pool *p, uint64_t *bytes);
 /* The pool's store of large sets: IC samples that outgrow one BFS per wave
  * are drawn 64 at a time by colour groups, and a group whose sets hold more
  * than 2 * hot_nodes members among the graph's hot vertices (a quarter of

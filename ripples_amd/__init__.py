@@ -140,6 +140,7 @@ def lib():
                                C.POINTER(C.c_uint32)]
     L.rip_pool_last_grid.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), _u64p,
                                      C.c_uint32]
+    L.rip_pool_device_bytes.argtypes = [vp, _u64p]
     L.rip_pool_columns.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint64)]
     L.rip_pool_last_kernel_stats.argtypes = [vp, _u64p, C.c_uint32]
@@ -330,10 +331,10 @@ class Graph:
                                 len(seeds), seed, first, replicas, act, lev))
         return act[:replicas], lev[:replicas]
 
-    def autotune(self, probe_sets=16384, force=False):
-        """rip_autotune: the fastest sampler setting on a probe batch; the
-        graph uses it from then on.  Returns (setting dict, ms per probe
-        batch, settings tried)."""
+    def autotune(self, probe_sets=0, force=False):
+        """rip_autotune: the fastest sampler setting on a probe batch (0: sized
+        to about 20 ms per setting); the graph uses it from then on.  Returns
+        (setting dict, ms per probe batch, settings tried)."""
         t = Tune()
         ms, tried = C.c_double(0), C.c_uint32(0)
         _check(lib().rip_autotune(self._h, probe_sets, 1 if force else 0, C.byref(t),
@@ -463,6 +464,12 @@ class Pool:
         out = np.zeros(len(self.STAT_NAMES), np.uint64)
         _check(lib().rip_pool_last_stats(self._h, out, len(out)))
         return {k: int(v) for k, v in zip(self.STAT_NAMES, out)}
+
+    def bytes(self):
+        """Device bytes the pool holds (rip_pool_device_bytes)."""
+        out = np.zeros(1, np.uint64)
+        _check(lib().rip_pool_device_bytes(self._h, out))
+        return int(out[0])
 
     def columns(self):
         """(column groups in use, hot vertices per column, column bytes allocated)."""

@@ -161,6 +161,13 @@ int rip_pool_columns(const rip_pool *p, uint32_t *groups, uint32_t *hot_nodes,
   });
 }
 
+int rip_pool_device_bytes(const rip_pool *p, uint64_t *bytes) {
+  return guarded([&] {
+    RIP_REQUIRE(p && bytes, RIP_EINVAL, "pool or bytes is NULL");
+    *bytes = rip::pool_device_bytes(p);
+  });
+}
+
 int rip_pool_size(const rip_pool *p, uint64_t *num_sets, uint64_t *num_elements) {
   return guarded([&] {
     RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");

@@ -5,6 +5,8 @@
 // double -> integer truncations agree.
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "internal.h"
@@ -52,6 +54,16 @@ struct PoolGuard {
   ~PoolGuard() { pool_destroy(&pool); }
 };
 
+// RIPPLES_AMD_LOG=1: one line per estimation round and for the final round
+// on stderr (the reference logs its rounds through spdlog).
+bool imm_log() {
+  static const bool on = [] {
+    const char *v = std::getenv("RIPPLES_AMD_LOG");
+    return v && *v && *v != '0';
+  }();
+  return on;
+}
+
 }  // namespace
 
 void imm_run(rip_graph *g, uint32_t k, double eps, uint64_t seed,
@@ -89,6 +101,9 @@ void imm_run(rip_graph *g, uint32_t k, double eps, uint64_t seed,
       r.round_select_ms[r.rounds] = sel_ms;
     }
     r.rounds++;
+    if (imm_log())
+      std::fprintf(stderr, "[imm] round %lld: theta' %lld, sampled in %.1f ms, selected in %.1f ms, "
+                   "f %.6f\n", static_cast<long long>(x), static_cast<long long>(tp), gen_ms, sel_ms, f);
     if (f >= std::pow(2, static_cast<double>(-x))) {
       lb = (static_cast<double>(n) * f) / (1 + eps_prime);
       break;
@@ -111,6 +126,10 @@ void imm_run(rip_graph *g, uint32_t k, double eps, uint64_t seed,
   r.lb = lb;
   r.final_f = theta ? static_cast<double>(cov) / static_cast<double>(theta) : 0.0;
   r.total_ms = ms_since(t_total);
+  if (imm_log())
+    std::fprintf(stderr, "[imm] final: theta %llu, sampled in %.1f ms, selected in %.1f ms, "
+                 "total %.1f ms\n", static_cast<unsigned long long>(theta), r.generate_ms,
+                 r.select_ms, r.total_ms);
   if (rec) *rec = r;
 }
 

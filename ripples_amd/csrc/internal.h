@@ -69,7 +69,7 @@ struct WaveScratch {
 struct GridScratch {
   uint32_t groups = 0, n = 0;
   uint64_t fcap = 0;
-  unsigned long long *d_vis = nullptr, *d_pend = nullptr;  // groups x n
+  unsigned long long *d_vis = nullptr;  // groups x n pairs (visited, pending colours)
   uint32_t *d_bm0 = nullptr, *d_bm1 = nullptr;
   unsigned long long *d_z = nullptr;   // groups x 64
   uint32_t *d_sid = nullptr;           // groups x 64
@@ -233,6 +233,7 @@ struct rip_pool {
   uint64_t last_stats[RIP_STAT_COUNT] = {};
   uint64_t main_stats[RIP_STAT_COUNT] = {};  // the main kernel's share of last_stats
   double avg_set_size = 8.0;            // running estimate for capacity planning
+  bool sized = false;                   // avg_set_size measured by a call
 };
 
 namespace rip {
@@ -270,6 +271,9 @@ void graph_destroy(rip_graph *g);
 void pool_create(rip_pool *p, rip_graph *g, uint64_t seed);
 void pool_create_resources(rip_pool *p);  // stream, events, control words
 void pool_destroy(rip_pool *p);
+// Device bytes the pool holds: lists, per-set offsets, lengths and group
+// references, columns and their tables.
+uint64_t pool_device_bytes(const rip_pool *p);
 void pool_reserve_sets(rip_pool *p, uint64_t cap_sets);
 void pool_reserve_data(rip_pool *p, uint64_t cap_data);
 // Column store: the graph's hot vertices (built on first use), room for

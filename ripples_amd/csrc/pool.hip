@@ -337,6 +337,14 @@ void pool_clear_gref(rip_pool *p, uint64_t first, uint64_t end) {
 // so the store grows without moving them; the pointer and sample tables are
 // reallocated.  The first call also sets up d_gref (every set so far a plain
 // list).
+uint64_t pool_device_bytes(const rip_pool *p) {
+  uint64_t b = 4ull * p->cap_data + 12ull * p->cap_sets;      // data, d_off, d_len
+  if (p->d_gref) b += 4ull * p->cap_sets;
+  b += 8ull * kColBlock * p->g->hot_n * p->col_blocks.size();
+  b += (8ull + 4ull * 64) * p->cap_groups;                     // d_colptr, d_gsample
+  return b;
+}
+
 void pool_reserve_groups(rip_pool *p, uint32_t need) {
   graph_ensure_hot(p->g, p->stream);
   if (!p->d_gref) {
@@ -447,8 +455,19 @@ void pool_read_ctl(rip_pool *p, bool main) {
     for (int i = 0; i < RIP_STAT_COUNT; ++i) p->main_stats[i] = p->h_ctl[kCtlStatBytes + i];
 }
 
+// Samples of the first call of a pool drawn apart to size its data.
+constexpr uint64_t kSizingPrefix = 16384;
+
 void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
   if (count == 0) return;
+  if (!p->sized && count > 4 * kSizingPrefix) {
+    // with no set size measured yet the data capacity is a guess, and a call
+    // that overflows it is drawn again: a prefix of the call measures the
+    // size first (the same sample ids in two calls give the same sets)
+    pool_sample(p, kSizingPrefix, tune);
+    pool_sample(p, count - kSizingPrefix, tune);
+    return;
+  }
   if (!tune && p->g->tuned_set) tune = &p->g->tuned;   // the autotuner's choice
   RIP_HIP(hipSetDevice(p->g->device));
   RIP_REQUIRE(p->num_sets + count < (1ull << 32), -1,
@@ -513,6 +532,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
       p->num_elements += ctl[kCtlElems];
       p->groups += static_cast<uint32_t>(ctl[kCtlGroupCounter2]);
       p->avg_set_size = std::max(1.0, static_cast<double>(added) / static_cast<double>(count));
+      p->sized = true;
       p->num_sets += count;
       return;
     }

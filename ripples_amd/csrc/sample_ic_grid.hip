@@ -12,8 +12,10 @@
 // Samples that reach a giant component (R-MAT with U[0,0.1] or U[0,0.02]
 // weights) visit hundreds of thousands of vertices each.  Per group g (64
 // samples, colour c = bit c) the scratch holds
-//   vis[g][v], pend[g][v] : colour masks of the visited and of the not yet
-//                           expanded colours of vertex v (dense, n words each)
+//   vp[g][v] = (vis, pend) : colour masks of the visited and of the not yet
+//                            expanded colours of vertex v, side by side in
+//                            one 16-byte pair (dense, n pairs per group), so
+//                            a discovery's two atomics touch one cache line
 //   bm0[g], bm1[g]        : bitmap of the group's union and its summary
 // so the groups of a batch cost 16 bytes per vertex each, which bounds the
 // batch by memory rather than by the number of workgroups: every level is
@@ -44,19 +46,28 @@ namespace rip {
 namespace {
 
 constexpr int kExpBT = 256;          // expand kernel
+// frontier items per chunk of the expand kernel: half a workgroup, which
+// keeps the chunk's tables small enough in LDS for six workgroups per CU
+constexpr uint32_t kExpItems = 128;
 // rows pending for this many colours or more are expanded a block per wave,
 // lane c drawing colour c, so a live edge is one discovery for all colours
 constexpr uint32_t kDenseColours = 8;
 constexpr uint32_t kDenseSpan = 128;   // largest block of the dense path (edges)
-constexpr uint32_t kDenseSlots = 16;   // dense tasks per round of a workgroup
+constexpr uint32_t kDenseSlots = 8;    // dense tasks per round of a workgroup
+constexpr uint32_t kDenseBatch = 4;    // candidates a dense lane reads thresholds for at once
+constexpr uint32_t kSparseSteps = 8;   // steps of a sparse lane between discoveries
 // level counters kept on the device (a ring), and levels enqueued per host
 // synchronisation: a BFS of hundreds of narrow levels costs a launch per
 // level, not a round trip
 constexpr uint32_t kRing = 64;
 constexpr uint32_t kLevelsPerSync = 16;
+// a frontier of at most this many items is expanded by one workgroup that
+// runs level after level with a barrier between them (no launch per level):
+// the long thin tails of deep samples
+constexpr uint64_t kTailItems = 2048;
 
 // control words of the grid scratch
-enum : int { kGOver = 0, kGLevels = 1, kGCount = 2, kGWords = 4 };
+enum : int { kGOver = 0, kGLevels = 1, kGCount = 2, kGTail = 3, kGWords = 4 };
 
 struct GridParams {
   const VertexRec *vrec;
@@ -64,8 +75,7 @@ struct GridParams {
   const uint32_t *val;
   uint32_t n;
   Affine step;
-  unsigned long long *vis;         // groups x n
-  unsigned long long *pend;        // groups x n
+  unsigned long long *vp;          // groups x n pairs (visited, pending colours)
   uint32_t *bm0, *bm1;             // groups x nw0, groups x nw1
   uint32_t nw0, nw1;
   const unsigned long long *z;     // groups x 64: element 0 of each colour's sample
@@ -112,10 +122,10 @@ __device__ inline void grid_discover(const GridParams &P, bool live, uint32_t g,
   if (live) {
     const uint64_t idx = static_cast<uint64_t>(g) * P.n + u;
     // vis is written by atomics, which complete in L2; the read goes there too
-    unsigned long long old = __hip_atomic_load(&P.vis[idx], __ATOMIC_RELAXED,
+    unsigned long long old = __hip_atomic_load(&P.vp[2 * idx], __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
     if (bits & ~old) {
-      old = atomicOr(&P.vis[idx], bits);
+      old = atomicOr(&P.vp[2 * idx], bits);
       const unsigned long long fresh = bits & ~old;
       if (fresh) {
         if (old == 0) {
@@ -124,7 +134,7 @@ __device__ inline void grid_discover(const GridParams &P, bool live, uint32_t g,
           if (o0 == 0)
             atomicOr(&P.bm1[static_cast<uint64_t>(g) * P.nw1 + (u >> 10)], 1u << ((u >> 5) & 31u));
         }
-        push = atomicOr(&P.pend[idx], fresh) == 0;
+        push = atomicOr(&P.vp[2 * idx + 1], fresh) == 0;
       }
     }
   }
@@ -137,6 +147,66 @@ __device__ inline void grid_discover(const GridParams &P, bool live, uint32_t g,
   if (push) {
     const unsigned long long at = base + __popcll(m & lanes_below());
     if (at < P.cap) P.next[at] = (static_cast<unsigned long long>(g) << 32) | u;
+  }
+}
+
+// grid_discover for K entries per lane (bits[k] == 0: no entry), stage by
+// stage over the entries: the K reads of vis, then the K atomics on vis, the
+// bitmaps and the pend words, so each lane keeps K independent requests in
+// flight instead of one chain; one frontier reservation per wave for all.
+// Entries of one lane may name the same (g, u): the atomics on vis settle
+// which of them finds each colour fresh, as between lanes.
+template <int K>
+__device__ inline void grid_discover_k(const GridParams &P, const uint32_t (&g)[K],
+                                       const uint32_t (&u)[K], const unsigned long long (&bits)[K]) {
+  uint64_t idx[K];
+  unsigned long long old[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    idx[k] = static_cast<uint64_t>(g[k]) * P.n + u[k];
+    old[k] = ~0ull;
+    if (bits[k])
+      old[k] = __hip_atomic_load(&P.vp[2 * idx[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (bits[k] & ~old[k]) old[k] = atomicOr(&P.vp[2 * idx[k]], bits[k]);
+    else old[k] = ~0ull;
+  }
+  unsigned long long fresh[K];
+  uint32_t o0[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    fresh[k] = bits[k] & ~old[k];
+    o0[k] = 1u;
+    if (fresh[k] && old[k] == 0)
+      o0[k] = atomicOr(&P.bm0[static_cast<uint64_t>(g[k]) * P.nw0 + (u[k] >> 5)], 1u << (u[k] & 31u));
+  }
+  bool push[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (o0[k] == 0)
+      atomicOr(&P.bm1[static_cast<uint64_t>(g[k]) * P.nw1 + (u[k] >> 10)], 1u << ((u[k] >> 5) & 31u));
+    push[k] = fresh[k] && atomicOr(&P.vp[2 * idx[k] + 1], fresh[k]) == 0;
+  }
+  uint64_t m[K];
+  uint32_t total = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    m[k] = __ballot(push[k]);
+    total += static_cast<uint32_t>(__popcll(m[k]));
+  }
+  if (total == 0) return;
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(P.next_cnt, static_cast<unsigned long long>(total));
+  base = __shfl(base, 0, 64);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (push[k]) {
+      const unsigned long long at = base + __popcll(m[k] & lanes_below());
+      if (at < P.cap) P.next[at] = (static_cast<unsigned long long>(g[k]) << 32) | u[k];
+    }
+    base += __popcll(m[k]);
   }
 }
 
@@ -162,8 +232,9 @@ struct ExpandLds {
   Affine p2[64];
   IcLogTables tab;
   uint32_t tmp[BT / 64];
-  uint32_t g[BT], beg[BT], deg[BT], T[BT], fl[BT], uf[BT], ul[BT], pre[BT], nch[BT], dpre[BT];
-  unsigned long long q[BT], rq[BT], ja[BT], jb[BT], m[BT];
+  uint32_t g[kExpItems], beg[kExpItems], deg[kExpItems], T[kExpItems], fl[kExpItems];
+  uint32_t uf[kExpItems], ul[kExpItems], pre[kExpItems], nch[kExpItems], dpre[kExpItems];
+  unsigned long long q[kExpItems], rq[kExpItems], ja[kExpItems], jb[kExpItems], m[kExpItems];
   uint32_t ucur;   // next unit of the chunk not claimed by a wave
   // the dense round's tasks: entry, block, first element, prefix of colours,
   // and per task and position the colours live there
@@ -176,7 +247,7 @@ struct ExpandStats {
   unsigned long long cand = 0, draws = 0, exp = 0, bytes = 0, live = 0;
 };
 
-// Expands the frontier items cur[c0, c0 + cl), cl <= BT: every thread of
+// Expands the frontier items cur[c0, c0 + cl), cl <= kExpItems: every thread of
 // the workgroup calls it.  A chunk's rows are split into units of one colour
 // and up to eight gap blocks (rows pending for few colours) or into wave
 // tasks of one block for every colour (rows pending for many).
@@ -196,7 +267,8 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
     const unsigned long long item = cur[c0 + tid];
     const uint32_t g = static_cast<uint32_t>(item >> 32), v = static_cast<uint32_t>(item);
     // the colours still to expand at v; a repeated item finds none
-    const unsigned long long M = atomicExch(&P.pend[static_cast<uint64_t>(g) * P.n + v], 0ull);
+    const unsigned long long M =
+        atomicExch(&P.vp[2 * (static_cast<uint64_t>(g) * P.n + v) + 1], 0ull);
     uint32_t nch = 0;
     if (M) {
       const VertexRec r = P.vrec[v];
@@ -241,7 +313,8 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
   // lane with no unit takes the next one, and every step each lane with a
   // unit makes one draw (the block's first element and its first gap, or the
   // next gap) and tests its candidate, so the lanes stay busy whatever the
-  // lengths of their units; a live edge is discovered in the same step.
+  // lengths of their units; the live edges of kSparseSteps steps are then
+  // discovered together.
   const uint32_t lane = lane_id();
   {
     const Affine Pst = s_p2[0];
@@ -251,94 +324,104 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
     unsigned long long bit = 0;
     uint64_t q = 0, rq = 0, x = 0;
     bool uniform = true;
-    for (;;) {
-      const uint64_t idle = __ballot(!has);
-      if (idle) {
-        if (nxt >= hi) {
-          uint32_t c0 = 0;
-          if (lane == 0) c0 = atomicAdd(&S.ucur, 64u);
-          c0 = uniform_u32(__shfl(c0, 0, 64));
-          nxt = min(c0, total);
-          hi = min(c0 + 64u, total);
-        }
-        if (!has) {
-          const uint32_t u = nxt + static_cast<uint32_t>(__popcll(idle & lanes_below()));
-          if (u < hi) {
-            uint32_t lo = 0, up = cl;
-            while (lo < up) {
-              const uint32_t mid = (lo + up) >> 1;
-              if (s_pre[mid] <= u) lo = mid + 1;
-              else up = mid;
-            }
-            const uint32_t ent = lo - 1;   // the last entry starting at or before u
-            const uint32_t r = u - s_pre[ent];
-            const uint32_t nblk = s_nch[ent];
-            const uint32_t j = r / nblk, k = r - j * nblk;
-            g = s_g[ent];
-            const uint32_t c = nth_bit(s_m[ent], j);
-            bit = 1ull << c;
-            const uint32_t fl = s_fl[ent];
-            const uint32_t beta = (fl >> kBetaShift) & 31u;
-            const uint32_t B = 1u << beta;
-            e0 = s_beg[ent] + (k << beta);
-            len = min(s_deg[ent] - (k << beta), B);
-            T = s_T[ent];
-            uskip = len == B ? s_uf[ent] : s_ul[ent];
-            q = s_q[ent];
-            rq = s_rq[ent];
-            uniform = (fl & kRowUniform) != 0;
-            // block k's first element: P^(2 B k) after the row's first
-            Affine E{s_ja[ent], s_jb[ent]};
-            for (uint32_t kk = k, b2 = beta + 1; kk; kk >>= 1, ++b2)
-              if (kk & 1) E = aff_then(E, s_p2[b2]);
-            x = E.a * P.z[64ull * g + c] + E.b;
-            pos = 0;
-            has = true;
-            started = false;
+    // kSparseSteps steps per lane, then their live edges discovered together
+    for (bool finished = false; !finished;) {
+      uint32_t kg[kSparseSteps], ku[kSparseSteps];
+      unsigned long long kb[kSparseSteps];
+#pragma unroll
+      for (uint32_t j = 0; j < kSparseSteps; ++j) {
+        kg[j] = 0;
+        ku[j] = 0;
+        kb[j] = 0;
+        if (finished) continue;   // wave-uniform
+        const uint64_t idle = __ballot(!has);
+        if (idle) {
+          if (nxt >= hi) {
+            uint32_t c0 = 0;
+            if (lane == 0) c0 = atomicAdd(&S.ucur, 64u);
+            c0 = uniform_u32(__shfl(c0, 0, 64));
+            nxt = min(c0, total);
+            hi = min(c0 + 64u, total);
           }
+          if (!has) {
+            const uint32_t u = nxt + static_cast<uint32_t>(__popcll(idle & lanes_below()));
+            if (u < hi) {
+              uint32_t lo = 0, up = cl;
+              while (lo < up) {
+                const uint32_t mid = (lo + up) >> 1;
+                if (s_pre[mid] <= u) lo = mid + 1;
+                else up = mid;
+              }
+              const uint32_t ent = lo - 1;   // the last entry starting at or before u
+              const uint32_t r = u - s_pre[ent];
+              const uint32_t nblk = s_nch[ent];
+              const uint32_t cj = r / nblk, k = r - cj * nblk;
+              g = s_g[ent];
+              const uint32_t c = nth_bit(s_m[ent], cj);
+              bit = 1ull << c;
+              const uint32_t fl = s_fl[ent];
+              const uint32_t beta = (fl >> kBetaShift) & 31u;
+              const uint32_t B = 1u << beta;
+              e0 = s_beg[ent] + (k << beta);
+              len = min(s_deg[ent] - (k << beta), B);
+              T = s_T[ent];
+              uskip = len == B ? s_uf[ent] : s_ul[ent];
+              q = s_q[ent];
+              rq = s_rq[ent];
+              uniform = (fl & kRowUniform) != 0;
+              // block k's first element: P^(2 B k) after the row's first
+              Affine E{s_ja[ent], s_jb[ent]};
+              for (uint32_t kk = k, b2 = beta + 1; kk; kk >>= 1, ++b2)
+                if (kk & 1) E = aff_then(E, s_p2[b2]);
+              x = E.a * P.z[64ull * g + c] + E.b;
+              pos = 0;
+              has = true;
+              started = false;
+            }
+          }
+          nxt = min(nxt + static_cast<uint32_t>(__popcll(idle)), hi);
         }
-        nxt = min(nxt + static_cast<uint32_t>(__popcll(idle)), hi);
-      }
-      if (__ballot(has) == 0) {
-        if (nxt >= total) break;
-        continue;
-      }
-      bool live = false;
-      uint32_t src = 0;
-      if (has) {
-        if (!started) {
-          started = true;
-          ++st_draws;
-          // a block without candidates costs one compare
-          if ((x >> 32) + 1 <= uskip) has = false;
+        if (__ballot(has) == 0) {
+          if (nxt >= total) finished = true;
+          continue;
         }
         if (has) {
-          const uint64_t gap = ic_gap_fast(x, q, rq, &S.tab);
-          if (gap >= len - pos) {
-            has = false;
-          } else {
-            pos += static_cast<uint32_t>(gap);
-            const uint32_t e = e0 + pos;
-            const uint64_t a = Pst.a * x + Pst.b;        // acceptance element
-            ++st_cand;
-            if (!uniform) {
-              ++st_draws;
-              st_bytes += 4;
-            }
-            live = uniform || ic_accept(a, T, P.val[e]);
-            ++pos;
-            x = Pst.a * a + Pst.b;                       // next gap element
-            if (pos < len) ++st_draws;
-            else has = false;
-            if (live) {
-              src = P.ids[e];
-              ++st_live;
-              st_bytes += 4;
+          if (!started) {
+            started = true;
+            ++st_draws;
+            // a block without candidates costs one compare
+            if ((x >> 32) + 1 <= uskip) has = false;
+          }
+          if (has) {
+            const uint64_t gap = ic_gap_fast(x, q, rq, &S.tab);
+            if (gap >= len - pos) {
+              has = false;
+            } else {
+              pos += static_cast<uint32_t>(gap);
+              const uint32_t e = e0 + pos;
+              const uint64_t a = Pst.a * x + Pst.b;        // acceptance element
+              ++st_cand;
+              if (!uniform) {
+                ++st_draws;
+                st_bytes += 4;
+              }
+              const bool live = uniform || ic_accept(a, T, P.val[e]);
+              ++pos;
+              x = Pst.a * a + Pst.b;                       // next gap element
+              if (pos < len) ++st_draws;
+              else has = false;
+              if (live) {
+                kg[j] = g;
+                ku[j] = P.ids[e];
+                kb[j] = bit;
+                ++st_live;
+                st_bytes += 4;
+              }
             }
           }
         }
       }
-      grid_discover(P, live, g, src, bit);
+      grid_discover_k<kSparseSteps>(P, kg, ku, kb);
     }
   }
   // Rows pending for many colours (blocks of at most kDenseSpan edges): task
@@ -402,52 +485,86 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
       // a block without candidates costs one compare
       if ((x >> 32) + 1 <= (len == B ? s_uf[ent] : s_ul[ent])) continue;
       unsigned long long *pm = S.pos + t * kDenseSpan;
-      for (uint32_t pos = 0;;) {
-        const uint64_t gap = ic_gap_fast(x, q, rq, &S.tab);
-        if (gap >= len - pos) break;
-        pos += static_cast<uint32_t>(gap);
-        const uint64_t a = Pst.a * x + Pst.b;        // acceptance element
-        ++st_cand;
-        if (!uniform) {
-          ++st_draws;
-          st_bytes += 4;
+      // candidates kDenseBatch at a time: their positions first (arithmetic
+      // only), then their thresholds read together, then the tests
+      uint32_t pos = 0;
+      for (bool more = true; more;) {
+        uint32_t cp[kDenseBatch];
+        uint64_t ca[kDenseBatch];
+        uint32_t nc = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kDenseBatch; ++j) {
+          cp[j] = 0;
+          ca[j] = 0;
+          if (more) {
+            const uint64_t gap = ic_gap_fast(x, q, rq, &S.tab);
+            if (gap >= len - pos) {
+              more = false;
+            } else {
+              pos += static_cast<uint32_t>(gap);
+              const uint64_t a = Pst.a * x + Pst.b;    // acceptance element
+              ++st_cand;
+              if (!uniform) {
+                ++st_draws;
+                st_bytes += 4;
+              }
+              cp[j] = pos;
+              ca[j] = a;
+              nc = j + 1;
+              ++pos;
+              if (pos >= len) {
+                more = false;
+              } else {
+                x = Pst.a * a + Pst.b;                 // next gap element
+                ++st_draws;
+              }
+            }
+          }
         }
-        if (uniform || ic_accept(a, T, P.val[e0 + pos])) {
-          atomicOr(&pm[pos], 1ull << c);
-          ++st_live;
+        uint32_t tv[kDenseBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kDenseBatch; ++j) tv[j] = (!uniform && j < nc) ? P.val[e0 + cp[j]] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < kDenseBatch; ++j) {
+          if (j < nc && (uniform || ic_accept(ca[j], T, tv[j]))) {
+            atomicOr(&pm[cp[j]], 1ull << c);
+            ++st_live;
+          }
         }
-        ++pos;
-        if (pos >= len) break;
-        x = Pst.a * a + Pst.b;                       // next gap element
-        ++st_draws;
       }
     }
     __syncthreads();
-    // phase 2
-    for (uint32_t p0 = 0; p0 < kDenseSlots * kDenseSpan; p0 += BT) {
-      const uint32_t pi = p0 + tid;
+    // phase 2: every thread takes KP positions and discovers them together
+    constexpr int KP = static_cast<int>(kDenseSlots * kDenseSpan / BT);
+    static_assert(KP * BT == kDenseSlots * kDenseSpan, "positions split evenly");
+    uint32_t dg[KP], du[KP];
+    unsigned long long dm[KP];
+#pragma unroll
+    for (int j = 0; j < KP; ++j) {
+      const uint32_t pi = static_cast<uint32_t>(j) * BT + tid;
       const uint32_t t = pi / kDenseSpan, pos = pi % kDenseSpan;
-      unsigned long long m = 0;
-      uint32_t g = 0, u = 0;
+      dm[j] = 0;
+      dg[j] = 0;
+      du[j] = 0;
       if (t < nt) {
-        m = S.pos[pi];
-        if (m) {
+        dm[j] = S.pos[pi];
+        if (dm[j]) {
           S.pos[pi] = 0;
           const uint32_t ent = S.tent[t];
-          g = s_g[ent];
-          u = P.ids[s_beg[ent] + (S.tk[t] << ((s_fl[ent] >> kBetaShift) & 31u)) + pos];
+          dg[j] = s_g[ent];
+          du[j] = P.ids[s_beg[ent] + (S.tk[t] << ((s_fl[ent] >> kBetaShift) & 31u)) + pos];
           st_bytes += 4;
         }
       }
-      grid_discover(P, m != 0, g, u, m);
     }
+    grid_discover_k<KP>(P, dg, du, dm);
     __syncthreads();
   }
 }
 
 // BFS level L: the frontier is buf[L % 2][0, n) with n = ring[L % kRing]
 // (its first cap entries when it overflowed, which raises the overflow
-// flag); every workgroup with a chunk to take pulls chunks of BT items from
+// flag); every workgroup with a chunk to take pulls chunks of kExpItems items from
 // the level's counter.  Levels are enqueued ahead of the host: a level whose
 // frontier is empty costs its launch only.
 template <int BT>
@@ -462,7 +579,7 @@ __global__ void __launch_bounds__(BT) grid_level_kernel(GridParams P, uint32_t L
     P.gctl[kGLevels] += 1;
     if (raw > P.cap) P.gctl[kGOver] = 1;
   }
-  if (static_cast<uint64_t>(blockIdx.x) * BT >= ncur) return;   // no chunk for this workgroup
+  if (static_cast<uint64_t>(blockIdx.x) * kExpItems >= ncur) return;   // no chunk for it
   GridParams Q = P;
   Q.cur = P.buf[L & 1u];
   Q.ncur = ncur;
@@ -475,13 +592,60 @@ __global__ void __launch_bounds__(BT) grid_level_kernel(GridParams P, uint32_t L
   ExpandStats st;
   for (;;) {
     __syncthreads();
-    if (tid == 0) s_chunk = atomicAdd(head, static_cast<unsigned long long>(BT));
+    if (tid == 0) s_chunk = atomicAdd(head, static_cast<unsigned long long>(kExpItems));
     __syncthreads();
     const uint64_t c0 = s_chunk;
     if (c0 >= ncur) break;
-    expand_chunk<BT>(Q, Q.cur, c0, static_cast<uint32_t>(min(static_cast<uint64_t>(BT), ncur - c0)),
+    expand_chunk<BT>(Q, Q.cur, c0,
+                     static_cast<uint32_t>(min(static_cast<uint64_t>(kExpItems), ncur - c0)),
                      S, st);
   }
+  report_stats(P.ctl, st.bytes, st.cand, st.draws, st.exp, st.live);
+}
+
+// Levels L0, L0 + 1, ... by one workgroup while the frontier holds between 1
+// and tail_cap items: each level zeroes the next level's length, expands its
+// items chunk by chunk and meets the next level at a barrier.  Stops at the
+// first level whose frontier is empty or longer than tail_cap and leaves its
+// number in gctl[kGTail]; that level's length stays in the ring for the
+// level kernels.
+template <int BT>
+__global__ void __launch_bounds__(BT) grid_tail_kernel(GridParams P, uint32_t L0,
+                                                       uint64_t tail_cap) {
+  __shared__ ExpandLds<BT> S;
+  __shared__ unsigned long long s_n;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) fill_step_powers(P.step, S.p2);
+  ic_load_tables(&S.tab);
+  for (uint32_t i = tid; i < kDenseSlots * kDenseSpan; i += BT) S.pos[i] = 0;
+  ExpandStats st;
+  uint32_t L = L0;
+  for (;; ++L) {
+    __syncthreads();   // the previous level's appends are done
+    if (tid == 0) {
+      const unsigned long long raw = __hip_atomic_load(&P.ring[L % kRing], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+      s_n = raw;
+      if (raw != 0 && raw <= tail_cap) {
+        __hip_atomic_store(&P.ring[(L + 1) % kRing], 0ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        P.gctl[kGLevels] += 1;
+      }
+    }
+    __syncthreads();
+    const uint64_t ncur = s_n;
+    if (ncur == 0 || ncur > tail_cap) break;
+    GridParams Q = P;
+    Q.cur = P.buf[L & 1u];
+    Q.ncur = ncur;
+    Q.next = P.buf[(L + 1) & 1u];
+    Q.next_cnt = &P.ring[(L + 1) % kRing];
+    for (uint64_t c0 = 0; c0 < ncur; c0 += kExpItems)
+      expand_chunk<BT>(Q, Q.cur, c0,
+                       static_cast<uint32_t>(min(static_cast<uint64_t>(kExpItems), ncur - c0)),
+                       S, st);
+  }
+  if (tid == 0) P.gctl[kGTail] = L;
   report_stats(P.ctl, st.bytes, st.cand, st.draws, st.exp, st.live);
 }
 
@@ -509,7 +673,7 @@ __global__ void grid_rebuild_kernel(GridParams P, uint32_t groups) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; i0 < total; i0 += stride) {
     const uint64_t i = i0 + threadIdx.x;
-    const bool push = i < total && __hip_atomic_load(&P.pend[i], __ATOMIC_RELAXED,
+    const bool push = i < total && __hip_atomic_load(&P.vp[2 * i + 1], __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT) != 0;
     const uint64_t m = __ballot(push);
     if (m == 0) continue;
@@ -560,7 +724,7 @@ constexpr uint32_t kChunk = 2048;   // staging entries per wave of steps 2 and 4
 
 struct OutParams {
   uint32_t n, nw0, nw1;
-  unsigned long long *vis;
+  unsigned long long *vp;
   uint32_t *bm0, *bm1;
   uint32_t g0;                          // first group of this sub-batch
   uint32_t groups;                      // groups of the sub-batch
@@ -636,7 +800,7 @@ __global__ void __launch_bounds__(BT) out_range_write_kernel(OutParams P) {
   const uint32_t tid = threadIdx.x;
   uint32_t *bm0 = P.bm0 + static_cast<uint64_t>(g) * P.nw0;
   uint32_t *bm1 = P.bm1 + static_cast<uint64_t>(g) * P.nw1;
-  unsigned long long *vis = P.vis + static_cast<uint64_t>(g) * P.n;
+  unsigned long long *vp = P.vp + 2 * static_cast<uint64_t>(g) * P.n;
   if (tid == 0) {
     unsigned long long t = 0;
     for (uint32_t q = 0; q < r; ++q) t += P.rc[i * kRanges + q];
@@ -665,7 +829,7 @@ __global__ void __launch_bounds__(BT) out_range_write_kernel(OutParams P) {
       for (uint32_t bb = w0; bb; bb &= bb - 1) {
         const uint32_t v = w0i * 32 + (__ffs(bb) - 1);
         list[p] = v;
-        masks[p] = atomicExch(&vis[v], 0ull);
+        masks[p] = atomicExch(&vp[2ull * v], 0ull);
         ++p;
       }
     }
@@ -874,7 +1038,6 @@ static void grid_ensure(rip_graph *g, uint32_t groups, uint64_t fcap, hipStream_
   const uint32_t n = g->n;
   if (s.n != n || s.groups < groups) {
     dev_free(s.d_vis);
-    dev_free(s.d_pend);
     dev_free(s.d_bm0);
     dev_free(s.d_bm1);
     dev_free(s.d_z);
@@ -882,16 +1045,14 @@ static void grid_ensure(rip_graph *g, uint32_t groups, uint64_t fcap, hipStream_
     dev_free(s.d_nb);
     s.groups = 0;
     const uint64_t nw0 = (n + 31) / 32, nw1 = (n + 1023) / 1024;
-    s.d_vis = dev_alloc<unsigned long long>(static_cast<uint64_t>(groups) * n);
-    s.d_pend = dev_alloc<unsigned long long>(static_cast<uint64_t>(groups) * n);
+    s.d_vis = dev_alloc<unsigned long long>(2ull * groups * n);
     s.d_bm0 = dev_alloc<uint32_t>(groups * nw0);
     s.d_bm1 = dev_alloc<uint32_t>(groups * nw1);
     s.d_z = dev_alloc<unsigned long long>(64ull * groups);
     s.d_sid = dev_alloc<uint32_t>(64ull * groups);
     s.d_nb = dev_alloc<uint32_t>(groups);
     // zero once; every batch leaves them zero again
-    RIP_HIP(hipMemsetAsync(s.d_vis, 0, 8ull * groups * n, st));
-    RIP_HIP(hipMemsetAsync(s.d_pend, 0, 8ull * groups * n, st));
+    RIP_HIP(hipMemsetAsync(s.d_vis, 0, 16ull * groups * n, st));
     RIP_HIP(hipMemsetAsync(s.d_bm0, 0, 4ull * groups * nw0, st));
     RIP_HIP(hipMemsetAsync(s.d_bm1, 0, 4ull * groups * nw1, st));
     s.groups = groups;
@@ -934,7 +1095,6 @@ static const unsigned long long *grid_read(GridScratch &s, const unsigned long l
 
 void grid_scratch_free(GridScratch &s) {
   dev_free(s.d_vis);
-  dev_free(s.d_pend);
   dev_free(s.d_bm0);
   dev_free(s.d_bm1);
   dev_free(s.d_z);
@@ -976,8 +1136,7 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
   P.val = gr->d_val;
   P.n = n;
   P.step = gr->step;
-  P.vis = s.d_vis;
-  P.pend = s.d_pend;
+  P.vp = s.d_vis;
   P.bm0 = s.d_bm0;
   P.bm1 = s.d_bm1;
   P.nw0 = (n + 31) / 32;
@@ -987,6 +1146,7 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
   // keep room for one whole union, the output staging)
   const uint64_t lcap = frontier_cap ? std::min<uint64_t>(frontier_cap, s.fcap) : s.fcap;
   P.cap = lcap;
+  const uint64_t tail_cap = std::min<uint64_t>(kTailItems, lcap);
   P.gctl = s.d_gctl;
   P.ctl = p->d_ctl;
   P.ring = s.d_ring;
@@ -1011,45 +1171,56 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
     uint32_t L = 0;            // the level whose frontier is ring[L % kRing]
     bool dirty = false;        // some vertices with pend bits are in no list
     const uint32_t levels0 = p->grid_levels;
+    auto frontier_len = [&](uint32_t level) {
+      RIP_HIP(hipMemcpyAsync(s.h_ctl + kGWords, s.d_ring + level % kRing,
+                             sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+      RIP_HIP(hipStreamSynchronize(st));
+      return static_cast<uint64_t>(s.h_ctl[kGWords]);
+    };
+    uint64_t pending = frontier_len(L);
     for (;;) {
-      // kLevelsPerSync levels, then one look at the frontier length
-      hipLaunchKernelGGL(grid_ring_kernel, dim3(1), dim3(kLevelsPerSync), 0, st, s.d_ring,
-                         s.d_gctl, L + 1, kLevelsPerSync, 0u);
+      if (pending == 0) {
+        if (!dirty) break;
+        // the vertices the overflowing lists could not hold, as level L
+        P.next = P.buf[L & 1u];
+        P.next_cnt = &s.d_ring[L % kRing];
+        hipLaunchKernelGGL(grid_rebuild_kernel, dim3(2048), dim3(256), 0, st, P, Gb);
+        launches += 1;
+        pending = frontier_len(L);
+        dirty = pending > lcap;
+        if (pending == 0) break;
+        continue;
+      }
       RIP_HIP(hipEventRecord(p->ev3, st));
-      for (uint32_t i = 0; i < kLevelsPerSync; ++i)
-        hipLaunchKernelGGL((grid_level_kernel<kExpBT>), dim3(exp_grid), dim3(kExpBT), 0, st, P,
-                           L + i);
+      if (pending <= tail_cap) {
+        // a thin frontier: one workgroup runs the levels until it widens
+        hipLaunchKernelGGL((grid_tail_kernel<kExpBT>), dim3(1), dim3(kExpBT), 0, st, P, L,
+                           tail_cap);
+        launches += 1;
+      } else {
+        // kLevelsPerSync levels, then one look at the frontier length
+        hipLaunchKernelGGL(grid_ring_kernel, dim3(1), dim3(kLevelsPerSync), 0, st, s.d_ring,
+                           s.d_gctl, L + 1, kLevelsPerSync, 0u);
+        for (uint32_t i = 0; i < kLevelsPerSync; ++i)
+          hipLaunchKernelGGL((grid_level_kernel<kExpBT>), dim3(exp_grid), dim3(kExpBT), 0, st, P,
+                             L + i);
+        launches += 1 + kLevelsPerSync;
+      }
       RIP_HIP(hipGetLastError());
       RIP_HIP(hipEventRecord(p->ev4, st));
-      launches += 1 + kLevelsPerSync;
-      L += kLevelsPerSync;
       const unsigned long long *h = grid_read(s, s.d_gctl, kGWords, st);
-      const bool over = h[kGOver] != 0;
-      const uint32_t lv = static_cast<uint32_t>(h[kGLevels]);
-      uint64_t pending = 0;
-      RIP_HIP(hipMemcpyAsync(s.h_ctl + kGWords, s.d_ring + L % kRing, sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, st));
-      RIP_HIP(hipStreamSynchronize(st));
-      pending = s.h_ctl[kGWords];
+      if (pending <= tail_cap) {
+        L = static_cast<uint32_t>(h[kGTail]);
+      } else {
+        L += kLevelsPerSync;
+        if (h[kGOver] != 0) dirty = true;
+      }
+      p->grid_levels = levels0 + static_cast<uint32_t>(h[kGLevels]);
       float ms = 0.0f;
       RIP_HIP(hipEventElapsedTime(&ms, p->ev3, p->ev4));
       p->grid_ms += ms;
-      p->grid_levels = levels0 + lv;
-      if (over) dirty = true;
+      pending = frontier_len(L);
       if (pending > lcap) dirty = true;
-      if (pending != 0) continue;
-      if (!dirty) break;
-      // the vertices the overflowing lists could not hold, as level L
-      P.next = P.buf[L & 1u];
-      P.next_cnt = &s.d_ring[L % kRing];
-      hipLaunchKernelGGL(grid_rebuild_kernel, dim3(2048), dim3(256), 0, st, P, Gb);
-      launches += 1;
-      RIP_HIP(hipMemcpyAsync(s.h_ctl + kGWords, s.d_ring + L % kRing, sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, st));
-      RIP_HIP(hipStreamSynchronize(st));
-      pending = s.h_ctl[kGWords];
-      dirty = pending > lcap;
-      if (pending == 0) break;
     }
     // output: groups whose unions fit the staging together share a launch
     hipLaunchKernelGGL(grid_union_kernel, dim3(Gb), dim3(256), 0, st, s.d_bm0, P.nw0, s.d_union);
@@ -1060,7 +1231,7 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
     O.n = n;
     O.nw0 = P.nw0;
     O.nw1 = P.nw1;
-    O.vis = s.d_vis;
+    O.vp = s.d_vis;
     O.bm0 = s.d_bm0;
     O.bm1 = s.d_bm1;
     O.slist = reinterpret_cast<uint32_t *>(s.d_fa);

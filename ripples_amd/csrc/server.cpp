@@ -103,9 +103,7 @@ struct rip_server {
 namespace rip {
 namespace {
 
-uint64_t pool_bytes(const rip_pool &p) {
-  return p.cap_data * 4ull + p.cap_sets * 12ull;
-}
+uint64_t pool_bytes(const rip_pool &p) { return pool_device_bytes(&p); }
 
 // The callers below hold the server lock.
 uint64_t resident_bytes(const rip_server *s) {

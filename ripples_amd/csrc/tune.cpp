@@ -3,6 +3,7 @@
 // alike.  The reference picks its GPU batch size by a similar probe
 // (streaming_rrr_generator.h:1150-1241, called at tools/imm.cc:238-244);
 // every setting tried here gives the same sets, so only the time differs.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <map>
@@ -14,6 +15,9 @@
 
 namespace rip {
 namespace {
+
+constexpr uint64_t kProbeBase = 16384;   // sets of the first probe (probe_sets = 0)
+constexpr double kProbeTargetMs = 20.0;    // time per candidate the probe is scaled to
 
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int>, rip_tune> g_tune_cache;
@@ -61,7 +65,6 @@ void pool_clear(rip_pool *p) {
 
 void autotune(rip_graph *g, uint64_t probe, bool force, rip_tune *best, double *best_ms,
               uint32_t *tried) {
-  RIP_REQUIRE(probe > 0, RIP_EINVAL, "autotune: probe_sets must be > 0");
   const auto key = skew_key(g);
   if (!force) {
     std::lock_guard<std::mutex> hold(g_tune_mu);
@@ -78,6 +81,26 @@ void autotune(rip_graph *g, uint64_t probe, bool force, rip_tune *best, double *
   const std::vector<rip_tune> cand = candidates(g->model);
   rip_pool pool;
   pool_create(&pool, g, 0x5eedull);
+  if (probe == 0) {
+    // about 20 ms of sampling per candidate: a probe much shorter than the
+    // device's fill (hundreds of thousands of lanes) would favour settings
+    // that only win on tiny batches
+    try {
+      double t = 0.0;
+      for (int rep = 0; rep < 2; ++rep) {
+        pool_clear(&pool);
+        const auto t0 = std::chrono::steady_clock::now();
+        pool_sample(&pool, kProbeBase, nullptr);
+        t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count();
+      }
+      const double f = std::floor(kProbeTargetMs / std::max(t, 1e-3));
+      probe = kProbeBase * static_cast<uint64_t>(std::min(16.0, std::max(1.0, f)));
+    } catch (...) {
+      pool_destroy(&pool);
+      throw;
+    }
+  }
   double best_t = 0.0;
   rip_tune pick{};
   try {

@@ -13,6 +13,10 @@ the CPU oracle on the same seed:
   - Served queries (one IC and one LT graph per scale 18/20/22, seed = the
     arrival counter) submitted together to the continuous-batching server,
     each against orc_imm.
+  - IC, scale 24, w ~ U[0,0.02]: 120,000 sets drawn into a pool that keeps
+    large colour groups in its column store, 16 sample ids at each of eight
+    offsets against orc_sample, and the column-store coverage of a seed set
+    against the coverage counted on the exported sets.
 The oracle runs on every host thread the environment allows."""
 import os
 
@@ -119,3 +123,28 @@ def test_served_queries_match_oracle_imm(orc):
         assert_imm_equal(seeds, rec, es, erec)
     for g in graphs:
         g.close()
+
+
+def test_ic_uniform_scale24_spread_ids(orc):
+    index, edges, w = rmat_csr(24, R.W_UNIFORM, 0.02)
+    g = R.Graph(index, edges, w, model=R.IC)
+    count = 120000
+    p = g.pool(0)
+    p.generate_rrr_sets(count)
+    groups, hot_n, _ = p.columns()
+    assert groups > 0, "scale 24 U[0,0.02] groups keep their hot members as columns"
+    for first in np.linspace(0, count - 16, 8).astype(np.int64).tolist():
+        off, ver = p.export(first, 16)
+        exp = orc.sample(R.IC, index, edges, w, 0, first, 16, threads=threads())
+        assert_same_sets((off, ver), exp)
+    # coverage of the first 2048 sets through the column store against the
+    # count on their exported lists
+    seeds = np.array([0, 1, 2, 3, 5, 8, 13, 21, 34, 55], np.uint32)
+    num = 2048
+    off, ver = p.export(0, num)
+    member = np.isin(ver, seeds)
+    hits = np.add.reduceat(member.astype(np.int64), off[:-1].astype(np.int64))
+    hits[np.diff(off.astype(np.int64)) == 0] = 0
+    assert p.coverage(seeds, num) == int((hits > 0).sum())
+    p.close()
+    g.close()

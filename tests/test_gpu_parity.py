@@ -185,7 +185,8 @@ def test_selector_graph_replays_across_pool_growth(orc):
                                   dict(sampler=2, colors=64, heavy_degree=64),
                                   dict(out_chunk=1), dict(out_chunk=5, sampler=1),
                                   dict(out_chunk=1 << 20),
-                                  dict(frontier_cap=7), dict(frontier_cap=64, queue_cap=64)])
+                                  dict(frontier_cap=7), dict(frontier_cap=64, queue_cap=64),
+                                  dict(block=512), dict(block=1024, queue_cap=64)])
 def test_ic_tuning_does_not_change_results(tune, orc):
     index, edges, w = rmat_csr(12, R.W_UNIFORM, 0.1, 1)
     g = R.Graph(index, edges, w, model=R.IC)
@@ -456,4 +457,29 @@ def test_pool_coverage(orc):
     with pytest.raises(R.RipplesError, match="out of range"):
         p.coverage([n], 10)
     p.close()
+    g.close()
+
+
+@pytest.mark.parametrize("model", [R.IC, R.LT])
+def test_autotune_keeps_results(model, orc):
+    """rip_autotune times every candidate setting and the graph then samples
+    with the fastest; the sets stay those of the sampling contract, and a
+    graph of the same (model, skew, size) key takes the cached choice."""
+    if model == R.IC:
+        index, edges, w = rmat_csr(12, R.W_UNIFORM, 0.1, 1)
+    else:
+        index, edges, w = rmat_csr(12, R.W_LT_NORM, 0.9, 1)
+    g = R.Graph(index, edges, w, model=model)
+    best, ms, tried = g.autotune(probe_sets=2048, force=True)
+    assert tried == (9 if model == R.IC else 4) and ms > 0
+    if model == R.IC:
+        assert best["queue_cap"] in (64, 256, 1024) and best["block"] in (256, 512, 1024)
+    p = g.pool(5)
+    p.generate_rrr_sets(2500)
+    assert_same_sets(p.export(), orc.sample(model, index, edges, w, 5, 0, 2500))
+    p.close()
+    g2 = R.Graph(index, edges, w, model=model)
+    best2, ms2, tried2 = g2.autotune(probe_sets=2048)
+    assert tried2 == 0 and best2 == best
+    g2.close()
     g.close()
