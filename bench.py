@@ -28,6 +28,10 @@ launch time from HIP events on the pool's stream, against 8 TB/s.  The
 reference's per-edge figures (SURVEY 8d formulas) are reported beside it,
 labelled, never as the fraction.
 
+IC workloads sample with the autotuner's choice (rip_autotune: lane list
+capacity x lane workgroup size, timed on a probe of 2^20 sets up to scale 22,
+of ~20 ms above), reported in `tune`; --no-autotune keeps the defaults.
+
 CPU baselines are the oracle's OpenMP restatements of the reference's CPU
 algorithms (IC: the 64-colour BatchedBFSNeighborColor sweep,
 batched_add_rrrset.h:184-257; LT: one walk per sample), timed on a bounded
@@ -180,7 +184,7 @@ def sampling_steps(pool, batch, steps, warmup, tune):
 
 
 def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, fused_steps,
-            spread_replicas):
+            spread_replicas, run_imm=True, autotune=True):
     wl = WORKLOADS[name]
     index, edges, w = build_graph(wl, seed)
     t = time.time()
@@ -188,17 +192,34 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
     graph_create_s = time.time() - t
     log("%s: rip_graph_create %.2fs (device layout%s)" % (
         name, graph_create_s, ", LT breakpoints and index" if wl["model"] == R.LT else ""))
-    if wl.get("imm_warmup", True):
-        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)   # warm-up
-    else:
+    tune_info = {"given": tune} if tune else None
+    if autotune and not tune and wl["model"] == R.IC:
+        # the graph samples with the autotuner's choice from here on (IMM,
+        # steps); probe: 2^20 sets up to scale 22, else sized to ~20 ms
+        t = time.time()
+        best, probe_ms, tried = g.autotune(probe_sets=(1 << 20) if wl["scale"] <= 22 else 0,
+                                           force=True)
+        tune_info = {"autotuned": best, "probe_ms": probe_ms, "tried": tried,
+                     "seconds": time.time() - t}
+        log("%s autotune: %s (%d settings, %.2f ms per probe, %.1fs)" % (
+            name, best, tried, probe_ms, time.time() - t))
+    if not run_imm:
         warm = g.pool(seed + 1)
         warm.generate_rrr_sets(20000, tune=tune)
         warm.close()
-    t = time.time()
-    seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)
-    imm_wall = time.time() - t
-    log("%s IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
-        name, rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
+        seeds, rec, imm_wall = None, None, None
+    else:
+        if wl.get("imm_warmup", True):
+            seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)   # warm-up
+        else:
+            warm = g.pool(seed + 1)
+            warm.generate_rrr_sets(20000, tune=tune)
+            warm.close()
+        t = time.time()
+        seeds, rec = R.imm(g, wl["k"], wl["eps"], seed=seed, tune=tune)
+        imm_wall = time.time() - t
+        log("%s IMM: theta=%d rounds=%d wall=%.3fs seeds[:5]=%s" % (
+            name, rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
     batch = batch or wl.get("batch") or int(rec["Theta"])
     pool = g.pool(seed)
     wall, kms, calls, kstats, grids = sampling_steps(pool, batch, steps, warmup, tune)
@@ -267,10 +288,13 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
                    "batch_sets": batch, "k": wl["k"], "epsilon": wl["eps"],
                    "parallelism": "single GPU"},
         "imm_wall_s": imm_wall,
-        "imm": {"theta": rec["Theta"], "rounds": rec["Rounds"], "num_seeds": rec["NumSeeds"],
-                "final_f": rec["FinalF"], "estimation_ms": rec["ThetaEstimation"],
-                "generate_ms": rec["GenerateRRRSets"], "select_ms": rec["FindMostInfluentialSet"],
-                "rrr_elements": rec["RRRElements"]},
+        "imm": None if rec is None else {
+            "theta": rec["Theta"], "rounds": rec["Rounds"], "num_seeds": rec["NumSeeds"],
+            "final_f": rec["FinalF"], "estimation_ms": rec["ThetaEstimation"],
+            "generate_ms": rec["GenerateRRRSets"], "select_ms": rec["FindMostInfluentialSet"],
+            "rrr_elements": rec["RRRElements"], "theta_prime": rec["ThetaPrime"],
+            "round_generate_ms": rec["ThetaEstimationGenerateRRR"]},
+        "tune": tune_info,
         "graph_create_s": graph_create_s,
         "sum_indeg_squared": int(np.sum(np.diff(index.astype(np.int64)) ** 2)),
         "pool": {"sets": n_sets, "elements": n_elem, "device_bytes": pool.bytes()},
@@ -282,13 +306,13 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
                                               host["threads_used"])
         log("%s cpu baseline: %.0f sets/s on %d threads" % (
             name, res["cpu_baseline"]["value"], host["threads_used"]))
-    if with_cpu_imm and host is not None:
+    if with_cpu_imm and host is not None and run_imm:
         cw, cseeds, crec = cpu_imm_wall(wl, index, edges, w, seed, host["threads_used"])
         res["cpu_imm"] = {"wall_s": cw, "threads": host["threads_used"], "kind": "port",
                           "seeds_equal_gpu": cseeds.tolist() == seeds.tolist(),
                           "theta": crec.theta}
         log("%s cpu IMM: %.2fs, seeds equal: %s" % (name, cw, res["cpu_imm"]["seeds_equal_gpu"]))
-    if spread_replicas and host is not None:
+    if spread_replicas and host is not None and run_imm:
         res["spread"] = spread_report(g, wl, index, edges, w, seeds, seed, spread_replicas[0],
                                       spread_replicas[1], host["threads_used"])
         log("%s spread of the seeds: device %.1f (%d replicas), oracle %.1f (%d replicas)" % (
@@ -473,7 +497,7 @@ def copy_probe(gib):
     return 0
 
 
-SECONDARY_KEYS = ("value", "ms_per_step", "config", "imm_wall_s", "imm", "graph_create_s",
+SECONDARY_KEYS = ("value", "ms_per_step", "config", "imm_wall_s", "imm", "tune", "graph_create_s",
                   "sum_indeg_squared", "pool", "kernel_counters", "roofline", "cpu_baseline", "cpu_imm", "spread",
                   "plain_wave", "colour_groups")
 
@@ -506,6 +530,11 @@ def main():
                          "shared (every query seed 0)")
     ap.add_argument("--serve-load", default="0.8,2.0",
                     help="Poisson rates as multiples of the plain back-to-back capacity")
+    ap.add_argument("--no-imm", action="store_true",
+                    help="skip the IMM runs (profiling runs of the sampling steps; needs --batch "
+                         "for workloads whose step is theta)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="sample with the default settings instead of the autotuner's choice")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device copy of this many GiB (counter calibration)")
     a = ap.parse_args()
@@ -527,7 +556,7 @@ def main():
     res = measure(a.workload, a.seed, a.steps, a.warmup, a.batch, tune, host, cpu_s,
                   WORKLOADS[a.workload]["scale"] <= 20,
                   a.fused_steps if a.workload == "rmat22_wc" else 0,
-                  (1024, 32) if big else (4096, 64))
+                  (256, 32) if big else (4096, 64), not a.no_imm, not a.no_autotune)
     out = {
         "metric": "RRR sets/s + IMM wall s at fixed k,eps; p50/p99 latency of served IMM queries",
         "value": res["value"],
@@ -542,7 +571,7 @@ def main():
         "dtype": "u32 vertex ids, integer coins (u64 lcg64 draws, fixed-point gaps)",
         "data": "synthetic R-MAT (generated in-process, seed %d)" % a.seed,
         "config": res["config"],
-        "tune": tune,
+        "tune": res["tune"],
         "host": host,
     }
     for k in ("imm_wall_s", "imm", "graph_create_s", "pool", "kernel_counters", "call_counters",
@@ -554,7 +583,8 @@ def main():
     for name in others:
         sec = measure(name, a.seed, max(1, a.steps // 2), 1, 0, None, host, cpu_s,
                       WORKLOADS[name]["scale"] <= 20,
-                      a.fused_steps if name == "rmat22_wc" else 0, None)
+                      a.fused_steps if name == "rmat22_wc" else 0, None, not a.no_imm,
+                      not a.no_autotune)
         out["secondary_" + name] = {k: sec[k] for k in SECONDARY_KEYS if k in sec}
     if a.serve_scales:
         out["serving"] = serve_bench([int(x) for x in a.serve_scales.split(",")],
