@@ -292,7 +292,8 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
             "theta": rec["Theta"], "rounds": rec["Rounds"], "num_seeds": rec["NumSeeds"],
             "final_f": rec["FinalF"], "estimation_ms": rec["ThetaEstimation"],
             "generate_ms": rec["GenerateRRRSets"], "select_ms": rec["FindMostInfluentialSet"],
-            "rrr_elements": rec["RRRElements"], "theta_prime": rec["ThetaPrime"],
+            "rrr_elements": rec["RRRElements"], "pool_device_bytes": rec["PoolBytes"],
+            "theta_prime": rec["ThetaPrime"],
             "round_generate_ms": rec["ThetaEstimationGenerateRRR"]},
         "tune": tune_info,
         "graph_create_s": graph_create_s,

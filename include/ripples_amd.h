@@ -95,6 +95,7 @@ typedef struct {
   double f[64];              /* coverage fraction per estimation round */
   double round_generate_ms[64];  /* ThetaEstimationGenerateRRR */
   double round_select_ms[64];    /* ThetaEstimationMostInfluential */
+  uint64_t pool_bytes;       /* device bytes of the pool at the end (rip_pool_device_bytes) */
 } rip_imm_record;
 
 /* OPIMCExecutionRecord (include/ripples/opimc_execution_record.h) plus the

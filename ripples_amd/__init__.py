@@ -59,6 +59,7 @@ class ImmRecord(C.Structure):
         ("generate_ms", C.c_double), ("select_ms", C.c_double),
         ("theta_prime", C.c_int64 * 64), ("f", C.c_double * 64),
         ("round_generate_ms", C.c_double * 64), ("round_select_ms", C.c_double * 64),
+        ("pool_bytes", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -74,6 +75,7 @@ class ImmRecord(C.Structure):
             "F": list(self.f[:min(r, 64)]),
             "ThetaEstimationGenerateRRR": list(self.round_generate_ms[:min(r, 64)]),
             "ThetaEstimationMostInfluential": list(self.round_select_ms[:min(r, 64)]),
+            "PoolBytes": self.pool_bytes,
         }
 
 

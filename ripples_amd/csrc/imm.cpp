@@ -123,6 +123,7 @@ void imm_run(rip_graph *g, uint32_t k, double eps, uint64_t seed,
   r.num_sets = pool.num_sets;
   r.rrr_elements = pool.num_elements;
   r.num_seeds = got;
+  r.pool_bytes = pool_device_bytes(&pool);
   r.lb = lb;
   r.final_f = theta ? static_cast<double>(cov) / static_cast<double>(theta) : 0.0;
   r.total_ms = ms_since(t_total);

@@ -61,18 +61,18 @@ constexpr uint32_t kSparseSteps = 8;   // steps of a sparse lane between discove
 // level, not a round trip
 constexpr uint32_t kRing = 64;
 constexpr uint32_t kLevelsPerSync = 16;
-// a frontier of at most this many items is expanded by one workgroup that
-// runs level after level with a barrier between them (no launch per level):
-// the long thin tails of deep samples
-constexpr uint64_t kTailItems = 2048;
+// a frontier of at most this many items (one chunk) is expanded by one
+// workgroup that runs level after level with a barrier between them (no
+// launch per level): the long thin tails of deep samples.  Wider frontiers go
+// to the level kernels: one workgroup would take their chunks one by one.
+constexpr uint64_t kTailItems = kExpItems;
 
 // control words of the grid scratch
 enum : int { kGOver = 0, kGLevels = 1, kGCount = 2, kGTail = 3, kGWords = 4 };
 
 struct GridParams {
   const VertexRec *vrec;
-  const uint32_t *ids;
-  const uint32_t *val;
+  const uint2 *iv;                 // (in-neighbour, threshold) of each edge
   uint32_t n;
   Affine step;
   unsigned long long *vp;          // groups x n pairs (visited, pending colours)
@@ -405,14 +405,15 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
                 ++st_draws;
                 st_bytes += 4;
               }
-              const bool live = uniform || ic_accept(a, T, P.val[e]);
+              const uint2 ev = P.iv[e];   // threshold and in-neighbour: one line
+              const bool live = uniform || ic_accept(a, T, ev.y);
               ++pos;
               x = Pst.a * a + Pst.b;                       // next gap element
               if (pos < len) ++st_draws;
               else has = false;
               if (live) {
                 kg[j] = g;
-                ku[j] = P.ids[e];
+                ku[j] = ev.x;
                 kb[j] = bit;
                 ++st_live;
                 st_bytes += 4;
@@ -523,7 +524,8 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
         }
         uint32_t tv[kDenseBatch];
 #pragma unroll
-        for (uint32_t j = 0; j < kDenseBatch; ++j) tv[j] = (!uniform && j < nc) ? P.val[e0 + cp[j]] : 0u;
+        for (uint32_t j = 0; j < kDenseBatch; ++j)
+          tv[j] = (!uniform && j < nc) ? P.iv[e0 + cp[j]].y : 0u;
 #pragma unroll
         for (uint32_t j = 0; j < kDenseBatch; ++j) {
           if (j < nc && (uniform || ic_accept(ca[j], T, tv[j]))) {
@@ -552,7 +554,7 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
           S.pos[pi] = 0;
           const uint32_t ent = S.tent[t];
           dg[j] = s_g[ent];
-          du[j] = P.ids[s_beg[ent] + (S.tk[t] << ((s_fl[ent] >> kBetaShift) & 31u)) + pos];
+          du[j] = P.iv[s_beg[ent] + (S.tk[t] << ((s_fl[ent] >> kBetaShift) & 31u)) + pos].x;
           st_bytes += 4;
         }
       }
@@ -1132,8 +1134,7 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
   const uint32_t n = gr->n;
   GridParams P;
   P.vrec = gr->d_vrec;
-  P.ids = gr->d_ids;
-  P.val = gr->d_val;
+  P.iv = gr->d_iv;
   P.n = n;
   P.step = gr->step;
   P.vp = s.d_vis;

@@ -255,6 +255,7 @@ void step_query(rip_pool *pool, uint64_t n, Query *q) {
   rec.theta = q->need;
   rec.num_sets = q->drawn;
   rec.rrr_elements = pool->num_elements;  // elements of the shared pool
+  rec.pool_bytes = pool_device_bytes(pool);
   rec.num_seeds = got;
   rec.lb = q->lb;
   rec.final_f = q->need ? static_cast<double>(cov) / static_cast<double>(q->need) : 0.0;
