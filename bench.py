@@ -25,7 +25,7 @@ row expanded, 4 per candidate threshold of rows with mixed thresholds, 4 per
 live in-neighbour, 4 per output id; LT breakpoint walk 32 per visited vertex,
 64 per index node or breakpoint block searched, 8 per output id) over its
 launch time from HIP events on the pool's stream, against 8 TB/s.  The
-reference's per-edge figures (SURVEY 8d formulas) are reported beside it,
+reference's per-edge figures (one coin per in-edge, as its samplers draw) are reported beside it,
 labelled, never as the fraction.
 
 IC workloads sample with the autotuner's choice (rip_autotune: lane list
@@ -110,7 +110,7 @@ def lt_search_bytes(stats, n_elem):
 
 
 def ic_reference_bytes(index, off, ver):
-    """SURVEY 8d's per-edge formula: sum over the union of the launch's sets of
+    """The reference's per-edge cost: sum over the union of the launch's sets of
     (8 + 8 indeg) + 4 per output id (every in-edge of every expanded row read
     once, as the reference's per-edge coins need)."""
     u = np.unique(ver)
@@ -259,7 +259,7 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
             off, ver = pool.export(0, 4096)
             roof["reference_per_edge_bytes_first_4096_sets"] = ic_reference_bytes(index, off, ver)
         roof["reference_per_edge_note"] = (
-            "SURVEY 8d: sum over the union of the launch's sets of (8 + 8 indeg) + 4 per "
+            "reference per-edge cost: sum over the union of the launch's sets of (8 + 8 indeg) + 4 per "
             "output id, the bytes of one coin per in-edge; the gap sampler draws only the "
             "candidates, so this figure is not the fraction's numerator")
         del off, ver
@@ -271,7 +271,7 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
                                       "4 per output id")
         roof["reference_subtraction_scan_bytes"] = ks["algo_bytes"]
         roof["reference_subtraction_scan_note"] = (
-            "SURVEY 8d: 8 per visited vertex + 8 per in-edge the reference's subtraction loop "
+            "reference cost: 8 per visited vertex + 8 per in-edge the reference's subtraction loop "
             "scans + 4 per output id; the walk searches precomputed breakpoints instead")
     roof["algo_bytes_per_launch"] = per_launch
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9

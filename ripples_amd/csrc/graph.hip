@@ -6,8 +6,6 @@
 //               before its candidate edges
 //   d_val[e]  : IC integer threshold | LT float weight bits
 //   d_ids[e]  : in-neighbour (read only for live / chosen edges)
-//   d_iv[e]   : IC: (d_ids[e], d_val[e]) in one 8-byte word, so a candidate's
-//               threshold and in-neighbour share a line
 //   d_lrec[v] : LT walk record (common.h), one 128-byte line per vertex
 //   d_bp      : LT rows in blocks of 16 edges, one aligned 128-byte line
 //               each: the 16 row breakpoints of the float chain (lt_scan.h)
@@ -111,15 +109,6 @@ __global__ void build_bp_kernel(const unsigned long long *index, const uint32_t 
 
 // One thread per indexed LT row: the fanout-16 index over its breakpoints,
 // top level first, and a copy of the top level in the walk record.
-// d_iv[e] = (ids[e], val[e]).
-__global__ void build_iv_kernel(const uint32_t *ids, const uint32_t *val, uint64_t count,
-                                uint2 *iv) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < count;
-       e += stride)
-    iv[e] = make_uint2(ids[e], val[e]);
-}
-
 __global__ void build_bidx_kernel(const uint32_t *rows, uint32_t nrows, LtRec *lrec,
                                   const uint32_t *bp, uint32_t *bidx) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,12 +286,6 @@ void graph_create(rip_graph *g, int device, int model, uint32_t n, uint64_t m,
     hipLaunchKernelGGL(build_vrec_kernel, grid, dim3(256), 0, 0, d_index, d_uthr, d_flags,
                        n, g->step, model, g->d_vrec);
     RIP_HIP(hipGetLastError());
-    if (model == RIP_IC) {
-      g->d_iv = dev_alloc<uint2>(m + kEdgePad);
-      hipLaunchKernelGGL(build_iv_kernel, dim3(2048), dim3(256), 0, 0, g->d_ids, g->d_val,
-                         m + kEdgePad, g->d_iv);
-      RIP_HIP(hipGetLastError());
-    }
     if (model == RIP_LT && m > 0) {
       // padded so a search may read 16-byte groups past the last row
       // pair offsets: every row padded to whole blocks of 16 pairs
@@ -360,7 +343,6 @@ void graph_destroy(rip_graph *g) {
   dev_free(g->d_hot_vertex);
   dev_free(g->d_ids);
   dev_free(g->d_val);
-  dev_free(g->d_iv);
   dev_free(g->ic.d_vp);
   dev_free(g->ic.d_bm0);
   dev_free(g->ic.d_bm1);

@@ -168,7 +168,6 @@ struct rip_graph {
   rip::VertexRec *d_vrec = nullptr;  // n records
   uint32_t *d_ids = nullptr;         // m + kEdgePad: in-neighbour of each edge
   uint32_t *d_val = nullptr;         // m + kEdgePad: IC threshold | LT weight bits
-  uint2 *d_iv = nullptr;             // IC, m + kEdgePad: (in-neighbour, threshold) side by side
   rip::LtRec *d_lrec = nullptr;      // LT: n walk records
   uint32_t *d_bp = nullptr;          // LT: (row breakpoint, in-neighbour) pairs, rows
                                      // padded to multiples of 16 pairs

@@ -72,7 +72,8 @@ enum : int { kGOver = 0, kGLevels = 1, kGCount = 2, kGTail = 3, kGWords = 4 };
 
 struct GridParams {
   const VertexRec *vrec;
-  const uint2 *iv;                 // (in-neighbour, threshold) of each edge
+  const uint32_t *ids;
+  const uint32_t *val;
   uint32_t n;
   Affine step;
   unsigned long long *vp;          // groups x n pairs (visited, pending colours)
@@ -405,15 +406,14 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
                 ++st_draws;
                 st_bytes += 4;
               }
-              const uint2 ev = P.iv[e];   // threshold and in-neighbour: one line
-              const bool live = uniform || ic_accept(a, T, ev.y);
+              const bool live = uniform || ic_accept(a, T, P.val[e]);
               ++pos;
               x = Pst.a * a + Pst.b;                       // next gap element
               if (pos < len) ++st_draws;
               else has = false;
               if (live) {
                 kg[j] = g;
-                ku[j] = ev.x;
+                ku[j] = P.ids[e];
                 kb[j] = bit;
                 ++st_live;
                 st_bytes += 4;
@@ -525,7 +525,7 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
         uint32_t tv[kDenseBatch];
 #pragma unroll
         for (uint32_t j = 0; j < kDenseBatch; ++j)
-          tv[j] = (!uniform && j < nc) ? P.iv[e0 + cp[j]].y : 0u;
+          tv[j] = (!uniform && j < nc) ? P.val[e0 + cp[j]] : 0u;
 #pragma unroll
         for (uint32_t j = 0; j < kDenseBatch; ++j) {
           if (j < nc && (uniform || ic_accept(ca[j], T, tv[j]))) {
@@ -554,7 +554,7 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
           S.pos[pi] = 0;
           const uint32_t ent = S.tent[t];
           dg[j] = s_g[ent];
-          du[j] = P.iv[s_beg[ent] + (S.tk[t] << ((s_fl[ent] >> kBetaShift) & 31u)) + pos].x;
+          du[j] = P.ids[s_beg[ent] + (S.tk[t] << ((s_fl[ent] >> kBetaShift) & 31u)) + pos];
           st_bytes += 4;
         }
       }
@@ -1134,7 +1134,8 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
   const uint32_t n = gr->n;
   GridParams P;
   P.vrec = gr->d_vrec;
-  P.iv = gr->d_iv;
+  P.ids = gr->d_ids;
+  P.val = gr->d_val;
   P.n = n;
   P.step = gr->step;
   P.vp = s.d_vis;

@@ -33,7 +33,8 @@ constexpr uint32_t kLaneClaim = 64;     // sample ids a wave claims at a time
 
 struct LaneParams {
   const VertexRec *vrec;
-  const uint2 *iv;                 // (in-neighbour, threshold) of each edge
+  const uint32_t *ids;
+  const uint32_t *val;
   uint32_t n;
   Affine step;
   uint64_t seed;
@@ -223,15 +224,14 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
             ++st_draws;
             st_bytes += 4;
           }
-          const uint2 ev = P.iv[e];     // threshold and in-neighbour: one line
-          const bool live = uniform || ic_accept(a, T, ev.y);
+          const bool live = uniform || ic_accept(a, T, P.val[e]);
           ++pos;
           x = Pst.a * a + Pst.b;                     // next gap element
           if (pos < len) ++st_draws;
           if (live) {
             ++st_live;
             st_bytes += 4;
-            const uint32_t u = ev.x;
+            const uint32_t u = P.ids[e];
             if (lane_insert(table, P.tmask, tag, u)) {
               if (tail == P.cap) {
                 // too large for a lane: the wave sampler redoes it
@@ -403,7 +403,8 @@ uint64_t ic_lane_launch(rip_pool *p, uint64_t first, uint64_t count, const rip_t
   }
   LaneParams P;
   P.vrec = g->d_vrec;
-  P.iv = g->d_iv;
+  P.ids = g->d_ids;
+  P.val = g->d_val;
   P.n = g->n;
   P.step = g->step;
   P.seed = p->seed;
