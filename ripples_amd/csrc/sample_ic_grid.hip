@@ -183,12 +183,17 @@ __device__ inline void grid_discover_k(const GridParams &P, const uint32_t (&g)[
     if (fresh[k] && old[k] == 0)
       o0[k] = atomicOr(&P.bm0[static_cast<uint64_t>(g[k]) * P.nw0 + (u[k] >> 5)], 1u << (u[k] & 31u));
   }
+  // the pend atomics go out with the bitmap ones (neither waits on the
+  // other); the summary bits, which need the bitmap's old word, follow
+  unsigned long long pp[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) pp[k] = fresh[k] ? atomicOr(&P.vp[2 * idx[k] + 1], fresh[k]) : 1ull;
   bool push[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (o0[k] == 0)
       atomicOr(&P.bm1[static_cast<uint64_t>(g[k]) * P.nw1 + (u[k] >> 10)], 1u << ((u[k] >> 5) & 31u));
-    push[k] = fresh[k] && atomicOr(&P.vp[2 * idx[k] + 1], fresh[k]) == 0;
+    push[k] = pp[k] == 0;
   }
   uint64_t m[K];
   uint32_t total = 0;
