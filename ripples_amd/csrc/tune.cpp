@@ -31,13 +31,13 @@ std::tuple<int, int, int> skew_key(const rip_graph *g) {
                          static_cast<int>(std::lround(std::log2(std::max(1.0, double(g->n))))));
 }
 
-// IC: the lane list capacity (how large a sample grows before the grid
+// IC: the lane list capacity 64..4096 (how large a sample grows before the grid
 // sampler takes it) and the lane kernel's workgroup size; LT: the output
 // chunk of the walk kernel's waves.
 std::vector<rip_tune> candidates(int model) {
   std::vector<rip_tune> c;
   if (model == RIP_IC) {
-    for (uint32_t cap : {64u, 256u, 1024u})
+    for (uint32_t cap : {64u, 256u, 1024u, 4096u})
       for (uint32_t block : {256u, 512u, 1024u}) {
         rip_tune t{};
         t.queue_cap = cap;
