@@ -29,12 +29,6 @@ namespace rip {
 namespace {
 
 constexpr uint32_t kLaneSmall = 16;     // sets sorted in registers
-// A sample's first kLaneNear vertices go to the first 2 kLaneNear slots of
-// its lane's table (a few lines that stay near in the cache); the sample
-// that reaches kLaneNear vertices re-inserts them under the table's full
-// mask and goes on there.  Entries left in the near slots are members of the
-// set under the current epoch, so probes pass over them as occupied slots.
-constexpr uint32_t kLaneNear = 32;
 constexpr uint32_t kLaneClaim = 64;     // sample ids a wave claims at a time
 
 struct LaneParams {
@@ -161,7 +155,7 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
           const uint32_t root = uniform_index(z, P.n);
           ++epoch;
           tag = static_cast<unsigned long long>(epoch) << 32;
-          lane_insert(table, 2 * kLaneNear - 1, tag, root);
+          lane_insert(table, P.tmask, tag, root);
           list[0] = root;
           head = 0;
           tail = 1;
@@ -238,8 +232,7 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
             ++st_live;
             st_bytes += 4;
             const uint32_t u = P.ids[e];
-            const uint32_t mask = tail < kLaneNear ? 2 * kLaneNear - 1 : P.tmask;
-            if (lane_insert(table, mask, tag, u)) {
+            if (lane_insert(table, P.tmask, tag, u)) {
               if (tail == P.cap) {
                 // too large for a lane: the wave sampler redoes it
                 const unsigned long long b = atomicAdd(&P.ctl[kCtlLongCount], 1ull);
@@ -247,8 +240,6 @@ __global__ void __launch_bounds__(BT) ic_lane_kernel(LaneParams P) {
                 state = kIdle;
               } else {
                 list[tail++] = u;
-                if (tail == kLaneNear)   // the near slots are full enough: the whole table
-                  for (uint32_t i = 0; i < tail; ++i) lane_insert(table, P.tmask, tag, list[i]);
               }
             }
           }
