@@ -29,8 +29,9 @@ reference's per-edge figures (one coin per in-edge, as its samplers draw) are re
 labelled, never as the fraction.
 
 IC workloads sample with the autotuner's choice (rip_autotune: lane list
-capacity x lane workgroup size, timed on a probe of 2^20 sets up to scale 22,
-of ~20 ms above), reported in `tune`; --no-autotune keeps the defaults.
+capacity x lane workgroup size, timed on a probe of 2^20 sets on the scale-22
+graph and of ~20 ms on the others), reported in `tune`; --no-autotune keeps
+the defaults.
 
 CPU baselines are the oracle's OpenMP restatements of the reference's CPU
 algorithms (IC: the 64-colour BatchedBFSNeighborColor sweep,
@@ -195,9 +196,11 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
     tune_info = {"given": tune} if tune else None
     if autotune and not tune and wl["model"] == R.IC:
         # the graph samples with the autotuner's choice from here on (IMM,
-        # steps); probe: 2^20 sets up to scale 22, else sized to ~20 ms
+        # steps); probe: 2^20 sets on the scale-22 graph, whose steps of
+        # 2.7M small sets a probe of ~20 ms (16K sets) misrepresents; ~20 ms
+        # elsewhere
         t = time.time()
-        best, probe_ms, tried = g.autotune(probe_sets=(1 << 20) if wl["scale"] <= 22 else 0,
+        best, probe_ms, tried = g.autotune(probe_sets=(1 << 20) if wl["scale"] == 22 else 0,
                                            force=True)
         tune_info = {"autotuned": best, "probe_ms": probe_ms, "tried": tried,
                      "seconds": time.time() - t}
