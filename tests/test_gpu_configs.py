@@ -133,6 +133,9 @@ def test_ic_uniform_scale24_spread_ids(orc):
     p.generate_rrr_sets(count)
     groups, hot_n, _ = p.columns()
     assert groups > 0, "scale 24 U[0,0.02] groups keep their hot members as columns"
+    # the column store holds the pool in fewer bytes than its lists would take
+    n_sets, n_elem = p.size()
+    assert n_sets == count and 0 < p.bytes() < 4 * n_elem
     for first in np.linspace(0, count - 16, 8).astype(np.int64).tolist():
         off, ver = p.export(first, 16)
         exp = orc.sample(R.IC, index, edges, w, 0, first, 16, threads=threads())

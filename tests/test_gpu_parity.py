@@ -347,6 +347,7 @@ def test_imm_matches_oracle(model, k, eps, orc):
     # elements of the pool's sets (the LT pool's data also holds unused gaps)
     pool_off = orc.sample(model, index, edges, w, 0, 0, rec["NumSets"])[0]
     assert rec["RRRElements"] == int(pool_off[-1])
+    assert rec["PoolBytes"] >= 12 * rec["NumSets"]   # at least the per-set offsets and lengths
     g.close()
 
 
