@@ -468,7 +468,14 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
     pool_sample(p, count - kSizingPrefix, tune);
     return;
   }
-  if (!tune && p->g->tuned_set) tune = &p->g->tuned;   // the autotuner's choice
+  rip_tune tuned;
+  if (!tune) {   // the autotuner's choice, copied under the lock it is written under
+    std::lock_guard<std::mutex> hold(p->g->sampler_mu);
+    if (p->g->tuned_set) {
+      tuned = p->g->tuned;
+      tune = &tuned;
+    }
+  }
   RIP_HIP(hipSetDevice(p->g->device));
   RIP_REQUIRE(p->num_sets + count < (1ull << 32), -1,
               "pool: at most 2^32-1 samples per stream");

@@ -73,6 +73,7 @@ void autotune(rip_graph *g, uint64_t probe, bool force, rip_tune *best, double *
       *best = it->second;
       if (best_ms) *best_ms = 0.0;
       if (tried) *tried = 0;
+      std::lock_guard<std::mutex> graph_hold(g->sampler_mu);
       g->tuned = it->second;
       g->tuned_set = true;
       return;
@@ -127,8 +128,12 @@ void autotune(rip_graph *g, uint64_t probe, bool force, rip_tune *best, double *
     std::lock_guard<std::mutex> hold(g_tune_mu);
     g_tune_cache[key] = pick;
   }
-  g->tuned = pick;
-  g->tuned_set = true;
+  {
+    // pools of the graph read the choice under the same lock
+    std::lock_guard<std::mutex> hold(g->sampler_mu);
+    g->tuned = pick;
+    g->tuned_set = true;
+  }
   *best = pick;
   if (best_ms) *best_ms = best_t;
   if (tried) *tried = static_cast<uint32_t>(cand.size());
