@@ -51,7 +51,7 @@ typedef struct {
   uint32_t slots;        /* IC: resident colour groups (persistent workgroups) */
   uint32_t queue_cap;    /* IC lane sampler: vertices one lane's sample may
                             reach before the wave sampler takes it over:
-                            64, 256, 1024 or 4096 (0 = 1024) */
+                            64, 256, 1024, 2048 or 4096 (0 = 1024) */
   uint32_t heavy_degree; /* IC: rows this long wait for the hub phase */
   uint32_t sampler;      /* IC: RIP_SAMPLER_AUTO, _WAVE or _COLOUR */
   uint32_t out_chunk;    /* LT and IC wave sampler: pool elements a wave
@@ -163,7 +163,7 @@ int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
                          const uint32_t *vertices);
 /* Autotuner: draws probe_sets samples (0: 16384 times the factor, 1 to 16,
  * that brings one candidate's draw to about 20 ms) with each candidate setting (IC: the
- * lane list capacity 64/256/1024/4096 x the lane kernel's workgroup size
+ * lane list capacity 64/256/1024/2048/4096 x the lane kernel's workgroup size
  * 256/512/1024; LT: the walk's output chunk) and keeps the fastest, which
  * the graph then uses for every sampling call given no tune (rip_pool_sample,
  * rip_imm, the server).  The choice is remembered per (model, degree skew =

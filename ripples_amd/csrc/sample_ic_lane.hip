@@ -333,8 +333,8 @@ constexpr int kLaneBT = 256;
 
 uint32_t ic_lane_cap(const rip_tune *tune) {
   const uint32_t c = tune && tune->queue_cap ? tune->queue_cap : kLaneCapDefault;
-  RIP_REQUIRE(c == 64 || c == 256 || c == 1024 || c == 4096, -1,
-              "tune: queue_cap must be 64, 256, 1024 or 4096");
+  RIP_REQUIRE(c == 64 || c == 256 || c == 1024 || c == 2048 || c == 4096, -1,
+              "tune: queue_cap must be 64, 256, 1024, 2048 or 4096");
   return c;
 }
 
@@ -450,6 +450,10 @@ uint64_t ic_lane_launch(rip_pool *p, uint64_t first, uint64_t count, const rip_t
       break;
     case 1024:
       hipLaunchKernelGGL((ic_lane_sort_kernel<256, 1024>), dim3(1024), dim3(256), 0, p->stream,
+                         w.d_lsort, p->d_ctl, first, p->d_off, p->d_len, p->d_data);
+      break;
+    case 2048:
+      hipLaunchKernelGGL((ic_lane_sort_kernel<256, 2048>), dim3(1024), dim3(256), 0, p->stream,
                          w.d_lsort, p->d_ctl, first, p->d_off, p->d_len, p->d_data);
       break;
     default:   // 4096: 64 KB of LDS per workgroup

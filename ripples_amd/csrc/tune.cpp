@@ -37,7 +37,7 @@ std::tuple<int, int, int> skew_key(const rip_graph *g) {
 std::vector<rip_tune> candidates(int model) {
   std::vector<rip_tune> c;
   if (model == RIP_IC) {
-    for (uint32_t cap : {64u, 256u, 1024u, 4096u})
+    for (uint32_t cap : {64u, 256u, 1024u, 2048u, 4096u})
       for (uint32_t block : {256u, 512u, 1024u}) {
         rip_tune t{};
         t.queue_cap = cap;
