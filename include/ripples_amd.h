@@ -46,7 +46,8 @@ typedef struct rip_pool rip_pool;
 /* Sampler tuning (gpu_batch_size / streaming engine knobs of
  * include/ripples/imm_configuration.h:64-109).  Zero fields = automatic. */
 typedef struct {
-  uint32_t colors;       /* IC: samples fused per colour group, 1..64 */
+  uint32_t colors;       /* IC: samples fused per colour group, 1..64 (grid
+                            sampler and colour sampler; 0 = 64) */
   uint32_t block;        /* IC: threads per workgroup: 256, 512 or 1024 */
   uint32_t slots;        /* IC: resident colour groups (persistent workgroups) */
   uint32_t queue_cap;    /* IC lane sampler: vertices one lane's sample may
@@ -163,7 +164,8 @@ int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
                          const uint32_t *vertices);
 /* Autotuner: draws probe_sets samples (0: 16384 times the factor, 1 to 16,
  * that brings one candidate's draw to about 20 ms) with each candidate setting (IC: the
- * lane list capacity 64/256/1024/2048/4096 x the lane kernel's workgroup size
+ * lane list capacity 64/256/1024/2048/4096 x the lane kernel's workgroup size,
+ * then 32 against 64 samples per colour group of the grid sampler
  * 256/512/1024; LT: the walk's output chunk) and keeps the fastest, which
  * the graph then uses for every sampling call given no tune (rip_pool_sample,
  * rip_imm, the server).  The choice is remembered per (model, degree skew =

@@ -344,9 +344,10 @@ void ic_big_launch(rip_pool *p, uint64_t first, uint64_t count);
 // [0, count), 64 per colour group and many groups per batch, one BFS level
 // of all groups per kernel; with columns, groups with many hot members go to
 // the pool's column store.  frontier_cap limits the entries a level lists
-// (0 = automatic).  Returns the kernel launches made.
+// (0 = automatic); colors (1..64) samples share a group.  Returns the kernel
+// launches made.
 uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
-                        bool columns, uint32_t frontier_cap);
+                        bool columns, uint32_t frontier_cap, uint32_t colors);
 void grid_scratch_free(GridScratch &s);
 
 // sample_ic_lane.hip: one BFS per lane over samples [first, first + count);

@@ -670,7 +670,8 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
     wave_ids = p->g->wave.d_over;
     if (cfg.sampler == RIP_SAMPLER_AUTO)
       return launches + ic_grid_launch(p, first, wave_count, wave_ids, true,
-                                       tune ? tune->frontier_cap : 0u);
+                                       tune ? tune->frontier_cap : 0u,
+                                       tune && tune->colors ? tune->colors : 64u);
   }
   const uint64_t big = ic_wave_launch(p, first, wave_count, wave_ids, cfg.out_chunk, !lanes);
   launches += 2;

@@ -999,20 +999,23 @@ __global__ void __launch_bounds__(64) out_scatter_kernel(OutParams P) {
   }
 }
 
-// Element 0 and sample id of every colour of the batch's groups.
+// Element 0 and sample id of every colour of the batch's groups: group g
+// holds samples b0 + g C + c, c < C (C = colors); colours c >= C have none.
 __global__ void grid_samples_kernel(uint64_t seed, uint64_t first, const uint32_t *ids,
-                                    uint64_t b0, uint64_t count, uint32_t groups,
+                                    uint64_t b0, uint64_t count, uint32_t groups, uint32_t C,
                                     unsigned long long *z, uint32_t *sid, uint32_t *nb) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 64u * groups) return;
-  const uint64_t t = b0 + i;
-  const bool has = t < count;
+  const uint32_t g = i >> 6, c = i & 63u;
+  const uint64_t t0 = b0 + static_cast<uint64_t>(g) * C;   // the group's first sample
+  const uint64_t t = t0 + c;
+  const bool has = c < C && t < count;
   const uint64_t s = has ? first + (ids ? ids[t] : t) : 0;
   z[i] = has ? sample_z(seed, s) : 0ull;
   sid[i] = has ? static_cast<uint32_t>(s) : kNoGroup;
-  if ((i & 63u) == 0) {
-    const uint64_t left = count > t ? count - t : 0;
-    nb[i >> 6] = static_cast<uint32_t>(left < 64 ? left : 64);
+  if (c == 0) {
+    const uint64_t left = count > t0 ? count - t0 : 0;
+    nb[g] = static_cast<uint32_t>(left < C ? left : C);
   }
 }
 
@@ -1024,7 +1027,8 @@ uint64_t group_bytes(uint32_t n) {
 }  // namespace
 
 // Groups per batch and frontier capacity for `count` samples.
-static void grid_plan(const rip_graph *g, uint64_t count, uint32_t *groups, uint64_t *fcap) {
+static void grid_plan(const rip_graph *g, uint64_t count, uint32_t colors, uint32_t *groups,
+                      uint64_t *fcap) {
   size_t free_b = 0, total_b = 0;
   RIP_HIP(hipMemGetInfo(&free_b, &total_b));
   const GridScratch &s = g->grid;
@@ -1033,7 +1037,7 @@ static void grid_plan(const rip_graph *g, uint64_t count, uint32_t *groups, uint
   const uint64_t budget = std::min<uint64_t>((free_b + held) / 8, 32ull << 30);
   const uint64_t per = group_bytes(g->n) + 16ull * std::min<uint64_t>(g->n, 1u << 22);
   uint64_t G = std::max<uint64_t>(1, budget / per);
-  G = std::min<uint64_t>({G, 256, (count + 63) / 64});
+  G = std::min<uint64_t>({G, 256, (count + colors - 1) / colors});
   *groups = static_cast<uint32_t>(G);
   // room for every group's frontier up to 2^22 vertices, and for one whole
   // union (the output staging reuses the frontier buffers)
@@ -1125,16 +1129,18 @@ void grid_scratch_free(GridScratch &s) {
 // grid sampler; with columns, groups with many hot members go to the pool's
 // column store.  Returns the kernel launches made.
 uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
-                        bool columns, uint32_t frontier_cap) {
+                        bool columns, uint32_t frontier_cap, uint32_t colors) {
   if (count == 0) return 0;
+  RIP_REQUIRE(colors >= 1 && colors <= 64, RIP_EINVAL, "tune: colors must be in [1, 64]");
+  const uint32_t C = colors;
   rip_graph *gr = p->g;
   hipStream_t st = p->stream;
   uint32_t G = 0;
   uint64_t fcap = 0;
-  grid_plan(gr, count, &G, &fcap);
+  grid_plan(gr, count, C, &G, &fcap);
   grid_ensure(gr, G, fcap, st);
   GridScratch &s = gr->grid;
-  const uint64_t total_groups = (count + 63) / 64;
+  const uint64_t total_groups = (count + C - 1) / C;
   if (columns) pool_reserve_groups(p, p->groups + static_cast<uint32_t>(total_groups));
   const uint32_t n = gr->n;
   GridParams P;
@@ -1163,10 +1169,10 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
                                    kExpBT);
   const uint32_t exp_grid = static_cast<uint32_t>(per_cu * cu_count());
   uint32_t launches = 0;
-  for (uint64_t b0 = 0; b0 < count; b0 += 64ull * G) {
-    const uint32_t Gb = static_cast<uint32_t>(std::min<uint64_t>(G, (count - b0 + 63) / 64));
+  for (uint64_t b0 = 0; b0 < count; b0 += static_cast<uint64_t>(C) * G) {
+    const uint32_t Gb = static_cast<uint32_t>(std::min<uint64_t>(G, (count - b0 + C - 1) / C));
     hipLaunchKernelGGL(grid_samples_kernel, dim3((64 * Gb + 255) / 256), dim3(256), 0, st,
-                       p->seed, first, ids, b0, count, Gb, s.d_z, s.d_sid, s.d_nb);
+                       p->seed, first, ids, b0, count, Gb, C, s.d_z, s.d_sid, s.d_nb);
     hipLaunchKernelGGL(grid_ring_kernel, dim3(1), dim3(2 * kRing), 0, st, s.d_ring, s.d_gctl, 0u,
                        0u, 1u);
     // level 0: the roots, in buf[0]

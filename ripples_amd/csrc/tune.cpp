@@ -104,6 +104,7 @@ void autotune(rip_graph *g, uint64_t probe, bool force, rip_tune *best, double *
   }
   double best_t = 0.0;
   rip_tune pick{};
+  uint32_t extra = 0;   // settings timed after the lane candidates
   try {
     for (size_t i = 0; i < cand.size(); ++i) {
       double t = 0.0;
@@ -117,6 +118,27 @@ void autotune(rip_graph *g, uint64_t probe, bool force, rip_tune *best, double *
       if (i == 0 || t < best_t) {
         best_t = t;
         pick = cand[i];
+      }
+    }
+    if (g->model == RIP_IC) {
+      // then the samples per colour group of the grid sampler, with the
+      // lane setting chosen
+      for (uint32_t colors : {32u}) {
+        rip_tune c = pick;
+        c.colors = colors;
+        double t = 0.0;
+        for (int rep = 0; rep < 2; ++rep) {
+          pool_clear(&pool);
+          const auto t0 = std::chrono::steady_clock::now();
+          pool_sample(&pool, probe, &c);
+          t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                  .count();
+        }
+        ++extra;
+        if (t < best_t) {
+          best_t = t;
+          pick = c;
+        }
       }
     }
   } catch (...) {
@@ -136,7 +158,7 @@ void autotune(rip_graph *g, uint64_t probe, bool force, rip_tune *best, double *
   }
   *best = pick;
   if (best_ms) *best_ms = best_t;
-  if (tried) *tried = static_cast<uint32_t>(cand.size());
+  if (tried) *tried = static_cast<uint32_t>(cand.size()) + extra;
 }
 
 }  // namespace rip

@@ -188,7 +188,8 @@ def test_selector_graph_replays_across_pool_growth(orc):
                                   dict(frontier_cap=7), dict(frontier_cap=64, queue_cap=64),
                                   dict(block=512), dict(block=1024, queue_cap=64),
                                   dict(queue_cap=4096), dict(queue_cap=4096, sampler=3),
-                                  dict(queue_cap=2048, block=512)])
+                                  dict(queue_cap=2048, block=512), dict(colors=32),
+                                  dict(colors=5, queue_cap=64)])
 def test_ic_tuning_does_not_change_results(tune, orc):
     index, edges, w = rmat_csr(12, R.W_UNIFORM, 0.1, 1)
     g = R.Graph(index, edges, w, model=R.IC)
@@ -474,7 +475,7 @@ def test_autotune_keeps_results(model, orc):
         index, edges, w = rmat_csr(12, R.W_LT_NORM, 0.9, 1)
     g = R.Graph(index, edges, w, model=model)
     best, ms, tried = g.autotune(probe_sets=2048, force=True)
-    assert tried == (15 if model == R.IC else 4) and ms > 0
+    assert tried == (16 if model == R.IC else 4) and ms > 0
     if model == R.IC:
         assert best["queue_cap"] in (64, 256, 1024, 2048, 4096) and best["block"] in (256, 512, 1024)
     p = g.pool(5)
