@@ -61,6 +61,10 @@ typedef struct {
   uint32_t frontier_cap; /* IC grid sampler (AUTO): frontier entries a BFS
                             level may list (0 = automatic); vertices past it
                             are found again by a scan of the pending masks */
+  uint32_t wave_stage;   /* IC AUTO: nonzero = samples passing queue_cap are
+                            first run one per wave, 64 frontier vertices at a
+                            time, up to 4096 vertices, and only the larger
+                            ones go to the grid sampler */
 } rip_tune;
 
 /* IC sampler kinds (all give the same sets):
@@ -165,7 +169,8 @@ int rip_pool_append_host(rip_pool *p, uint64_t count, const uint64_t *offsets,
 /* Autotuner: draws probe_sets samples (0: 16384 times the factor, 1 to 16,
  * that brings one candidate's draw to about 20 ms) with each candidate setting (IC: the
  * lane list capacity 64/256/1024/2048/4096 x the lane kernel's workgroup size,
- * then 32 against 64 samples per colour group of the grid sampler
+ * capacities 64/256/1024 with the wave stage, then 32 against 64 samples per
+ * colour group of the grid sampler
  * 256/512/1024; LT: the walk's output chunk) and keeps the fastest, which
  * the graph then uses for every sampling call given no tune (rip_pool_sample,
  * rip_imm, the server).  The choice is remembered per (model, degree skew =

@@ -41,7 +41,8 @@ class Tune(C.Structure):
     _fields_ = [("colors", C.c_uint32), ("block", C.c_uint32),
                 ("slots", C.c_uint32), ("queue_cap", C.c_uint32),
                 ("heavy_degree", C.c_uint32), ("sampler", C.c_uint32),
-                ("out_chunk", C.c_uint32), ("frontier_cap", C.c_uint32)]
+                ("out_chunk", C.c_uint32), ("frontier_cap", C.c_uint32),
+                ("wave_stage", C.c_uint32)]
 
 
 class LoadOptions(C.Structure):

@@ -332,9 +332,10 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
 // sample_ic_wave.hip: one BFS per wave over samples first + i, i in
 // [0, count), or first + ids[i]; returns how many samples outgrew the wave's
 // list, their indices (relative to first) in g->wave.d_big.  record_events:
-// bracket the kernel with the pool's ev0 / ev1.
+// bracket the kernel with the pool's ev0 / ev1.  vertex_parallel: expand 64
+// queued vertices at a time, a lane each, instead of one row per step.
 uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
-                        uint32_t out_chunk, bool record_events);
+                        uint32_t out_chunk, bool record_events, bool vertex_parallel = false);
 // Most pool elements an IC sampling call of `count` samples can leave
 // unused (the rest of each wave's last output chunk).
 uint64_t ic_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune);

@@ -668,10 +668,21 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
     launches += 3;
     if (wave_count == 0) return launches;
     wave_ids = p->g->wave.d_over;
-    if (cfg.sampler == RIP_SAMPLER_AUTO)
-      return launches + ic_grid_launch(p, first, wave_count, wave_ids, true,
-                                       tune ? tune->frontier_cap : 0u,
-                                       tune && tune->colors ? tune->colors : 64u);
+    if (cfg.sampler == RIP_SAMPLER_AUTO) {
+      const uint32_t frontier_cap = tune ? tune->frontier_cap : 0u;
+      const uint32_t colors = tune && tune->colors ? tune->colors : 64u;
+      if (tune && tune->wave_stage) {
+        // one BFS per wave, 64 frontier vertices at a time, for the samples
+        // up to the wave list; the grid sampler takes the larger ones
+        const uint64_t big = ic_wave_launch(p, first, wave_count, wave_ids, cfg.out_chunk, false,
+                                            true);
+        launches += 2;
+        if (big == 0) return launches;
+        return launches + ic_grid_launch(p, first, big, p->g->wave.d_big, true, frontier_cap,
+                                         colors);
+      }
+      return launches + ic_grid_launch(p, first, wave_count, wave_ids, true, frontier_cap, colors);
+    }
   }
   const uint64_t big = ic_wave_launch(p, first, wave_count, wave_ids, cfg.out_chunk, !lanes);
   launches += 2;

@@ -169,7 +169,106 @@ __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
   return true;
 }
 
-template <int BT>
+// Expands up to 64 queued vertices of the sample whose element 0 is z at
+// once, lane l taking vertex v (when `have`): the lane walks its row's gap
+// blocks one draw per step, as a lane of the lane sampler does; the live
+// edges of a step go through visit() and the first visits are appended to
+// list[tail ...].  Every lane of the wave must call it.  Returns false when
+// the live edges of a step could take the list past cap.  Rows of a few
+// edges (weighted cascade: one candidate per row) keep 64 lanes busy here,
+// where expand_row would give each row a whole wave.
+template <typename Visit>
+__device__ inline bool expand_rows_lanes(const VertexRec *vrec, const uint32_t *ids,
+                                         const uint32_t *val, bool have, uint32_t v, uint64_t z,
+                                         const Affine *p2, const IcLogTables *tab,
+                                         uint32_t *list, uint32_t &tail, uint32_t cap,
+                                         unsigned long long &st_cand,
+                                         unsigned long long &st_draws,
+                                         unsigned long long &st_bytes,
+                                         unsigned long long &st_live, Visit &&visit) {
+  uint32_t T = 0, rbegin = 0, deg = 0, beta = 0, nblk = 0, k = 0, len = 0, pos = 0;
+  bool uniform = true;
+  uint64_t q = 0, rq = 0, ufull = 0, ulast = 0, xb = 0, x = 0;
+  if (have) {
+    const VertexRec rv = vrec[v];
+    T = rv.thr;
+    have = T != 0;                               // no edge of the row can be live
+    if (have) {
+      rbegin = rv.begin;
+      deg = rv.end - rv.begin;
+      uniform = (rv.flags & kRowUniform) != 0;
+      beta = (rv.flags >> kBetaShift) & 31u;
+      q = rv.q;
+      rq = rv.rq;
+      ufull = rv.ufull;
+      ulast = rv.ulast;
+      nblk = ((deg - 1) >> beta) + 1;
+      xb = rv.ja * z + rv.jb;                    // the row's first element
+    }
+  }
+  // block k of the row: its length, and no candidate when one compare says so
+  auto open_block = [&]() {
+    len = min(deg - (k << beta), 1u << beta);
+    x = xb;
+    pos = 0;
+    ++st_draws;
+    if ((xb >> 32) + 1 <= (len == (1u << beta) ? ufull : ulast)) len = 0;
+  };
+  if (have) open_block();
+  const Affine P = p2[0];
+  for (;;) {
+    bool live = false;
+    uint32_t e = 0;
+    if (have) {
+      if (pos >= len) {
+        if (++k < nblk) {
+          const Affine jb = p2[beta + 1];          // 2B elements: the next block
+          xb = aff_apply(jb, xb);
+          open_block();
+        } else {
+          have = false;
+        }
+      } else {
+        const uint64_t gap = ic_gap_fast(x, q, rq, tab);
+        if (gap >= len - pos) {
+          pos = len;
+        } else {
+          pos += static_cast<uint32_t>(gap);
+          e = rbegin + (k << beta) + pos;
+          const uint64_t a = aff_apply(P, x);      // acceptance element
+          ++st_cand;
+          if (!uniform) {
+            ++st_draws;
+            st_bytes += 4;
+          }
+          live = uniform || ic_accept(a, T, val[e]);
+          if (live) {
+            ++st_live;
+            st_bytes += 4;
+          }
+          ++pos;
+          x = aff_apply(P, a);                     // next gap element
+          if (pos < len) ++st_draws;
+        }
+      }
+    }
+    const uint64_t lm = __ballot(live);
+    if (lm) {
+      if (tail + __popcll(lm) > cap) return false;
+      const uint32_t src = live ? ids[e] : 0u;
+      const bool fresh = live && visit(src);
+      const uint64_t fm = __ballot(fresh);
+      if (fresh) list[tail + __popcll(fm & lanes_below())] = src;
+      tail += __popcll(fm);
+    }
+    if (__ballot(have) == 0) break;
+  }
+  return true;
+}
+
+// VP: expand 64 queued vertices at a time (expand_rows_lanes) instead of
+// one row per step over its gap blocks (expand_row, the plain form).
+template <int BT, bool VP>
 __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
   constexpr int NW = BT / 64;
   __shared__ uint32_t s_sort[NW][kSortCap];
@@ -209,12 +308,24 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
     bool big = false;
     while (head < tail && !big) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      const uint32_t v = uniform_u32(list[head++]);
-      ++st_exp;
-      if (!expand_row(P.vrec, P.ids, P.val, v, z, s_p2, &s_tab, list, tail, P.cap, st_cand, st_draws,
-                      st_bytes, st_live,
-                      [&](uint32_t src) { return table_insert(table, tmask, tag, src); }))
-        big = true;
+      if (VP) {
+        const uint32_t take = min(tail - head, 64u);
+        const bool have = lane < take;
+        const uint32_t v = have ? list[head + lane] : 0u;
+        head += take;
+        st_exp += take;
+        if (!expand_rows_lanes(P.vrec, P.ids, P.val, have, v, z, s_p2, &s_tab, list, tail, P.cap,
+                               st_cand, st_draws, st_bytes, st_live,
+                               [&](uint32_t src) { return table_insert(table, tmask, tag, src); }))
+          big = true;
+      } else {
+        const uint32_t v = uniform_u32(list[head++]);
+        ++st_exp;
+        if (!expand_row(P.vrec, P.ids, P.val, v, z, s_p2, &s_tab, list, tail, P.cap, st_cand,
+                        st_draws, st_bytes, st_live,
+                        [&](uint32_t src) { return table_insert(table, tmask, tag, src); }))
+          big = true;
+      }
     }
     if (big) {
       if (lane == 0) {
@@ -409,7 +520,7 @@ __global__ void reset_big_kernel(unsigned long long *ctl) { ctl[kCtlBigCounter] 
 template <int BT>
 uint32_t wave_grid() {
   return static_cast<uint32_t>(
-      blocks_per_cu(reinterpret_cast<const void *>(&ic_wave_kernel<BT>), BT) * cu_count());
+      blocks_per_cu(reinterpret_cast<const void *>(&ic_wave_kernel<BT, false>), BT) * cu_count());
 }
 
 }  // namespace
@@ -448,7 +559,7 @@ __global__ void reset_wave_kernel(unsigned long long *ctl) {
 }
 
 uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
-                        uint32_t out_chunk, bool record_events) {
+                        uint32_t out_chunk, bool record_events, bool vertex_parallel) {
   constexpr int BT = kWaveBT;
   constexpr uint32_t CAP = 4096, TCAP = 8192;
   rip_graph *g = p->g;
@@ -496,7 +607,10 @@ uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
   P.sample_ids = ids;
   hipLaunchKernelGGL(reset_wave_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
   if (record_events) RIP_HIP(hipEventRecord(p->ev0, p->stream));
-  hipLaunchKernelGGL((ic_wave_kernel<BT>), dim3(blocks), dim3(BT), 0, p->stream, P);
+  if (vertex_parallel)
+    hipLaunchKernelGGL((ic_wave_kernel<BT, true>), dim3(blocks), dim3(BT), 0, p->stream, P);
+  else
+    hipLaunchKernelGGL((ic_wave_kernel<BT, false>), dim3(blocks), dim3(BT), 0, p->stream, P);
   RIP_HIP(hipGetLastError());
   if (record_events) RIP_HIP(hipEventRecord(p->ev1, p->stream));
   pool_read_ctl(p, record_events);

@@ -32,8 +32,9 @@ std::tuple<int, int, int> skew_key(const rip_graph *g) {
 }
 
 // IC: the lane list capacity 64..4096 (how large a sample grows before the grid
-// sampler takes it) and the lane kernel's workgroup size; LT: the output
-// chunk of the walk kernel's waves.
+// sampler takes it) and the lane kernel's workgroup size, and three settings
+// with the wave stage between the lanes and the grid; LT: the output chunk of
+// the walk kernel's waves.
 std::vector<rip_tune> candidates(int model) {
   std::vector<rip_tune> c;
   if (model == RIP_IC) {
@@ -44,6 +45,14 @@ std::vector<rip_tune> candidates(int model) {
         t.block = block;
         c.push_back(t);
       }
+    // the wave stage between the lanes and the grid sampler
+    for (uint32_t cap : {64u, 256u, 1024u}) {
+      rip_tune t{};
+      t.queue_cap = cap;
+      t.block = 256;
+      t.wave_stage = 1;
+      c.push_back(t);
+    }
   } else {
     for (uint32_t chunk : {0u, 256u, 1024u, 4096u}) {
       rip_tune t{};

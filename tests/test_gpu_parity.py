@@ -189,7 +189,9 @@ def test_selector_graph_replays_across_pool_growth(orc):
                                   dict(block=512), dict(block=1024, queue_cap=64),
                                   dict(queue_cap=4096), dict(queue_cap=4096, sampler=3),
                                   dict(queue_cap=2048, block=512), dict(colors=32),
-                                  dict(colors=5, queue_cap=64)])
+                                  dict(colors=5, queue_cap=64), dict(wave_stage=1),
+                                  dict(wave_stage=1, queue_cap=64),
+                                  dict(wave_stage=1, queue_cap=256, block=512, colors=9)])
 def test_ic_tuning_does_not_change_results(tune, orc):
     index, edges, w = rmat_csr(12, R.W_UNIFORM, 0.1, 1)
     g = R.Graph(index, edges, w, model=R.IC)
@@ -233,13 +235,14 @@ def test_pool_growth_retry(orc):
     g.close()
 
 
-@pytest.mark.parametrize("sampler", [0, 1, 3])
+@pytest.mark.parametrize("sampler", [0, 1, 3, 4])
 @pytest.mark.parametrize("n", [1500, 3000, 6000, 20000, 70000])
 def test_ic_large_sets(n, sampler, orc):
     """Sets of ~n vertices: LDS sort (<= 1024), HBM sort (<= 4096) and, for
     sets a wave's list cannot hold, the colour groups (AUTO) or the large-set
     kernel (WAVE: bitmap visited, summary walk over several passes of 64
-    summary words at n = 70000)."""
+    summary words at n = 70000).  4: AUTO with the wave stage (lanes up to
+    64 vertices, then one BFS per wave 64 vertices at a time, then the grid)."""
     rng = np.random.default_rng(n)
     s = rng.integers(0, n, 8 * n)
     d = rng.integers(0, n, 8 * n)
@@ -251,7 +254,8 @@ def test_ic_large_sets(n, sampler, orc):
     g = R.Graph(index, edges, w, model=R.IC)
     p = g.pool(4)
     cnt = 300 if n <= 20000 else 80
-    p.generate_rrr_sets(cnt, tune={"sampler": sampler})
+    tune = {"sampler": sampler} if sampler < 4 else {"wave_stage": 1, "queue_cap": 64}
+    p.generate_rrr_sets(cnt, tune=tune)
     got = p.export()
     assert_same_sets(got, orc.sample(R.IC, index, edges, w, 4, 0, cnt))
     assert np.diff(got[0]).max() > min(n - 100, 4096) or n < 4096
@@ -475,7 +479,7 @@ def test_autotune_keeps_results(model, orc):
         index, edges, w = rmat_csr(12, R.W_LT_NORM, 0.9, 1)
     g = R.Graph(index, edges, w, model=model)
     best, ms, tried = g.autotune(probe_sets=2048, force=True)
-    assert tried == (16 if model == R.IC else 4) and ms > 0
+    assert tried == (19 if model == R.IC else 4) and ms > 0
     if model == R.IC:
         assert best["queue_cap"] in (64, 256, 1024, 2048, 4096) and best["block"] in (256, 512, 1024)
     p = g.pool(5)
