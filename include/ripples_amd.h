@@ -71,6 +71,8 @@ typedef struct {
  *   AUTO    one reverse BFS per lane (64 samples per wave); samples passing
  *           queue_cap vertices go to the grid sampler: 64-sample colour groups,
  *           many groups per batch, one BFS level of all of them per kernel
+ *           (with rip_tune.wave_stage, first to one BFS per wave expanding 64
+ *           queued vertices at a time, and to the grid past 4096 vertices)
  *   WAVE    one reverse BFS per wave for every sample (the plain form), sets
  *           passing 4096 vertices finished by a bitmap-visited BFS per wave
  *   COLOUR  64-sample colour groups for every sample
