@@ -171,7 +171,7 @@ def sampling_steps(pool, batch, steps, warmup, tune):
     for _ in range(warmup):
         pool.reset()
         pool.generate_rrr_sets(batch, tune=tune)
-    kms, calls, kstats, grids = [], [], [], []
+    kms, calls, kstats, grids, waves = [], [], [], [], []
     t0 = time.time()
     for _ in range(steps):
         pool.reset()
@@ -180,8 +180,9 @@ def sampling_steps(pool, batch, steps, warmup, tune):
         calls.append(pool.last_call_ms())
         kstats.append(pool.last_kernel_stats())
         grids.append(pool.last_grid())
+        waves.append(pool.last_wave())
     wall = time.time() - t0
-    return wall, kms, calls, kstats, grids
+    return wall, kms, calls, kstats, grids, waves
 
 
 def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, fused_steps,
@@ -225,7 +226,7 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
             name, rec["Theta"], rec["Rounds"], imm_wall, seeds[:5].tolist()))
     batch = batch or wl.get("batch") or int(rec["Theta"])
     pool = g.pool(seed)
-    wall, kms, calls, kstats, grids = sampling_steps(pool, batch, steps, warmup, tune)
+    wall, kms, calls, kstats, grids, waves = sampling_steps(pool, batch, steps, warmup, tune)
     stats = pool.last_stats()
     n_sets, n_elem = pool.size()
     kernel_ms = float(np.mean(kms))
@@ -240,16 +241,23 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
         grid_ms = float(np.mean([gr[0] for gr in grids]))
         grid_levels = float(np.mean([gr[1] for gr in grids]))
         grid_bytes = float(np.mean([gr[2]["algo_bytes"] for gr in grids]))
+        wave_ms = float(np.mean([wv[0] for wv in waves]))
+        wave_bytes = float(np.mean([wv[1]["algo_bytes"] for wv in waves]))
         roof["lane_kernel_ms"] = kernel_ms
+        roof["wave_stage_ms"] = wave_ms
         roof["grid_levels_ms"] = grid_ms
         roof["grid_levels"] = grid_levels
-        if grid_ms > kernel_ms:
-            # the samples that outgrew a lane dominate: the grid sampler's
-            # level kernels are the dominant kernel, timed over all levels
-            roof["kernel"] = "grid_level_kernel (all BFS levels of the call)"
-            per_launch = grid_bytes
-            kernel_ms = grid_ms
-            roof["kernel_ms"] = grid_ms
+        # the dominant kernel of the call: the lane kernel, the wave stage's
+        # kernel (samples past the lane capacity) or the grid sampler's level
+        # kernels (timed over all levels)
+        stage_ms, stage_kernel, stage_bytes = max(
+            (kernel_ms, roof["kernel"], per_launch),
+            (wave_ms, "ic_wave_kernel (vertex-parallel wave stage)", wave_bytes),
+            (grid_ms, "grid_level_kernel (all BFS levels of the call)", grid_bytes))
+        roof["kernel"] = stage_kernel
+        per_launch = stage_bytes
+        kernel_ms = stage_ms
+        roof["kernel_ms"] = stage_ms
         roof["algo_bytes_formula"] = ("64 per row expanded (vertex record) + 4 per candidate "
                                       "edge of a row with mixed thresholds + 4 per live edge "
                                       "(in-neighbour) + 4 per output id, counted on the device")
@@ -328,7 +336,7 @@ def measure(name, seed, steps, warmup, batch, tune, host, cpu_s, with_cpu_imm, f
         for label, kind in (("plain_wave", 1), ("colour_groups", 2)):
             tz = dict(tune or {})
             tz["sampler"] = kind
-            pw, pk, _, _, _ = sampling_steps(pool, batch, fused_steps, 1, tz)
+            pw, pk, _, _, _, _ = sampling_steps(pool, batch, fused_steps, 1, tz)
             res[label] = {"value": batch * fused_steps / pw, "ms_per_step": pw / fused_steps * 1e3,
                           "kernel_ms": float(np.mean(pk))}
             res[label]["batched_lanes_over_this"] = res["value"] / res[label]["value"]

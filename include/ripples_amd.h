@@ -250,6 +250,11 @@ int rip_pool_last_kernel_ms(const rip_pool *p, double *ms, uint32_t *launches);
  * order, as rip_pool_last_stats) into stats[0 .. count). */
 int rip_pool_last_grid(const rip_pool *p, double *ms, uint32_t *levels, uint64_t *stats,
                        uint32_t count);
+/* The wave stage's share of the last rip_pool_sample call (IC, AUTO with
+ * rip_tune.wave_stage): the device time of its kernel (HIP events on the
+ * pool's stream, 0 when it did not run) and its work counters (RIP_STAT_*
+ * order) into stats[0 .. count).  No counterpart call in the reference. */
+int rip_pool_last_wave(const rip_pool *p, double *ms, uint64_t *stats, uint32_t count);
 /* Device time from the start of the last rip_pool_sample call's main kernel
  * to the end of its last kernel (fallback kernels, sorts and the host round
  * trips between them included), measured with HIP events. */

@@ -143,6 +143,7 @@ def lib():
                                C.POINTER(C.c_uint32)]
     L.rip_pool_last_grid.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), _u64p,
                                      C.c_uint32]
+    L.rip_pool_last_wave.argtypes = [vp, C.POINTER(C.c_double), _u64p, C.c_uint32]
     L.rip_pool_device_bytes.argtypes = [vp, _u64p]
     L.rip_pool_columns.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint64)]
@@ -453,6 +454,15 @@ class Pool:
         _check(lib().rip_pool_last_grid(self._h, C.byref(ms), C.byref(lv), st, 5))
         keys = ("algo_bytes", "edges", "coins", "expansions", "levels")
         return ms.value, lv.value, {k: int(v) for k, v in zip(keys, st)}
+
+    def last_wave(self):
+        """(ms of the wave stage's kernel, its work counters) for the last
+        sampling call; 0 and zeros when the stage did not run."""
+        ms = C.c_double(0)
+        st = np.zeros(5, np.uint64)
+        _check(lib().rip_pool_last_wave(self._h, C.byref(ms), st, 5))
+        keys = ("algo_bytes", "edges", "coins", "expansions", "levels")
+        return ms.value, {k: int(v) for k, v in zip(keys, st)}
 
     def last_call_ms(self):
         """Device time of the whole last sampling call (every kernel)."""

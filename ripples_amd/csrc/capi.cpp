@@ -237,8 +237,19 @@ int rip_pool_last_grid(const rip_pool *p, double *ms, uint32_t *levels, uint64_t
     RIP_REQUIRE(count == 0 || stats, RIP_EINVAL, "stats is NULL");
     if (ms) *ms = p->grid_ms;
     if (levels) *levels = p->grid_levels;
+    const uint64_t *before = p->wave_ran ? p->wave_stats : p->main_stats;
     for (uint32_t i = 0; i < count; ++i)
-      stats[i] = i < RIP_STAT_COUNT && p->grid_levels ? p->last_stats[i] - p->main_stats[i] : 0;
+      stats[i] = i < RIP_STAT_COUNT && p->grid_levels ? p->last_stats[i] - before[i] : 0;
+  });
+}
+
+int rip_pool_last_wave(const rip_pool *p, double *ms, uint64_t *stats, uint32_t count) {
+  return guarded([&] {
+    RIP_REQUIRE(p, RIP_EINVAL, "pool is NULL");
+    RIP_REQUIRE(count == 0 || stats, RIP_EINVAL, "stats is NULL");
+    if (ms) *ms = p->wave_ran ? p->wave_ms : 0.0;
+    for (uint32_t i = 0; i < count; ++i)
+      stats[i] = i < RIP_STAT_COUNT && p->wave_ran ? p->wave_stats[i] - p->main_stats[i] : 0;
   });
 }
 

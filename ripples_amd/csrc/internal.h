@@ -227,6 +227,9 @@ struct rip_pool {
   hipEvent_t ev3 = nullptr, ev4 = nullptr;  // around each grid-sampler level
   double grid_ms = 0.0;       // last sampling call: the grid sampler's level kernels
   uint32_t grid_levels = 0;   // and how many ran
+  double wave_ms = 0.0;       // last sampling call: the wave stage's kernel (AUTO)
+  bool wave_ran = false;
+  uint64_t wave_stats[RIP_STAT_COUNT] = {};  // last_stats as they stood after it
   double last_kernel_ms = 0.0;
   double last_call_ms = 0.0;
   uint32_t last_launches = 0;
@@ -334,8 +337,10 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
 // list, their indices (relative to first) in g->wave.d_big.  record_events:
 // bracket the kernel with the pool's ev0 / ev1.  vertex_parallel: expand 64
 // queued vertices at a time, a lane each, instead of one row per step.
+// stage: the wave stage of AUTO, timed into p->wave_ms / p->wave_stats.
 uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
-                        uint32_t out_chunk, bool record_events, bool vertex_parallel = false);
+                        uint32_t out_chunk, bool record_events, bool vertex_parallel = false,
+                        bool stage = false);
 // Most pool elements an IC sampling call of `count` samples can leave
 // unused (the rest of each wave's last output chunk).
 uint64_t ic_output_slack(const rip_pool *p, uint64_t count, const rip_tune *tune);

@@ -494,6 +494,8 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
     RIP_REQUIRE(attempt < 40, -2, "pool: data capacity did not converge");
     p->grid_ms = 0.0;     // timings of the attempt that succeeds
     p->grid_levels = 0;
+    p->wave_ms = 0.0;
+    p->wave_ran = false;
     hipLaunchKernelGGL(set_ctl_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl,
                        static_cast<unsigned long long>(p->used_data));
     pool_clear_gref(p, first, first + count);

@@ -675,7 +675,7 @@ uint32_t ic_sample(rip_pool *p, uint64_t first, uint64_t count, const IcLaunch &
         // one BFS per wave, 64 frontier vertices at a time, for the samples
         // up to the wave list; the grid sampler takes the larger ones
         const uint64_t big = ic_wave_launch(p, first, wave_count, wave_ids, cfg.out_chunk, false,
-                                            true);
+                                            true, true);
         launches += 2;
         if (big == 0) return launches;
         return launches + ic_grid_launch(p, first, big, p->g->wave.d_big, true, frontier_cap,

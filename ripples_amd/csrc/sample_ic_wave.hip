@@ -559,7 +559,8 @@ __global__ void reset_wave_kernel(unsigned long long *ctl) {
 }
 
 uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint32_t *ids,
-                        uint32_t out_chunk, bool record_events, bool vertex_parallel) {
+                        uint32_t out_chunk, bool record_events, bool vertex_parallel,
+                        bool stage) {
   constexpr int BT = kWaveBT;
   constexpr uint32_t CAP = 4096, TCAP = 8192;
   rip_graph *g = p->g;
@@ -607,13 +608,22 @@ uint64_t ic_wave_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
   P.sample_ids = ids;
   hipLaunchKernelGGL(reset_wave_kernel, dim3(1), dim3(1), 0, p->stream, p->d_ctl);
   if (record_events) RIP_HIP(hipEventRecord(p->ev0, p->stream));
+  if (stage) RIP_HIP(hipEventRecord(p->ev3, p->stream));
   if (vertex_parallel)
     hipLaunchKernelGGL((ic_wave_kernel<BT, true>), dim3(blocks), dim3(BT), 0, p->stream, P);
   else
     hipLaunchKernelGGL((ic_wave_kernel<BT, false>), dim3(blocks), dim3(BT), 0, p->stream, P);
   RIP_HIP(hipGetLastError());
   if (record_events) RIP_HIP(hipEventRecord(p->ev1, p->stream));
+  if (stage) RIP_HIP(hipEventRecord(p->ev4, p->stream));
   pool_read_ctl(p, record_events);
+  if (stage) {
+    float ms = 0.0f;
+    RIP_HIP(hipEventElapsedTime(&ms, p->ev3, p->ev4));
+    p->wave_ms = ms;
+    p->wave_ran = true;
+    for (int i = 0; i < RIP_STAT_COUNT; ++i) p->wave_stats[i] = p->h_ctl[kCtlStatBytes + i];
+  }
   return p->h_ctl[kCtlLongCount];
 }
 
