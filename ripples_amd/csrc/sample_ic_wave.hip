@@ -14,9 +14,10 @@
 //   list[CAP]    BFS queue and output list (FIFO in discovery order)
 // Expansion of a vertex: lane l draws the candidates of the row's gap
 // blocks l, l + 64, ... (one compare for a full block without candidates);
-// only live edges read their in-neighbour and touch the table.  The
-// finished list is sorted in LDS (bitonic, 1024 entries per wave) or, if
-// longer, in its HBM buffer.  Samples whose sets outgrow CAP are listed for
+// only live edges read their in-neighbour and touch the table (in the
+// vertex-parallel form, an LDS table first).  The finished list is sorted in
+// LDS (bitonic, 1024 entries per wave, 2048 in the vertex-parallel form) or,
+// if longer, in its HBM buffer.  Samples whose sets outgrow CAP are listed for
 // the large-set kernels.
 #include <algorithm>
 
@@ -29,6 +30,14 @@ namespace rip {
 namespace {
 
 constexpr uint32_t kSortCap = 1024;
+// Vertex-parallel form: the first kWaveLdsLimit vertices of a sample are
+// marked in a per-wave LDS table of kWaveLdsSlots 32-bit slots (vertex + 1,
+// 0 empty), cleared per sample; only later ones go to the wave's HBM table.
+// Most sets of this stage end below the limit, so most of its visited checks
+// never leave the CU.  The slots double as the wave's sort buffer once the
+// BFS is done.
+constexpr uint32_t kWaveLdsSlots = 2048;
+constexpr uint32_t kWaveLdsLimit = kWaveLdsSlots / 2;
 
 struct WaveParams {
   const VertexRec *vrec;
@@ -71,6 +80,34 @@ __device__ inline bool table_insert(unsigned long long *table, uint32_t mask,
       if (static_cast<uint32_t>(cur) == v) return false;
       break;  // occupied by another vertex of this sample
     }
+    h = (h + 1) & mask;
+  }
+}
+
+// Insert v into a wave's LDS table (power-of-two slots, fewer entries than
+// slots); true if it was not there yet.
+__device__ inline bool lds_insert(uint32_t *t, uint32_t mask, uint32_t v) {
+  const uint32_t want = v + 1u;
+  uint32_t h = hash_vertex(v) & mask;
+  for (;;) {
+    uint32_t cur = __hip_atomic_load(&t[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if (cur == 0) {
+      cur = atomicCAS(&t[h], 0u, want);
+      if (cur == 0) return true;
+    }
+    if (cur == want) return false;
+    h = (h + 1) & mask;
+  }
+}
+
+// Whether v is in a wave's LDS table.
+__device__ inline bool lds_find(const uint32_t *t, uint32_t mask, uint32_t v) {
+  const uint32_t want = v + 1u;
+  uint32_t h = hash_vertex(v) & mask;
+  for (;;) {
+    const uint32_t cur = __hip_atomic_load(&t[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if (cur == want) return true;
+    if (cur == 0) return false;
     h = (h + 1) & mask;
   }
 }
@@ -267,11 +304,13 @@ __device__ inline bool expand_rows_lanes(const VertexRec *vrec, const uint32_t *
 }
 
 // VP: expand 64 queued vertices at a time (expand_rows_lanes) instead of
-// one row per step over its gap blocks (expand_row, the plain form).
+// one row per step over its gap blocks (expand_row, the plain form), with
+// the first vertices of each sample marked in LDS.
 template <int BT, bool VP>
 __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
   constexpr int NW = BT / 64;
-  __shared__ uint32_t s_sort[NW][kSortCap];
+  constexpr uint32_t kBuf = VP && kWaveLdsSlots > kSortCap ? kWaveLdsSlots : kSortCap;
+  __shared__ uint32_t s_sort[NW][kBuf];
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x >> 6;
   const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * NW + wid;
@@ -300,12 +339,29 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
     const unsigned long long tag = static_cast<unsigned long long>(epoch) << 32;
     const uint64_t z = sample_z(P.seed, s);
     const uint32_t root = uniform_index(z, P.n);
+    uint32_t *ltab = s_sort[wid];
+    if (VP) {
+      for (uint32_t i = 4 * lane; i < kWaveLdsSlots; i += 256)
+        *reinterpret_cast<uint4 *>(&ltab[i]) = make_uint4(0, 0, 0, 0);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
     if (lane == 0) {
-      table_insert(table, tmask, tag, root);
+      if (VP)
+        lds_insert(ltab, kWaveLdsSlots - 1, root);
+      else
+        table_insert(table, tmask, tag, root);
       list[0] = root;
     }
     uint32_t head = 0, tail = 1;
     bool big = false;
+    // a visit call adds at most 64 vertices: while the sample stays within
+    // the LDS limit they go to the LDS table, then to the HBM table (after a
+    // look in the LDS one)
+    auto visit_vp = [&](uint32_t src) {
+      if (tail + 64u <= kWaveLdsLimit) return lds_insert(ltab, kWaveLdsSlots - 1, src);
+      if (lds_find(ltab, kWaveLdsSlots - 1, src)) return false;
+      return table_insert(table, tmask, tag, src);
+    };
     while (head < tail && !big) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       if (VP) {
@@ -315,8 +371,7 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
         head += take;
         st_exp += take;
         if (!expand_rows_lanes(P.vrec, P.ids, P.val, have, v, z, s_p2, &s_tab, list, tail, P.cap,
-                               st_cand, st_draws, st_bytes, st_live,
-                               [&](uint32_t src) { return table_insert(table, tmask, tag, src); }))
+                               st_cand, st_draws, st_bytes, st_live, visit_vp))
           big = true;
       } else {
         const uint32_t v = uniform_u32(list[head++]);
@@ -340,7 +395,7 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
     while (cap2 < len) cap2 <<= 1;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const uint32_t *sorted;
-    if (cap2 <= kSortCap) {
+    if (cap2 <= kBuf) {
       uint32_t *a = s_sort[wid];
       for (uint32_t i = lane; i < len; i += 64) a[i] = list[i];
       wave_bitonic(a, len, cap2);
