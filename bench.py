@@ -369,11 +369,13 @@ SERVE_EPS = (0.5, 0.2, 0.13)
 SERVE_IC_UNIFORM_MAX_SCALE = 18
 
 
-def serve_graphs(scales, seed):
+def serve_graphs(scales, seed, autotune=True):
     """Resident graphs of the serving workload: R-MAT at each scale, one IC
     graph (U[0,0.1] up to SERVE_IC_UNIFORM_MAX_SCALE, weighted cascade above)
-    and one LT graph (in-weights normalised to 0.9)."""
-    out = []
+    and one LT graph (in-weights normalised to 0.9).  With autotune, each IC
+    graph samples with the autotuner's choice, in plain and served solves
+    alike; returns the graphs and those choices."""
+    out, tuned = [], {}
     for sc in scales:
         t = time.time()
         s, d = R.rmat_edges(sc)
@@ -382,9 +384,16 @@ def serve_graphs(scales, seed):
             (R.W_CASCADE, 1.0, "wc")
         for model, scheme, ws, tag in ((R.IC,) + ic, (R.LT, R.W_LT_NORM, 0.9, "lt")):
             index, edges, w = R.csr_from_edges(n, s, d, scheme, ws, seed)
-            out.append(("rmat%d_%s" % (sc, tag), R.Graph(index, edges, w, model=model)))
+            name = "rmat%d_%s" % (sc, tag)
+            g = R.Graph(index, edges, w, model=model)
+            if autotune and model == R.IC:
+                best, _, _ = g.autotune(probe_sets=(1 << 20) if sc == 22 else 0)
+                tuned[name] = best
+            out.append((name, g))
         log("serving graphs scale %d built in %.1fs" % (sc, time.time() - t))
-    return out
+    if tuned:
+        log("serving graphs autotuned: %s" % tuned)
+    return out, tuned
 
 
 def fifo_latencies(arrivals, service):
@@ -465,21 +474,22 @@ def serve_traffic(graphs, queries, rng, slots, workers, load, budget):
     return out
 
 
-def serve_bench(scales, n_queries, seed, slots, workers, load, budget, modes):
+def serve_bench(scales, n_queries, seed, slots, workers, load, budget, modes, autotune=True):
     """Continuous-batching serving against back-to-back plain solves.
     Queries: graph uniform over the resident graphs, k in {10,50,100,200},
     eps in {0.5,0.2,0.13}, Poisson arrivals.  Seed modes: "arrival" gives
     query i the sampling seed i (the served workload's seeding); "shared"
     gives every query seed 0, the fixed seeding of the reference's imm tool,
     so queries on one graph reuse each other's RRR sets."""
-    graphs = serve_graphs(scales, seed)
+    graphs, tuned = serve_graphs(scales, seed, autotune)
     rng = np.random.default_rng(seed + 1)
     mix = [(int(rng.integers(len(graphs))), int(rng.choice(SERVE_K)),
             float(rng.choice(SERVE_EPS))) for _ in range(n_queries)]
     for gi in range(len(graphs)):      # first-touch costs outside the timing
         R.imm(graphs[gi][1], 10, 0.5, seed=10 ** 6)
     out = {"graphs": [name for name, _ in graphs], "k": list(SERVE_K), "eps": list(SERVE_EPS),
-           "queries": n_queries, "slots": slots, "workers": workers, "budget_bytes": budget}
+           "queries": n_queries, "slots": slots, "workers": workers, "budget_bytes": budget,
+           "tune": tuned or None}
     for mode in modes:
         log("serving traffic with %s seeds" % mode)
         queries = [(gi, k, eps, i if mode == "arrival" else 0) for i, (gi, k, eps) in
@@ -603,7 +613,7 @@ def main():
                                      a.serve_queries, a.seed, a.serve_slots, a.serve_workers,
                                      [float(x) for x in a.serve_load.split(",")],
                                      int(a.serve_budget_gib * (1 << 30)),
-                                     a.serve_seeds.split(","))
+                                     a.serve_seeds.split(","), not a.no_autotune)
     print(json.dumps(out), flush=True)
     return 0
 
