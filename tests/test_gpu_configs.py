@@ -87,6 +87,15 @@ def test_ic_cascade_scale22_spread_ids_and_selection(orc):
     assert got.tolist() == es.tolist() and cov == ec
     # rip_imm's final selection is the same greedy on the same theta sets
     assert seeds.tolist() == es.tolist() and rec["FinalF"] == ec / theta
+    # the setting the autotuner picks on this graph (lanes up to 64 vertices,
+    # the wave stage with its LDS table, 32 colours per grid group) draws
+    # the same theta sets
+    q = g.pool(0)
+    q.generate_rrr_sets(theta, tune={"queue_cap": 64, "wave_stage": 1, "colors": 32})
+    off2, ver2 = q.export()
+    assert np.array_equal(off2, off) and np.array_equal(ver2, ver)
+    assert q.last_wave()[0] > 0
+    q.close()
     p.close()
     g.close()
 
