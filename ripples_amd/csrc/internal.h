@@ -134,6 +134,14 @@ struct SelectScratch {
   unsigned long long *d_inv_start = nullptr;  // n + 1
   unsigned long long *d_cursor = nullptr;     // n
   uint32_t *d_inv = nullptr;           // total elements: set ids by vertex
+  // sparse pools: the (vertex, set) pairs of the sets, sorted by vertex into
+  // the inverted index (keys in two buffers, set ids in d_inv and d_pval)
+  uint64_t cap_pairs = 0;
+  uint32_t *d_pkey[2] = {nullptr, nullptr};
+  uint32_t *d_pval = nullptr;
+  unsigned long long *d_soff = nullptr;  // cap_sets + 1: each set's first pair
+  void *d_stemp = nullptr;               // the sort's temporary storage
+  size_t stemp_bytes = 0;
   uint32_t *d_kill = nullptr;          // cap_kill: dense pools' kill list
   uint64_t cap_kill = 0;
   uint8_t *d_covered = nullptr;        // cap_sets
@@ -386,6 +394,11 @@ uint64_t coverage_run_noalloc(rip_pool *p, uint64_t num_sets, const uint32_t *se
                               uint32_t num_seeds);
 
 // scan.hip: exclusive scan of uint32 values into uint64 prefixes (n + 1 out)
+// select_sort.hip: sorts `count` (key, value) pairs by the low `bits` bits
+// of the key in the double buffers (k0, k1), (v0, v1); returns the buffer
+// holding the result (0 or 1).  *temp grows to the sort's temporary storage.
+int sort_pairs_u32(uint32_t *k0, uint32_t *k1, uint32_t *v0, uint32_t *v1, uint64_t count,
+                   uint32_t bits, void **temp, size_t *temp_bytes, hipStream_t st);
 void exclusive_scan_u32(const uint32_t *d_in, unsigned long long *d_out,
                         uint64_t n, hipStream_t stream);
 

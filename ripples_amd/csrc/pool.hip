@@ -262,6 +262,11 @@ void pool_destroy(rip_pool *p) {
   dev_free(p->sel.d_inv_start);
   dev_free(p->sel.d_cursor);
   dev_free(p->sel.d_inv);
+  dev_free(p->sel.d_pkey[0]);
+  dev_free(p->sel.d_pkey[1]);
+  dev_free(p->sel.d_pval);
+  dev_free(p->sel.d_soff);
+  dev_free(p->sel.d_stemp);
   dev_free(p->sel.d_kill);
   dev_free(p->sel.d_covered);
   dev_free(p->sel.d_keys);

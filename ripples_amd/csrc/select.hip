@@ -93,6 +93,67 @@ fill_inverted_kernel(const unsigned long long *set_off, const uint32_t *set_len,
                    });
 }
 
+// ------------------------------------------------------------ sorted index
+// Sparse pools of up to kSortMaxPairs elements build the inverted index by
+// sorting their (vertex, set) pairs by vertex: no atomic per element, where
+// the count and fill passes above make one each (and the hub vertices of
+// R-MAT graphs share their counters).
+constexpr uint64_t kSortMaxPairs = 1ull << 31;
+
+// The pairs of sets [0, num_sets) in set order: pair soff[s] + j is
+// (member j of s, s).  A wave takes 64 consecutive sets, whose pairs are
+// consecutive from soff of the first.
+__global__ void __launch_bounds__(kBT)
+pairs_kernel(const unsigned long long *set_off, const uint32_t *set_len, const uint32_t *data,
+             uint64_t num_sets, const unsigned long long *soff, uint32_t *keys, uint32_t *vals) {
+  __shared__ uint32_t s_pre[kBT / 64][65];
+  __shared__ unsigned long long s_off[kBT / 64][64];
+  const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBT / 64);
+  for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * (kBT / 64) + wid; w * 64 < num_sets;
+       w += nwaves) {
+    const uint64_t s = w * 64 + lane;
+    const uint32_t len = s < num_sets ? set_len[s] : 0u;
+    s_off[wid][lane] = s < num_sets ? set_off[s] : 0ull;
+    const uint32_t incl = wave_inclusive_scan(len);
+    s_pre[wid][lane + 1] = incl;
+    if (lane == 0) s_pre[wid][0] = 0;
+    const uint32_t total = __shfl(incl, 63, 64);
+    const unsigned long long base = soff[w * 64];
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < total; i += 64) {
+      uint32_t lo = 0, hi = 64;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[wid][mid + 1] <= i) lo = mid + 1;
+        else hi = mid;
+      }
+      keys[base + i] = data[s_off[wid][lo] + (i - s_pre[wid][lo])];
+      vals[base + i] = static_cast<uint32_t>(w * 64 + lo);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// inv_start[v] = the first of the `count` sorted pairs whose vertex is >= v,
+// for v in [0, n]: pair i starts the vertices after its predecessor's up to
+// its own (i = count: up to n).
+__global__ void starts_kernel(const uint32_t *keys, uint64_t count, uint32_t n,
+                              unsigned long long *inv_start) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i <= count;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t hi = i < count ? keys[i] : n;
+    const uint64_t lo = i == 0 ? 0ull : static_cast<uint64_t>(keys[i - 1]) + 1;
+    for (uint64_t v = lo; v <= hi; ++v) inv_start[v] = i;
+  }
+}
+
+__global__ void counts_from_starts_kernel(const unsigned long long *inv_start, uint32_t n,
+                                          uint32_t *counts) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    counts[v] = static_cast<uint32_t>(inv_start[v + 1] - inv_start[v]);
+}
+
 // ------------------------------------------------------------ dense pools
 // Pools whose sets are long compared with the number of vertex ranges below
 // build no inverted index.  Counts come from LDS histograms: block (set chunk
@@ -642,8 +703,10 @@ void ensure_select_scratch(rip_pool *p, uint64_t num_sets, uint64_t inv_elems,
   }
   if (s.cap_sets < num_sets) {
     dev_free(s.d_covered);
+    dev_free(s.d_soff);
     s.cap_sets = std::max<uint64_t>(num_sets, s.cap_sets + s.cap_sets / 2);
     s.d_covered = dev_alloc<uint8_t>(s.cap_sets);
+    s.d_soff = dev_alloc<unsigned long long>(s.cap_sets + 1);
   }
   if (s.cap_elems < inv_elems) {
     dev_free(s.d_inv);
@@ -700,9 +763,46 @@ uint32_t dense_chunks(uint32_t ranges, uint64_t sets) {
 
 enum class Need { Counts, Select };
 
+// The inverted index of sparse sets [0, num_sets) (total elements) by sorting
+// their (vertex, set) pairs: counts, inv_start and the index itself, in
+// d_inv or d_pval; returns where the index is.
+const uint32_t *inverted_by_sort(rip_pool *p, uint64_t num_sets, uint64_t total) {
+  SelectScratch &s = p->sel;
+  hipStream_t st = p->stream;
+  const uint32_t n = p->g->n;
+  if (s.cap_pairs < total) {
+    dev_free(s.d_pkey[0]);
+    dev_free(s.d_pkey[1]);
+    dev_free(s.d_pval);
+    s.cap_pairs = 0;
+    const uint64_t cap = std::max<uint64_t>(total, s.cap_pairs + s.cap_pairs / 2);
+    s.d_pkey[0] = dev_alloc<uint32_t>(cap);
+    s.d_pkey[1] = dev_alloc<uint32_t>(cap);
+    s.d_pval = dev_alloc<uint32_t>(cap);
+    s.cap_pairs = cap;
+  }
+  exclusive_scan_u32(p->d_len, s.d_soff, num_sets, st);
+  hipLaunchKernelGGL(pairs_kernel, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)), dim3(kBT), 0,
+                     st, p->d_off, p->d_len, p->d_data, num_sets, s.d_soff, s.d_pkey[0], s.d_inv);
+  RIP_HIP(hipGetLastError());
+  uint32_t bits = 1;
+  while (bits < 32 && (static_cast<uint64_t>(n) - 1) >> bits) ++bits;
+  const int at = sort_pairs_u32(s.d_pkey[0], s.d_pkey[1], s.d_inv, s.d_pval, total, bits,
+                                &s.d_stemp, &s.stemp_bytes, st);
+  hipLaunchKernelGGL(starts_kernel, dim3(grid_for(total + 1, kBT, 4096)), dim3(kBT), 0, st,
+                     s.d_pkey[at], total, n, s.d_inv_start);
+  hipLaunchKernelGGL(counts_from_starts_kernel, dim3(grid_for(n, kBT, 2048)), dim3(kBT), 0, st,
+                     s.d_inv_start, n, s.d_counts);
+  RIP_HIP(hipGetLastError());
+  return at == 0 ? s.d_inv : s.d_pval;
+}
+
 // counts over sets [0, num_sets) and the scratch a selection needs; returns
-// whether the pool takes the dense path
-bool build_counts(rip_pool *p, uint64_t num_sets, uint32_t k, Need need) {
+// whether the pool takes the dense path.  *sorted_inv (selections only): the
+// inverted index when it was built here by sorting, else nullptr.
+bool build_counts(rip_pool *p, uint64_t num_sets, uint32_t k, Need need,
+                  const uint32_t **sorted_inv = nullptr) {
+  if (sorted_inv) *sorted_inv = nullptr;
   RIP_REQUIRE(num_sets <= p->num_sets, -1, "select: num_sets beyond the pool");
   const uint32_t n = p->g->n;
   hipStream_t st = p->stream;
@@ -737,9 +837,13 @@ bool build_counts(rip_pool *p, uint64_t num_sets, uint32_t k, Need need) {
                          p->g->hot_n, s.d_counts);
     }
   } else if (num_sets) {
-    hipLaunchKernelGGL(count_kernel, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
-                       dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets,
-                       s.d_counts);
+    if (sel && sorted_inv && total <= kSortMaxPairs) {
+      *sorted_inv = inverted_by_sort(p, num_sets, total);
+    } else {
+      hipLaunchKernelGGL(count_kernel, dim3(grid_for(num_sets, 64 * (kBT / 64), 8192)),
+                         dim3(kBT), 0, st, p->d_off, p->d_len, p->d_data, num_sets,
+                         s.d_counts);
+    }
   }
   RIP_HIP(hipGetLastError());
   return dense;
@@ -802,9 +906,11 @@ uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
                     uint64_t *covered) {
   const uint32_t n = p->g->n;
   hipStream_t st = p->stream;
-  const bool dense = build_counts(p, num_sets, std::max<uint32_t>(k, 1), Need::Select);
+  const uint32_t *sorted_inv = nullptr;
+  const bool dense = build_counts(p, num_sets, std::max<uint32_t>(k, 1), Need::Select,
+                                  &sorted_inv);
   SelectScratch &s = p->sel;
-  if (!dense) {
+  if (!dense && !sorted_inv) {
     exclusive_scan_u32(s.d_counts, s.d_inv_start, n, st);
     hipLaunchKernelGGL(copy_u64_kernel, dim3(grid_for(n, kBT, 2048)), dim3(kBT), 0, st,
                        s.d_cursor, s.d_inv_start, static_cast<uint64_t>(n));
@@ -818,7 +924,8 @@ uint32_t select_run(rip_pool *p, uint64_t num_sets, uint32_t k, uint32_t *seeds,
   // the seed rounds run as one graph; this call's buffers travel in d_args
   // (`args` stays alive until the stream is synchronised below)
   const bool cols = p->groups > 0;
-  const Args args{s.d_counts, s.d_inv_start, s.d_inv, p->d_off, p->d_len, p->d_data,
+  const Args args{s.d_counts, s.d_inv_start, sorted_inv ? sorted_inv : s.d_inv, p->d_off,
+                  p->d_len, p->d_data,
                   s.d_covered, s.d_keys, p->d_ctl, s.d_kill, s.d_partial, num_sets, n,
                   p->d_gref, p->d_colptr, p->g->d_hot_index, p->g->d_hot_vertex, s.d_gkill,
                   s.d_kgroups, cols ? p->groups : 0u, p->g->hot_n};
