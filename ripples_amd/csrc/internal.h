@@ -191,6 +191,10 @@ struct rip_graph {
   // calls that pass none.
   rip_tune tuned{};
   bool tuned_set = false;
+  // Mean set size (pool elements per set) of the last sampling call of at
+  // least kSizeHintMin sets by any pool of this graph, 0 before one: a new
+  // pool sizes its first call's data with it.  Guarded by sampler_mu.
+  double set_size_hint = 0.0;
   // Sampler scratch shared by every pool of this graph; pool_sample holds
   // sampler_mu while a launch uses it.
   std::mutex sampler_mu;

@@ -462,9 +462,21 @@ void pool_read_ctl(rip_pool *p, bool main) {
 
 // Samples of the first call of a pool drawn apart to size its data.
 constexpr uint64_t kSizingPrefix = 16384;
+// Calls of at least this many samples set the graph's set-size hint.
+constexpr uint64_t kSizeHintMin = 1024;
 
 void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
   if (count == 0) return;
+  if (!p->sized) {
+    // a pool's first call: the set size another pool of this graph measured
+    // (an earlier query's, the autotuner's probes) sizes its data, where the
+    // default guess would have the call overflow and be drawn again
+    std::lock_guard<std::mutex> hold(p->g->sampler_mu);
+    if (p->g->set_size_hint > 0.0) {
+      p->avg_set_size = p->g->set_size_hint;
+      p->sized = true;
+    }
+  }
   if (!p->sized && count > 4 * kSizingPrefix) {
     // with no set size measured yet the data capacity is a guess, and a call
     // that overflows it is drawn again: a prefix of the call measures the
@@ -547,6 +559,7 @@ void pool_sample(rip_pool *p, uint64_t count, const rip_tune *tune) {
       p->groups += static_cast<uint32_t>(ctl[kCtlGroupCounter2]);
       p->avg_set_size = std::max(1.0, static_cast<double>(added) / static_cast<double>(count));
       p->sized = true;
+      if (count >= kSizeHintMin) p->g->set_size_hint = p->avg_set_size;
       p->num_sets += count;
       return;
     }
