@@ -90,6 +90,7 @@ struct GridParams {
                                    // ring[kRing + L % kRing] chunk counter
   unsigned long long *buf[2];      // level L reads buf[L % 2], writes buf[(L + 1) % 2]
   unsigned long long *ctl;         // the pool's control words (work counters)
+  uint32_t chunk;                  // frontier items a level kernel's workgroup takes at a time
 };
 
 // The j-th set bit of m (j < popcount(m)).
@@ -571,9 +572,9 @@ __device__ void expand_chunk(const GridParams &P, const unsigned long long *cur,
 
 // BFS level L: the frontier is buf[L % 2][0, n) with n = ring[L % kRing]
 // (its first cap entries when it overflowed, which raises the overflow
-// flag); every workgroup with a chunk to take pulls chunks of kExpItems items from
-// the level's counter.  Levels are enqueued ahead of the host: a level whose
-// frontier is empty costs its launch only.
+// flag); every workgroup with a chunk to take pulls chunks of P.chunk items
+// (at most kExpItems) from the level's counter.  Levels are enqueued ahead of
+// the host: a level whose frontier is empty costs its launch only.
 template <int BT>
 __global__ void __launch_bounds__(BT) grid_level_kernel(GridParams P, uint32_t L) {
   __shared__ ExpandLds<BT> S;
@@ -586,7 +587,8 @@ __global__ void __launch_bounds__(BT) grid_level_kernel(GridParams P, uint32_t L
     P.gctl[kGLevels] += 1;
     if (raw > P.cap) P.gctl[kGOver] = 1;
   }
-  if (static_cast<uint64_t>(blockIdx.x) * kExpItems >= ncur) return;   // no chunk for it
+  const uint32_t chunk = P.chunk;
+  if (static_cast<uint64_t>(blockIdx.x) * chunk >= ncur) return;   // no chunk for it
   GridParams Q = P;
   Q.cur = P.buf[L & 1u];
   Q.ncur = ncur;
@@ -599,13 +601,13 @@ __global__ void __launch_bounds__(BT) grid_level_kernel(GridParams P, uint32_t L
   ExpandStats st;
   for (;;) {
     __syncthreads();
-    if (tid == 0) s_chunk = atomicAdd(head, static_cast<unsigned long long>(kExpItems));
+    if (tid == 0) s_chunk = atomicAdd(head, static_cast<unsigned long long>(chunk));
     __syncthreads();
     const uint64_t c0 = s_chunk;
     if (c0 >= ncur) break;
     expand_chunk<BT>(Q, Q.cur, c0,
-                     static_cast<uint32_t>(min(static_cast<uint64_t>(kExpItems), ncur - c0)),
-                     S, st);
+                     static_cast<uint32_t>(min(static_cast<uint64_t>(chunk), ncur - c0)), S,
+                     st);
   }
   report_stats(P.ctl, st.bytes, st.cand, st.draws, st.exp, st.live);
 }
@@ -1171,6 +1173,13 @@ uint32_t ic_grid_launch(rip_pool *p, uint64_t first, uint64_t count, const uint3
   uint32_t launches = 0;
   for (uint64_t b0 = 0; b0 < count; b0 += static_cast<uint64_t>(C) * G) {
     const uint32_t Gb = static_cast<uint32_t>(std::min<uint64_t>(G, (count - b0 + C - 1) / C));
+    // small batches on small graphs take their levels in chunks of 32 items:
+    // a level's hub rows, discovered together, sit in few chunks, and with
+    // chunks of 128 the workgroups holding them set the level's time.  On
+    // R-MAT scale 16 IC (n Gb of 4M-25M), IMM's sampling calls of 4K-8K sets
+    // took 10-11.6 ms with 128 and 6.3-8.6 ms with 32; with 32 on every batch,
+    // scale 18 (100,000 sets) and scale 24 (20,000 sets) ran 4-5% slower.
+    P.chunk = static_cast<uint64_t>(n) * Gb <= (1ull << 25) ? kExpItems / 4 : kExpItems;
     hipLaunchKernelGGL(grid_samples_kernel, dim3((64 * Gb + 255) / 256), dim3(256), 0, st,
                        p->seed, first, ids, b0, count, Gb, C, s.d_z, s.d_sid, s.d_nb);
     hipLaunchKernelGGL(grid_ring_kernel, dim3(1), dim3(2 * kRing), 0, st, s.d_ring, s.d_gctl, 0u,
