@@ -775,11 +775,11 @@ const uint32_t *inverted_by_sort(rip_pool *p, uint64_t num_sets, uint64_t total)
   hipStream_t st = p->stream;
   const uint32_t n = p->g->n;
   if (s.cap_pairs < total) {
+    const uint64_t cap = std::max<uint64_t>(total, s.cap_pairs + s.cap_pairs / 2);
     dev_free(s.d_pkey[0]);
     dev_free(s.d_pkey[1]);
     dev_free(s.d_pval);
     s.cap_pairs = 0;
-    const uint64_t cap = std::max<uint64_t>(total, s.cap_pairs + s.cap_pairs / 2);
     s.d_pkey[0] = dev_alloc<uint32_t>(cap);
     s.d_pkey[1] = dev_alloc<uint32_t>(cap);
     s.d_pval = dev_alloc<uint32_t>(cap);
