@@ -207,7 +207,8 @@ __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
 }
 
 // Expands up to 64 queued vertices of the sample whose element 0 is z at
-// once, lane l taking vertex v (when `have`): the lane walks its row's gap
+// once, lane l taking the vertex whose record is rv (when `have`): the lane
+// walks its row's gap
 // blocks one draw per step, as a lane of the lane sampler does; the live
 // edges of a step go through visit() and the first visits are appended to
 // list[tail ...].  Every lane of the wave must call it.  Returns false when
@@ -215,8 +216,8 @@ __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
 // edges (weighted cascade: one candidate per row) keep 64 lanes busy here,
 // where expand_row would give each row a whole wave.
 template <typename Visit>
-__device__ inline bool expand_rows_lanes(const VertexRec *vrec, const uint32_t *ids,
-                                         const uint32_t *val, bool have, uint32_t v, uint64_t z,
+__device__ inline bool expand_rows_lanes(const uint32_t *ids, const uint32_t *val, bool have,
+                                         const VertexRec &rv, uint64_t z,
                                          const Affine *p2, const IcLogTables *tab,
                                          uint32_t *list, uint32_t &tail, uint32_t cap,
                                          unsigned long long &st_cand,
@@ -227,7 +228,6 @@ __device__ inline bool expand_rows_lanes(const VertexRec *vrec, const uint32_t *
   bool uniform = true;
   uint64_t q = 0, rq = 0, ufull = 0, ulast = 0, xb = 0, x = 0;
   if (have) {
-    const VertexRec rv = vrec[v];
     T = rv.thr;
     have = T != 0;                               // no edge of the row can be live
     if (have) {
@@ -354,6 +354,10 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
     }
     uint32_t head = 0, tail = 1;
     bool big = false;
+    // vertex-parallel rounds: the records of the next round's first npre
+    // vertices, loaded with the current round's where the queue held them
+    uint32_t npre = 0;
+    VertexRec prec{};
     // a visit call adds at most 64 vertices: while the sample stays within
     // the LDS limit they go to the LDS table, then to the HBM table (after a
     // look in the LDS one)
@@ -367,12 +371,25 @@ __global__ void __launch_bounds__(BT) ic_wave_kernel(WaveParams P) {
       if (VP) {
         const uint32_t take = min(tail - head, 64u);
         const bool have = lane < take;
-        const uint32_t v = have ? list[head + lane] : 0u;
-        head += take;
+        const uint32_t nhead = head + take;
+        // the next round's vertices already queued: their records load with
+        // this round's, off the next round's chain of dependent loads
+        const uint32_t nnext = min(tail - nhead, 64u);
+        const bool now = have && lane >= npre;
+        const bool next = lane < nnext;
+        const uint32_t v = now ? list[head + lane] : 0u;
+        const uint32_t vn = next ? list[nhead + lane] : 0u;
+        VertexRec rec = prec;
+        if (now) rec = P.vrec[v];
+        VertexRec recn{};
+        if (next) recn = P.vrec[vn];
+        head = nhead;
         st_exp += take;
-        if (!expand_rows_lanes(P.vrec, P.ids, P.val, have, v, z, s_p2, &s_tab, list, tail, P.cap,
+        if (!expand_rows_lanes(P.ids, P.val, have, rec, z, s_p2, &s_tab, list, tail, P.cap,
                                st_cand, st_draws, st_bytes, st_live, visit_vp))
           big = true;
+        prec = recn;
+        npre = nnext;
       } else {
         const uint32_t v = uniform_u32(list[head++]);
         ++st_exp;
