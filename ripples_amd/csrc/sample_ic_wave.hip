@@ -208,13 +208,12 @@ __device__ inline bool expand_row(const VertexRec *vrec, const uint32_t *ids,
 
 // Expands up to 64 queued vertices of the sample whose element 0 is z at
 // once, lane l taking the vertex whose record is rv (when `have`): the lane
-// walks its row's gap
-// blocks one draw per step, as a lane of the lane sampler does; the live
-// edges of a step go through visit() and the first visits are appended to
-// list[tail ...].  Every lane of the wave must call it.  Returns false when
-// the live edges of a step could take the list past cap.  Rows of a few
-// edges (weighted cascade: one candidate per row) keep 64 lanes busy here,
-// where expand_row would give each row a whole wave.
+// walks its row's gap blocks one draw per step, as a lane of the lane
+// sampler does; the live edges of a step go through visit() and the first
+// visits are appended to list[tail ...].  Every lane of the wave must call
+// it.  Returns false when the live edges of a step could take the list past
+// cap.  Rows of a few edges (weighted cascade: one candidate per row) keep
+// 64 lanes busy here, where expand_row would give each row a whole wave.
 template <typename Visit>
 __device__ inline bool expand_rows_lanes(const uint32_t *ids, const uint32_t *val, bool have,
                                          const VertexRec &rv, uint64_t z,
